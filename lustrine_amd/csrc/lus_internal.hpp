@@ -1,0 +1,94 @@
+// lus_internal.hpp -- types shared by the HIP kernels (lus_kernels.hip) and the
+// host side of the C ABI (lus_capi.cpp).  Not installed; the public surface is
+// include/lustrine_gpu.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lustrine_gpu.h"
+
+namespace lus {
+
+// Derived constants of the PBF substep, all float32, computed once on the host
+// (pbf_constants in lus_capi.cpp).  The kernels evaluate
+//   W(r)  = P6 * t^3,          t = hf2 - f2 * r2            (r2 <= h2)
+//   dW(r) = -s * r_vec,        s = SPf * g * g / r, g = hf - f * r
+// and keep the accumulated t^3 unnormalised (c_rho = m * P6 / rho0).
+struct PbfK {
+    float h2, hf, hf2, f, f2, SPf, c_rho, mr0sq, eps, inv_tq3, scorr_k, neg_inv_r0, cW, cs;
+    float lo[3], hi[3], g[3];
+    int gX, gY, gZ, gXZ, n_scorr;
+};
+
+// simulate_sand constants (src/Simulate.cpp:159-163 and the Simulation fields).
+struct SandK {
+    float r, d, h2, w, cs;
+    float mass_n, mass_sum;   // neighbor_mass and mass + neighbor_mass
+    float hi[3], g[3], player[3];
+    float collision, friction, mu_s, mu_k;
+    float attract_radius, blow_radius, attract_coeff, blow_coeff;
+    int gX, gY, gZ, gXZ;
+    int attract, blow, prev_attract;
+    int sym_limit;            // slots below it receive symmetric pushes
+};
+
+// ---- launchers (lus_kernels.hip); all enqueue on `s` and return hipError_t ----
+// prefix scan of m ints: out[k] = sum in[0..k); zero_in clears `in` after use.
+// tile_sums needs scan_tile_count(m) + 1 ints.
+int scan_tile_count(int m);
+hipError_t launch_exclusive_scan(const int* in, int* out, int m, int* tile_sums, int zero_in,
+                                 hipStream_t s);
+
+hipError_t launch_cell_keys(const float4* pos, int n, LusGrid g, int* key, hipStream_t s);
+hipError_t launch_count_keys(const int* key, int n, int* counts, int* slot, hipStream_t s);
+hipError_t launch_scatter(const int* key, const int* slot, const int* cell_start, int n, int* tmp,
+                          hipStream_t s);
+// generic stable rank: perm[o] = i for the stable order; key_sorted[o] = key[i]
+hipError_t launch_stable_rank(const int* tmp, const int* key, const int* cell_start, int n,
+                              int* perm, int* key_sorted, hipStream_t s);
+hipError_t launch_gather4(const float4* src, const int* perm, int n, float4* dst, hipStream_t s);
+hipError_t launch_gather1(const int* src, const int* perm, int n, int* dst, hipStream_t s);
+
+// fluid-path neighbor lists (parity entry point)
+hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, const int* sand_cs,
+                            const int* sand_orig, int n_sand, int sand_start,
+                            const float4* solid_sorted, const int* solid_cs, const int* solid_ref,
+                            LusGrid g, float h2, const int* inv_perm, const long* off, int* cnt,
+                            int* idx, int fill, hipStream_t s);
+
+// PBF substep
+hipError_t launch_pbf_key(const float4* x, const float4* v, int n, PbfK k, float dt, int* key,
+                          int* slot, int* counts, int* maxc, int nmaxc, hipStream_t s);
+hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_start,
+                              const float4* x, const float4* v, const int* id, int n, PbfK k,
+                              float dt, float4* x2, float4* xa, int* id2, int* key2,
+                              hipStream_t s);
+hipError_t launch_pbf_lambda(const float4* xa, const int* key2, const int* cs, int n, PbfK k,
+                             float4* xl, int* maxc, hipStream_t s);
+hipError_t launch_pbf_deltap(const float4* xl, const int* key2, const int* cs, int n, PbfK k,
+                             float4* xa, const float4* xold, float4* vout, float dt,
+                             hipStream_t s);
+hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, float dt,
+                               float4* vout, hipStream_t s);
+hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, const int* key2, const int* cs,
+                           int n, PbfK k, float4* vout, float4* xout, hipStream_t s);
+
+// sand (simulate_sand)
+hipError_t launch_sand_predict_key(float4* x, float4* v, int* att, int n, SandK k, float dt,
+                                   float4* xs, int* key, int* slot, int* counts, hipStream_t s);
+hipError_t launch_sand_permute(const int* tmp, const int* key, const int* cell_start,
+                               const float4* x, const float4* xs, const float4* v, const int* att,
+                               int n, float4* x2, float4* xs2, float4* v2, int* att2, int* perm,
+                               int* key2, hipStream_t s);
+hipError_t launch_sand_nbr(const float4* xs, const int* key2, const int* sand_cs, int n,
+                           const float4* solid, const int* solid_cs, SandK k, const int* off,
+                           int* cnt, int* cnt1, int* idx, int fill, hipStream_t s);
+hipError_t launch_sand_substep(const float4* A, const float4* xold, const float4* solid,
+                               const int* off, const int* idx, int n, SandK k, float4* B,
+                               hipStream_t s);
+hipError_t launch_sand_vupdate(const float4* A, float4* xold_to_new, float4* vout, int n, float dt,
+                               hipStream_t s);
+
+}  // namespace lus

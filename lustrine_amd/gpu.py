@@ -1,0 +1,323 @@
+"""ctypes binding of liblustrine_gpu.so (include/lustrine_gpu.h).
+
+The library is loaded from this package directory; if it is missing the import
+fails with an error naming the build step.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblustrine_gpu.so")
+
+NPHASES = 7
+PHASES = ("predict_key", "cell_scan", "scatter", "stable_permute", "lambda", "deltap", "xsph")
+
+
+class LusError(RuntimeError):
+    pass
+
+
+class LusGrid(C.Structure):
+    _fields_ = [("cell_size", C.c_float), ("gridX", C.c_int), ("gridY", C.c_int), ("gridZ", C.c_int)]
+
+
+class LusPbfParams(C.Structure):
+    _fields_ = [
+        ("h", C.c_float), ("f", C.c_float), ("cell_size", C.c_float),
+        ("gridX", C.c_int), ("gridY", C.c_int), ("gridZ", C.c_int),
+        ("lo", C.c_float * 3), ("hi", C.c_float * 3), ("gravity", C.c_float * 3),
+        ("mass", C.c_float), ("rest_density", C.c_float), ("epsilon", C.c_float),
+        ("scorr_k", C.c_float), ("scorr_dq", C.c_float), ("scorr_n", C.c_int),
+        ("c_xsph", C.c_float), ("iterations", C.c_int),
+    ]
+
+
+class LusSandParams(C.Structure):
+    _fields_ = [
+        ("particleRadius", C.c_float), ("particleDiameter", C.c_float),
+        ("kernelRadius", C.c_float), ("mass", C.c_float),
+        ("domain", C.c_float * 3), ("grid", LusGrid), ("gravity", C.c_float * 3),
+        ("collision_coeff", C.c_float), ("friction_coeff", C.c_float),
+        ("mu_s", C.c_float), ("mu_k", C.c_float), ("substeps", C.c_int),
+        ("attract_radius", C.c_float), ("blow_radius", C.c_float),
+        ("attract_coeff", C.c_float), ("blow_coeff", C.c_float),
+        ("ptr_solid_start", C.c_int),
+    ]
+
+
+_P = C.POINTER
+_f = _P(C.c_float)
+_i = _P(C.c_int)
+_l = _P(C.c_long)
+
+_SIGS = {
+    "lus_gpu_last_error": (C.c_int, [C.c_char_p, C.c_int]),
+    "lus_gpu_device_count": (C.c_int, []),
+    "lus_cell_ids": (C.c_int, [_P(LusGrid), C.c_int, _f, _i]),
+    "lus_counting_sort": (C.c_int, [_i, _i, C.c_size_t, C.c_size_t, _i]),
+    "lus_neighbors_fluid": (C.c_int, [_P(LusGrid), C.c_float, C.c_int, _f, C.c_int, C.c_int, _f, _i,
+                                      _l, _i, C.c_long, _l]),
+    "lus_gpu_create": (C.c_int, [_P(C.c_void_p), C.c_int]),
+    "lus_gpu_destroy": (None, [C.c_void_p]),
+    "lus_gpu_sync": (C.c_int, [C.c_void_p]),
+    "lus_pbf_init": (C.c_int, [C.c_void_p, _P(LusPbfParams), C.c_int, _f, _f]),
+    "lus_pbf_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
+    "lus_pbf_download": (C.c_int, [C.c_void_p, C.c_int, _f, _f, _i]),
+    "lus_pbf_last_max_constraint": (C.c_int, [C.c_void_p, _f]),
+    "lus_pbf_bench": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, _f, _f, _i]),
+    "lus_sand_init": (C.c_int, [C.c_void_p, _P(LusSandParams), C.c_int, _f, _f, _i, C.c_int, _f, _i]),
+    "lus_sand_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, _f]),
+    "lus_sand_download": (C.c_int, [C.c_void_p, _f, _f, _f, _i, _i]),
+    "lus_sand_neighbors": (C.c_int, [C.c_void_p, _l, _i, C.c_long, _l]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LusError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                           " or `python lustrine_amd/build.py`")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(512)
+    load().lus_gpu_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise LusError(f"{what} failed (code {rc}): {last_error()}")
+
+
+def device_count() -> int:
+    return load().lus_gpu_device_count()
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def _ptr_f(a):
+    return None if a is None else a.ctypes.data_as(_f)
+
+
+def _ptr_i(a):
+    return None if a is None else a.ctypes.data_as(_i)
+
+
+def _ptr_l(a):
+    return None if a is None else a.ctypes.data_as(_l)
+
+
+def grid(cell_size, gx, gy, gz) -> LusGrid:
+    return LusGrid(cell_size, gx, gy, gz)
+
+
+# ------------------------------------------------------------------ building blocks
+def cell_ids(g: LusGrid, pos) -> np.ndarray:
+    pos = _f32(pos, (-1, 3))
+    out = np.zeros(pos.shape[0], np.int32)
+    _check(load().lus_cell_ids(C.byref(g), pos.shape[0], _ptr_f(pos), _ptr_i(out)), "lus_cell_ids")
+    return out
+
+
+def counting_sort(keys, num_cells):
+    keys = np.ascontiguousarray(keys, np.int32)
+    counts = np.zeros(num_cells + 1, np.int32)
+    out = np.zeros(keys.shape[0], np.int32)
+    _check(load().lus_counting_sort(_ptr_i(counts), _ptr_i(keys), keys.shape[0], num_cells, _ptr_i(out)),
+           "lus_counting_sort")
+    return out, counts
+
+
+def neighbors_fluid(g: LusGrid, kernel_radius, sand_pos, sand_start=0, solid_pos=None, solid_index=None):
+    sand_pos = _f32(sand_pos, (-1, 3))
+    ns = 0 if solid_pos is None else int(np.asarray(solid_pos).shape[0])
+    solid_pos = _f32(solid_pos, (-1, 3)) if ns else None
+    solid_index = np.ascontiguousarray(solid_index, np.int32) if ns else None
+    n = sand_pos.shape[0]
+    off = np.zeros(n + 1, np.int64)
+    total = C.c_long(0)
+    cap = 0
+    lib = load()
+    rc = lib.lus_neighbors_fluid(C.byref(g), kernel_radius, n, _ptr_f(sand_pos), sand_start, ns,
+                                 _ptr_f(solid_pos), _ptr_i(solid_index), _ptr_l(off), None, cap,
+                                 C.byref(total))
+    if rc != 0 and total.value <= cap:
+        _check(rc, "lus_neighbors_fluid")
+    idx = np.zeros(max(total.value, 1), np.int32)
+    _check(lib.lus_neighbors_fluid(C.byref(g), kernel_radius, n, _ptr_f(sand_pos), sand_start, ns,
+                                   _ptr_f(solid_pos), _ptr_i(solid_index), _ptr_l(off), _ptr_i(idx),
+                                   idx.shape[0], C.byref(total)), "lus_neighbors_fluid")
+    return off, idx[: total.value]
+
+
+# ------------------------------------------------------------------ params
+def pbf_params(cfg, iterations=None) -> LusPbfParams:
+    """LusPbfParams of a scenes.PbfConfig."""
+    p = LusPbfParams()
+    r = np.float32(cfg.radius)
+    p.h = cfg.h
+    p.f = cfg.kernel_factor
+    p.cell_size = cfg.cell_size
+    p.gridX, p.gridY, p.gridZ = cfg.grid
+    for a in range(3):
+        p.lo[a] = r
+        p.hi[a] = np.float32(cfg.box[a]) - r
+        p.gravity[a] = cfg.gravity[a]
+    p.mass = cfg.mass
+    p.rest_density = cfg.rest_density()
+    p.epsilon = cfg.epsilon
+    p.scorr_k = cfg.scorr_k
+    p.scorr_dq = cfg.scorr_dq
+    p.scorr_n = cfg.scorr_n
+    p.c_xsph = cfg.c_xsph
+    p.iterations = cfg.iterations if iterations is None else iterations
+    return p
+
+
+def sand_params(X, Y, Z, radius=0.5, diameter=1.0, kernel_radius_scale=3.1, substeps=4,
+                gravity=(0.0, -10.0, 0.0), mass=5.0, collision=0.8, friction=0.7, mu_s=0.95,
+                mu_k=0.9) -> LusSandParams:
+    """simulate_sand constants of a scene built like init_simulation_extra_parameters
+    (src/Lustrine.cpp:383-386, 543-558); coefficients of src/Simulate.cpp:159-163."""
+    p = LusSandParams()
+    p.particleRadius = radius
+    p.particleDiameter = diameter
+    h = np.float32(kernel_radius_scale) * np.float32(radius)
+    p.kernelRadius = h
+    p.mass = mass
+    p.domain[:] = [float(X), float(Y), float(Z)]
+    cs = np.float32(1.0) * h
+    p.grid = LusGrid(cs, int(np.float32(X) / cs) + 1, int(np.float32(Y) / cs) + 1, int(np.float32(Z) / cs) + 1)
+    p.gravity[:] = list(gravity)
+    p.collision_coeff, p.friction_coeff, p.mu_s, p.mu_k = collision, friction, mu_s, mu_k
+    p.substeps = substeps
+    p.attract_radius, p.blow_radius, p.attract_coeff, p.blow_coeff = 1.5, 2.0, 1000.0, 500.0
+    p.ptr_solid_start = 1 << 30
+    return p
+
+
+# ------------------------------------------------------------------ context
+class Gpu:
+    """One device context (one HIP stream) holding a PBF and/or a sand scene."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        h = C.c_void_p()
+        _check(self._lib.lus_gpu_create(C.byref(h), device), "lus_gpu_create")
+        self._h = h
+        self.n = 0
+        self.n_sand = 0
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.lus_gpu_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(self._lib.lus_gpu_sync(self._h), "lus_gpu_sync")
+
+    # --- PBF
+    def pbf_init(self, params: LusPbfParams, pos, vel=None):
+        pos = _f32(pos, (-1, 3))
+        vel = None if vel is None else _f32(vel, (-1, 3))
+        _check(self._lib.lus_pbf_init(self._h, C.byref(params), pos.shape[0], _ptr_f(pos), _ptr_f(vel)),
+               "lus_pbf_init")
+        self.n = pos.shape[0]
+
+    def pbf_step(self, dt, steps=1):
+        _check(self._lib.lus_pbf_step(self._h, dt, steps), "lus_pbf_step")
+
+    def pbf_download(self, order=0):
+        pos = np.zeros((self.n, 3), np.float32)
+        vel = np.zeros((self.n, 3), np.float32)
+        ids = np.zeros(self.n, np.int32)
+        _check(self._lib.lus_pbf_download(self._h, order, _ptr_f(pos), _ptr_f(vel), _ptr_i(ids)),
+               "lus_pbf_download")
+        return pos, vel, ids
+
+    def pbf_last_max_constraint(self):
+        v = C.c_float(0)
+        _check(self._lib.lus_pbf_last_max_constraint(self._h, C.byref(v)), "lus_pbf_last_max_constraint")
+        return v.value
+
+    def pbf_bench(self, dt, warmup, steps, phases=True):
+        ms = C.c_float(0)
+        ph = np.zeros(NPHASES, np.float32)
+        nl = np.zeros(NPHASES, np.int32)
+        _check(self._lib.lus_pbf_bench(self._h, dt, warmup, steps, C.byref(ms),
+                                       _ptr_f(ph) if phases else None, _ptr_i(nl) if phases else None),
+               "lus_pbf_bench")
+        return ms.value, (dict(zip(PHASES, ph.tolist())) if phases else None), \
+            (dict(zip(PHASES, nl.tolist())) if phases else None)
+
+    # --- sand
+    def sand_init(self, params: LusSandParams, pos, vel=None, attracted=None, solid_pos=None,
+                  solid_index=None):
+        pos = _f32(pos, (-1, 3))
+        vel = None if vel is None else _f32(vel, (-1, 3))
+        att = None if attracted is None else np.ascontiguousarray(attracted, np.int32)
+        ns = 0 if solid_pos is None else int(np.asarray(solid_pos).shape[0])
+        sp = _f32(solid_pos, (-1, 3)) if ns else None
+        si = np.ascontiguousarray(solid_index, np.int32) if ns else None
+        _check(self._lib.lus_sand_init(self._h, C.byref(params), pos.shape[0], _ptr_f(pos), _ptr_f(vel),
+                                       _ptr_i(att), ns, _ptr_f(sp), _ptr_i(si)), "lus_sand_init")
+        self.n_sand = pos.shape[0]
+
+    def sand_step(self, dt, attract=False, blow=False, player=None):
+        pl = None if player is None else _f32(player, (3,))
+        _check(self._lib.lus_sand_step(self._h, dt, int(attract), int(blow), _ptr_f(pl)), "lus_sand_step")
+
+    def sand_download(self):
+        n = self.n_sand
+        pos = np.zeros((n, 3), np.float32)
+        ps = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        att = np.zeros(n, np.int32)
+        perm = np.zeros(n, np.int32)
+        _check(self._lib.lus_sand_download(self._h, _ptr_f(pos), _ptr_f(ps), _ptr_f(vel), _ptr_i(att),
+                                           _ptr_i(perm)), "lus_sand_download")
+        return pos, ps, vel, att, perm
+
+    def sand_neighbors(self):
+        n = self.n_sand
+        off = np.zeros(n + 1, np.int64)
+        total = C.c_long(0)
+        self._lib.lus_sand_neighbors(self._h, _ptr_l(off), None, 0, C.byref(total))
+        idx = np.zeros(max(total.value, 1), np.int32)
+        _check(self._lib.lus_sand_neighbors(self._h, _ptr_l(off), _ptr_i(idx), idx.shape[0], C.byref(total)),
+               "lus_sand_neighbors")
+        return off, idx[: total.value]

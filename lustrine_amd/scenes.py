@@ -1,0 +1,167 @@
+"""Scene construction for the particle substep: jittered lattices, heightfield
+solids and the five benchmark configurations.
+
+Lattice order follows ``init_chunk_from_grid`` (src/Lustrine.cpp:676-722):
+loops x (outer), y, z (inner); site = (x, y, z) * cell + grid.position +
+cell / 2, evaluated in float32 in that order.  Jitter is drawn from
+Philox4x32-10 (Salmon et al., "Parallel random numbers: as easy as 1, 2, 3",
+SC'11) with key = (seed, 0) and counter = (particle index, axis, 0, 0), so any
+implementation reproduces the same draws.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field, asdict
+
+import numpy as np
+
+_M0 = np.uint64(0xD2511F53)
+_M1 = np.uint64(0xCD9E8D57)
+_W0 = np.uint32(0x9E3779B9)
+_W1 = np.uint32(0xBB67AE85)
+_MASK = np.uint64(0xFFFFFFFF)
+
+
+def philox4x32_10(counter: np.ndarray, key) -> np.ndarray:
+    """Philox4x32 with 10 rounds.  counter: (n, 4) uint32; key: 2 uint32."""
+    c = np.asarray(counter, dtype=np.uint32).reshape(-1, 4)
+    c0 = c[:, 0].astype(np.uint64)
+    c1 = c[:, 1].astype(np.uint64)
+    c2 = c[:, 2].astype(np.uint64)
+    c3 = c[:, 3].astype(np.uint64)
+    k0 = np.uint32(key[0])
+    k1 = np.uint32(key[1])
+    for r in range(10):
+        p0 = _M0 * c0
+        p1 = _M1 * c2
+        hi0, lo0 = p0 >> np.uint64(32), p0 & _MASK
+        hi1, lo1 = p1 >> np.uint64(32), p1 & _MASK
+        n0 = hi1 ^ c1 ^ np.uint64(k0)
+        n1 = lo1
+        n2 = hi0 ^ c3 ^ np.uint64(k1)
+        n3 = lo0
+        c0, c1, c2, c3 = n0, n1, n2, n3
+        if r < 9:
+            k0 = np.uint32((int(k0) + int(_W0)) & 0xFFFFFFFF)
+            k1 = np.uint32((int(k1) + int(_W1)) & 0xFFFFFFFF)
+    return np.stack([c0, c1, c2, c3], axis=1).astype(np.uint32)
+
+
+def uniform01(n: int, axis_count: int, seed: int) -> np.ndarray:
+    """u[i, a] in [0, 1) from counter (i, a, 0, 0), 24-bit resolution."""
+    idx = np.repeat(np.arange(n, dtype=np.uint64), axis_count)
+    ax = np.tile(np.arange(axis_count, dtype=np.uint64), n)
+    ctr = np.zeros((n * axis_count, 4), dtype=np.uint32)
+    ctr[:, 0] = (idx & _MASK).astype(np.uint32)
+    ctr[:, 1] = ax.astype(np.uint32)
+    out = philox4x32_10(ctr, (seed & 0xFFFFFFFF, 0))[:, 0]
+    return ((out >> np.uint32(8)).astype(np.float64) * (1.0 / 16777216.0)).reshape(n, axis_count)
+
+
+def lattice(nx: int, ny: int, nz: int, cell: float = 1.0, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Sites of a full nx*ny*nz grid in init_chunk_from_grid order, float32 (n, 3)."""
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ijk = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    c = np.float32(cell)
+    pos = ijk.astype(np.float32) * c
+    pos = pos + np.asarray(origin, dtype=np.float32)
+    pos = pos + np.float32(cell * 0.5)
+    return pos.astype(np.float32)
+
+
+def jittered_lattice(nx, ny, nz, cell=1.0, origin=(0.0, 0.0, 0.0), amplitude=0.005, seed=0):
+    pos = lattice(nx, ny, nz, cell, origin)
+    u = uniform01(pos.shape[0], 3, seed)
+    j = ((2.0 * u - 1.0) * amplitude).astype(np.float32)
+    return (pos + j).astype(np.float32)
+
+
+@dataclass
+class PbfConfig:
+    """All constants of one plain-PBF scene (fluid in a closed box)."""
+    name: str
+    lattice: tuple                 # (nx, ny, nz) particles
+    box: tuple                     # (X, Y, Z) domain in particle diameters
+    origin: tuple = (0.0, 0.0, 0.0)
+    radius: float = 0.5
+    kernel_radius_scale: float = 4.0
+    kernel_factor: float = 0.5
+    mass: float = 5.0              # src/Simulation.hpp:158
+    epsilon: float = 10.0          # relaxation epsilon, src/Simulation.hpp:163
+    scorr_k: float = 1.0           # src/Simulation.hpp:166-168
+    scorr_n: int = 4
+    scorr_dq: float = 0.5
+    c_xsph: float = 0.01
+    gravity: tuple = (0.0, -9.81, 0.0)
+    dt: float = 1.0 / 60.0
+    iterations: int = 4
+    steps: int = 100
+    jitter: float = 0.005          # +-0.01 r
+    seed: int = 0
+
+    @property
+    def h(self) -> float:
+        return float(np.float32(self.kernel_radius_scale) * np.float32(self.radius))
+
+    @property
+    def cell_size(self) -> float:
+        return self.h
+
+    @property
+    def grid(self) -> tuple:
+        cs = np.float32(self.h)
+        return tuple(int(np.float32(d) / cs) + 1 for d in self.box)
+
+    @property
+    def n(self) -> int:
+        return int(np.prod(self.lattice))
+
+    def rest_density(self) -> float:
+        """m * sum_j W(r_ij) at an interior site of the unjittered lattice
+        (spacing 2r), self term once, W of the kernel the solver uses."""
+        h = self.h
+        f = self.kernel_factor
+        hf = h * f
+        p6 = 315.0 / (64.0 * 3.14 * hf ** 9)
+        d = 2.0 * self.radius
+        k = int(math.ceil(h / d)) + 1
+        s = 0.0
+        for a in range(-k, k + 1):
+            for b in range(-k, k + 1):
+                for c in range(-k, k + 1):
+                    r2 = (a * d) ** 2 + (b * d) ** 2 + (c * d) ** 2
+                    if r2 <= h * h:
+                        s += (hf * hf - f * f * r2) ** 3
+        return float(np.float32(self.mass * p6 * s))
+
+    def positions(self) -> np.ndarray:
+        return jittered_lattice(*self.lattice, cell=2.0 * self.radius, origin=self.origin,
+                                amplitude=self.jitter, seed=self.seed)
+
+    def as_dict(self):
+        d = asdict(self)
+        d.update(h=self.h, grid=self.grid, rest_density=self.rest_density(), n=self.n)
+        return d
+
+
+def cfg0(**kw) -> PbfConfig:
+    """Plain PBF dam break, 32,768 particles (32^3) in a 64^3 box, 200 steps."""
+    return PbfConfig(name="pbf_dam_break_32k", lattice=(32, 32, 32), box=(64, 64, 64),
+                     steps=200, **kw)
+
+
+def cfg2(**kw) -> PbfConfig:
+    """Plain PBF dam break at scale: 1,048,576 particles (128x64x128) in a
+    256x128x256 box, 4 iterations, 100 steps."""
+    return PbfConfig(name="pbf_dam_break_1m", lattice=(128, 64, 128), box=(256, 128, 256),
+                     steps=100, **kw)
+
+
+def cfg2_tail(**kw) -> PbfConfig:
+    """4,194,304 particles (256x64x256) in a 512x128x512 box."""
+    return PbfConfig(name="pbf_dam_break_4m", lattice=(256, 64, 256), box=(512, 128, 512),
+                     steps=100, **kw)
+
+
+def small(nx=8, ny=8, nz=8, box=(16, 16, 16), **kw) -> PbfConfig:
+    return PbfConfig(name=f"pbf_small_{nx}x{ny}x{nz}", lattice=(nx, ny, nz), box=box, **kw)

@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares.
+No compute call runs here (no GPU); argument validation paths are exercised."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from lustrine_amd import gpu as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:int|void|long|float)\s+\**\s*(lus_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(G.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 15
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    # and the Python binding declares the same set
+    assert names == set(G.EXPORTED)
+
+
+def test_no_gpu_errors_are_reported_not_faked():
+    if G.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(G.LusError):
+        G.Gpu(0)
+    assert "not available" in G.last_error()
+
+
+def test_argument_validation_without_gpu():
+    # keys outside [0, num_cells) are rejected before any device work
+    with pytest.raises(G.LusError):
+        G.counting_sort(np.array([0, 5, 2], np.int32), 3)
+    with pytest.raises(G.LusError):
+        G.neighbors_fluid(G.grid(2.0, 4, 4, 4), 2.0, np.array([[1.0, 1.0, 100.0]], np.float32))

@@ -1,0 +1,247 @@
+"""GPU path against the CPU oracle on identical inputs, through the C ABI.
+
+Bar: bit-exact for cell ids, the counting-sort permutation and the neighbor
+lists; positions and velocities within 1e-5 relative (max |gpu - cpu| over the
+array divided by max |cpu|) of the FP32 oracle after one step from the same
+state.  The kernels follow the oracle's operation order with no FMA
+contraction and IEEE fp32 divide and sqrt, so the float results are expected to
+be bit-identical as well; the tests report the largest ulp distance.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from lustrine_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+def rel_err(a, b):
+    scale = max(float(np.abs(b).max()), 1e-30)
+    return float(np.abs(a.astype(np.float64) - b.astype(np.float64)).max()) / scale
+
+
+def max_ulp(a, b):
+    ai = np.ascontiguousarray(a, np.float32).view(np.int32).astype(np.int64)
+    bi = np.ascontiguousarray(b, np.float32).view(np.int32).astype(np.int64)
+    ai = np.where(ai < 0, -(ai & 0x7FFFFFFF), ai)
+    bi = np.where(bi < 0, -(bi & 0x7FFFFFFF), bi)
+    return int(np.abs(ai - bi).max()) if ai.size else 0
+
+
+def glibc_rand_keys(num, mod):
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)
+    return np.array([libc.rand() % mod for _ in range(num)], np.int32)
+
+
+# ------------------------------------------------------------------ grid + sort
+def test_cell_ids(gpu_mod, oracle):
+    s = oracle.init_constants(64, 64, 64, 0.5, 1.0, 4.0)
+    g = gpu_mod.grid(s.cell_size, s.gridX, s.gridY, s.gridZ)
+    rng = np.random.default_rng(0)
+    pts = rng.uniform(0.5, 63.5, (100000, 3)).astype(np.float32)
+    k = np.arange(1, 32, dtype=np.float32) * np.float32(2.0)
+    faces = np.stack([k, np.nextafter(k, np.float32(0)), k], axis=1)
+    pts = np.concatenate([pts, faces])
+    assert np.array_equal(gpu_mod.cell_ids(g, pts), oracle.cell_ids(s, pts))
+
+
+def test_counting_sort_reference_unit_test(gpu_mod, oracle):
+    # experiments/unit_tests/main.cpp:46-89
+    keys = glibc_rand_keys(20000, 1987)
+    perm, counts = gpu_mod.counting_sort(keys, 1987)
+    operm, ocounts = oracle.counting_sort(keys, 1987)
+    assert np.array_equal(perm, operm)
+    assert np.array_equal(counts, ocounts)
+    assert np.array_equal(keys[perm], np.sort(keys))
+
+
+@pytest.mark.parametrize("n,c", [(0, 7), (1, 1), (5000, 1), (33, 2), (100000, 4096)])
+def test_counting_sort_edges(gpu_mod, oracle, n, c):
+    keys = np.random.default_rng(n).integers(0, c, n).astype(np.int32)
+    perm, counts = gpu_mod.counting_sort(keys, c)
+    operm, ocounts = oracle.counting_sort(keys, c)
+    assert np.array_equal(perm, operm) and np.array_equal(counts, ocounts)
+
+
+def test_counting_sort_full_size_cells(gpu_mod, oracle):
+    # the 1M-particle scene's predicted cell keys over its 1,081,665 cells
+    cfg = scenes.cfg2()
+    s = oracle.init_constants(*cfg.box, 0.5, 1.0, 4.0)
+    keys = oracle.cell_ids(s, cfg.positions())
+    perm, counts = gpu_mod.counting_sort(keys, s.num_grid_cells)
+    operm, ocounts = oracle.counting_sort(keys, s.num_grid_cells)
+    assert np.array_equal(perm, operm) and np.array_equal(counts, ocounts)
+
+
+# ------------------------------------------------------------------ neighbor lists
+def _scene_with_solids(n_side=10, seed=4, box=20):
+    pos = scenes.jittered_lattice(n_side, n_side, n_side, origin=(2.0, 3.0, 2.0), amplitude=0.25, seed=seed)
+    sol = scenes.lattice(box, 2, box, origin=(0.0, 0.0, 0.0))
+    return pos, sol
+
+
+def test_fluid_neighbor_lists_exact(gpu_mod, oracle):
+    pos, sol = _scene_with_solids()
+    s = oracle.init_constants(20, 20, 20, 0.5, 1.0, 4.0)
+    n, ns = pos.shape[0], sol.shape[0]
+    total = 8000 + 0
+    total = max(total, n + ns)
+    ps = np.zeros((total, 3), np.float32)
+    ps[:n] = pos
+    solid_slots = np.arange(total - ns, total, dtype=np.int32)
+    ps[solid_slots] = sol
+    ooff, oidx = oracle.neighbors_fluid(s, ps, (0, n), (total - ns, total))
+    g = gpu_mod.grid(s.cell_size, s.gridX, s.gridY, s.gridZ)
+    off, idx = gpu_mod.neighbors_fluid(g, s.kernelRadius, pos, 0, sol, solid_slots)
+    assert np.array_equal(off, ooff)
+    assert np.array_equal(idx, oidx)
+
+
+# ------------------------------------------------------------------ PBF substep
+def _pbf_compare(gpu_mod, oracle, cfg, steps=1, K=None):
+    x0 = cfg.positions()
+    v0 = np.zeros_like(x0)
+    p = oracle.pbf_params(cfg, iterations=K)
+    x, v = x0.copy(), v0.copy()
+    ids = np.arange(x.shape[0], dtype=np.int32)
+    for _ in range(steps):
+        omc = oracle.pbf_step(p, x, v, ids, np.float32(cfg.dt))
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_init(gpu_mod.pbf_params(cfg, iterations=K), x0, v0)
+        g.pbf_step(np.float32(cfg.dt), steps)
+        gx, gv, gid = g.pbf_download(order=0)
+        gmc = g.pbf_last_max_constraint()
+    return (gx, gv, gid, gmc), (x, v, ids, omc)
+
+
+@pytest.mark.parametrize("K", [0, 1, 4])
+def test_pbf_one_step_small(gpu_mod, oracle, K):
+    cfg = scenes.small(12, 12, 12, box=(24, 24, 24))
+    (gx, gv, gid, gmc), (x, v, ids, omc) = _pbf_compare(gpu_mod, oracle, cfg, K=K)
+    assert np.array_equal(gid, ids)
+    assert rel_err(gx, x) <= REL and rel_err(gv, v) <= REL
+    print(f"K={K} ulp(x)={max_ulp(gx, x)} ulp(v)={max_ulp(gv, v)}")
+    if K:
+        assert abs(gmc - omc) <= 1e-6 * max(1.0, abs(omc))
+
+
+def test_pbf_one_step_dam_break_32k(gpu_mod, oracle):
+    # the 32,768-particle dam break, one step from the initial state
+    cfg = scenes.cfg0()
+    (gx, gv, gid, gmc), (x, v, ids, omc) = _pbf_compare(gpu_mod, oracle, cfg)
+    assert np.array_equal(gid, ids)
+    assert rel_err(gx, x) <= REL and rel_err(gv, v) <= REL
+    print(f"32k ulp(x)={max_ulp(gx, x)} ulp(v)={max_ulp(gv, v)}")
+
+
+def test_pbf_ten_steps_32k(gpu_mod, oracle):
+    # after 10 steps the two paths still agree (both follow one operation order)
+    cfg = scenes.cfg0()
+    (gx, gv, gid, _), (x, v, ids, _) = _pbf_compare(gpu_mod, oracle, cfg, steps=10)
+    assert np.array_equal(gid, ids)
+    assert rel_err(gx, x) <= REL and rel_err(gv, v) <= 1e-4
+    print(f"32k x10 ulp(x)={max_ulp(gx, x)} ulp(v)={max_ulp(gv, v)}")
+
+
+def test_pbf_fp64_deviation_32k(gpu_mod, oracle):
+    # the FP32 GPU step against the FP64 oracle (reported; bound is loose)
+    cfg = scenes.cfg0()
+    x0 = cfg.positions()
+    p = oracle.pbf_params(cfg)
+    x, v = x0.astype(np.float64), np.zeros((x0.shape[0], 3), np.float64)
+    ids = np.arange(x0.shape[0], dtype=np.int32)
+    oracle.pbf_step(p, x, v, ids, cfg.dt, fp64=True)
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_init(gpu_mod.pbf_params(cfg), x0, None)
+        g.pbf_step(np.float32(cfg.dt), 1)
+        gx, gv, gid = g.pbf_download(order=0)
+    assert np.array_equal(gid, ids)
+    ex, ev = rel_err(gx, x), rel_err(gv, v)
+    print(f"32k fp64 deviation: x {ex:.3e} v {ev:.3e}")
+    assert ex < 1e-5 and ev < 1e-4
+
+
+def test_pbf_full_size_one_step_1m(gpu_mod, oracle):
+    # the 1,048,576-particle scene: one step against the oracle, plus
+    # size-independent properties (ids a permutation, cell order, box)
+    cfg = scenes.cfg2()
+    (gx, gv, gid, _), (x, v, ids, _) = _pbf_compare(gpu_mod, oracle, cfg)
+    assert np.array_equal(np.sort(gid), np.arange(cfg.n))
+    assert np.array_equal(gid, ids)
+    assert rel_err(gx, x) <= REL and rel_err(gv, v) <= REL
+    assert np.isfinite(gx).all() and np.isfinite(gv).all()
+    lo, hi = 0.5, np.array(cfg.box, np.float32) - 0.5
+    assert (gx >= lo).all() and (gx <= hi).all()
+
+
+def test_pbf_original_order_download(gpu_mod):
+    cfg = scenes.small(10, 10, 10, box=(20, 20, 20))
+    x0 = cfg.positions()
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_init(gpu_mod.pbf_params(cfg), x0, None)
+        g.pbf_step(np.float32(cfg.dt), 3)
+        sx, sv, sid = g.pbf_download(order=0)
+        ox, ov, oid = g.pbf_download(order=1)
+    assert np.array_equal(oid, np.arange(cfg.n))
+    assert np.array_equal(ox[sid], sx) and np.array_equal(ov[sid], sv)
+
+
+# ------------------------------------------------------------------ sand (simulate_sand)
+def _sand_compare(gpu_mod, oracle, pos, sol, steps, box=(20, 20, 20), flags=None, h_scale=3.1):
+    s = oracle.init_constants(*box, 0.5, 1.0, h_scale)
+    st = oracle.SandStream(s, pos, sol)
+    n, ns = pos.shape[0], (0 if sol is None else sol.shape[0])
+    total = st.total
+    solid_index = np.arange(total - 1, total - 1 - ns, -1, dtype=np.int32)  # written downward
+    sp = gpu_mod.sand_params(*box, kernel_radius_scale=h_scale)
+    sp.ptr_solid_start = total - ns if ns else total - 1
+    with gpu_mod.Gpu(0) as g:
+        g.sand_init(sp, pos, None, None, sol, solid_index)
+        for k in range(steps):
+            fl = flags[k] if flags else (False, False, None)
+            st.st.attract_flag, st.st.blow_flag = int(fl[0]), int(fl[1])
+            if fl[2] is not None:
+                st.st.player_position[:] = list(fl[2])
+            st.step(np.float32(0.016))
+            g.sand_step(np.float32(0.016), fl[0], fl[1], fl[2])
+        got = g.sand_download()
+        goff, gidx = g.sand_neighbors()
+    want = (st.p[:n], st.ps[:n], st.v[:n], st.att[:n], st.perm())
+    ooff, oidx = st.neighbors()
+    return got, want, (goff, gidx), (ooff, oidx)
+
+
+def test_sand_steps_match_reference(gpu_mod, oracle):
+    pos, sol = _scene_with_solids(n_side=12, seed=7)
+    got, want, (goff, gidx), (ooff, oidx) = _sand_compare(gpu_mod, oracle, pos, sol, steps=5)
+    assert np.array_equal(got[4], want[4])  # last counting-sort permutation
+    assert np.array_equal(goff, ooff) and np.array_equal(gidx, oidx)  # neighbor lists, in order
+    assert np.array_equal(got[3], want[3])
+    for a, b in zip(got[:3], want[:3]):
+        assert rel_err(a, b) <= REL
+    print(f"sand x5 ulp(x)={max_ulp(got[0], want[0])} ulp(v)={max_ulp(got[2], want[2])}")
+
+
+def test_sand_attract_and_blow_flags(gpu_mod, oracle):
+    pos, sol = _scene_with_solids(n_side=8, seed=9)
+    player = np.array([6.0, 6.0, 6.0], np.float32)
+    flags = [(True, False, player), (True, False, player), (False, True, player), (False, False, player)]
+    got, want, (goff, gidx), (ooff, oidx) = _sand_compare(gpu_mod, oracle, pos, sol, steps=4, flags=flags)
+    assert np.array_equal(got[4], want[4])
+    assert np.array_equal(gidx, oidx)
+    assert np.array_equal(got[3], want[3])
+    for a, b in zip(got[:3], want[:3]):
+        assert rel_err(a, b) <= REL
+
+
+def test_sand_no_solids(gpu_mod, oracle):
+    pos = scenes.jittered_lattice(10, 10, 10, origin=(3.0, 1.0, 3.0), amplitude=0.2, seed=2)
+    got, want, (goff, gidx), (ooff, oidx) = _sand_compare(gpu_mod, oracle, pos, None, steps=3)
+    assert np.array_equal(got[4], want[4]) and np.array_equal(gidx, oidx)
+    for a, b in zip(got[:3], want[:3]):
+        assert rel_err(a, b) <= REL
