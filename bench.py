@@ -2,13 +2,16 @@
 
 Workload: the 1,048,576-particle dam break (128x64x128 jittered lattice in a
 256x128x256 box, h = 2, dt = 1/60, 4 solver iterations, XSPH c = 0.01).  A
-step is one full substep: predict, cell key + histogram, cell scan, stable
-scatter/permute, 4 x (lambda pass, delta-p pass), XSPH + commit.  Inputs are
-resident in HBM before the timed region; hipEvents on the library's stream
-bracket exactly --steps steps after --warmup untimed steps.
+step is one full substep: predict + cell key, cell scan, scatter, stable
+permute, 4 x (lambda pass, delta-p pass), XSPH + commit.  Inputs are resident
+in HBM before the timed region; hipEvents on the library's stream bracket
+exactly --steps steps after --warmup untimed steps.
 
-Prints one JSON line.  The CPU baseline is the FP32 oracle (one thread) timed
-on a bounded sample of the same scene on this host.
+Per-kernel times come from a second run over the same simulation window (the
+scene is re-initialised, same warmup and steps) with an event after every
+kernel; the roofline entry uses the dominant kernel's average launch time from
+that run.  The CPU baseline is the FP32 oracle (one thread) timed on a bounded
+sample of the same scene on this host.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -27,14 +30,15 @@ sys.path.insert(0, ROOT)
 METRIC = "PBF particle-steps/s at 1M particles; learned vs plain at equal density error"
 PEAK_HBM_GBS = 8000.0
 
-# algorithmic bytes per particle per launch (compulsory float4 SoA traffic,
-# neighbour gathers served on chip)
+# algorithmic bytes per particle per launch: compulsory float4 SoA traffic,
+# neighbour gathers served on chip
 PHASE_BYTES = {
-    "predict_key": 64 + 24,   # read x, v; (predict) + cell key, histogram
+    "predict_key": 64 + 24,   # read x, v (predict) + cell key, histogram
     "cell_scan": None,        # 8 bytes per cell, set from the grid
     "scatter": 12,            # read key, slot; write slot index
     "stable_permute": 96,     # read rank, x, x*, v; write sorted x, x*, id, key
     "lambda": 28,             # read x*, cell range; write lambda
+    "lambda_build": 28,
     "deltap": 44,             # read x*, lambda, range; write x*
     "xsph": 88,               # read x, x*, v, range; write v, x
 }
@@ -74,9 +78,9 @@ def cpu_baseline(cfg, seconds_target: float = 15.0) -> dict:
         "unit": "particle-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{steps} steps of the {cfg.n}-particle scene after 1 warm step, "
-                  f"{el:.1f} s, FP32 oracle (-O2, no fast-math, no FMA), {cpu_model()}, "
-                  f"{os.cpu_count()} cpus visible",
+        "sample": f"steps 2-{steps + 1} of the {cfg.n}-particle scene ({el:.1f} s); FP32 oracle, "
+                  f"-O2 -fno-fast-math -ffp-contract=off, 1 thread of {cpu_model()} "
+                  f"({os.cpu_count()} cpus visible)",
     }
 
 
@@ -87,13 +91,16 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg2", "cfg2_tail"])
     ap.add_argument("--iterations", type=int, default=None)
+    ap.add_argument("--exact", action="store_true",
+                    help="oracle operation order (bit-identical results) instead of the fused path")
+    ap.add_argument("--no-phases", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is described, not built)", file=sys.stderr)
+              "(the multi-GPU sharding is described, not built)", file=sys.stderr)
         return 2
 
     from lustrine_amd import gpu as G
@@ -108,34 +115,19 @@ def main(argv=None):
     x0 = cfg.positions()
     gx, gy, gz = cfg.grid
     ncell = gx * gy * gz
+    params = G.pbf_params(cfg, exact=args.exact)
+    dt = np.float32(cfg.dt)
     with G.Gpu(0) as g:
-        g.pbf_init(G.pbf_params(cfg), x0, None)
-        ms, phase_ms, launches = g.pbf_bench(np.float32(cfg.dt), args.warmup, args.steps, phases=True)
+        g.pbf_init(params, x0, None)
+        ms, _, _ = g.pbf_bench(dt, args.warmup, args.steps, phases=False)
         maxc = g.pbf_last_max_constraint()
+        phase_ms = launches = None
+        if not args.no_phases:
+            g.pbf_init(params, x0, None)
+            _, phase_ms, launches = g.pbf_bench(dt, args.warmup, args.steps, phases=True)
     ms_per_step = ms / args.steps
     value = cfg.n * args.steps / (ms / 1e3)
-
-    # per-phase roofline from the event-bracketed second run
-    pb = dict(PHASE_BYTES)
-    pb["cell_scan"] = 8.0 * (ncell + 1) / cfg.n
-    phases = {}
-    for name, t in phase_ms.items():
-        nl = max(launches[name], 1)
-        avg_ms = t / nl
-        bytes_launch = pb[name] * cfg.n
-        gbs = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-        phases[name] = {"total_ms": round(t, 4), "launches": launches[name],
-                        "avg_launch_us": round(avg_ms * 1e3, 3), "alg_bytes_per_launch": int(bytes_launch),
-                        "alg_GBps": round(gbs, 1)}
-    dom = max(phase_ms, key=lambda k: phase_ms[k])
-    d = phases[dom]
-    achieved = d["alg_GBps"]
     step_bytes = (STEP_BYTES_K4 - 4 * 72 + cfg.iterations * 72) * cfg.n
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{cfg.name}.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC,
@@ -152,15 +144,39 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": cfg.name, "particles": cfg.n, "lattice": list(cfg.lattice),
                    "box": list(cfg.box), "cells": ncell, "h": cfg.h, "dt": cfg.dt,
-                   "iterations": cfg.iterations, "c_xsph": cfg.c_xsph, "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic},
+                   "iterations": cfg.iterations, "c_xsph": cfg.c_xsph,
+                   "arithmetic": "exact" if args.exact else "fused", "parallelism": "single-gpu"},
         "step_roofline": {"alg_bytes_per_step": int(step_bytes),
                           "achieved_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
-        "phases": phases,
         "max_density_constraint_last_step": maxc,
     }
+    if phase_ms is not None:
+        pb = dict(PHASE_BYTES)
+        pb["cell_scan"] = 8.0 * (ncell + 1) / cfg.n
+        phases = {}
+        for name, t in phase_ms.items():
+            nl = launches[name]
+            if nl == 0:
+                continue
+            avg_us = t / nl * 1e3
+            b = pb[name] * cfg.n
+            phases[name] = {"total_ms": round(t, 4), "launches": nl, "avg_launch_us": round(avg_us, 2),
+                            "alg_bytes_per_launch": int(b),
+                            "alg_GBps": round(b / (avg_us * 1e-6) / 1e9, 1)}
+        dom = max(phases, key=lambda k: phases[k]["total_ms"])
+        d = phases[dom]
+        achieved = d["alg_bytes_per_launch"] / (d["avg_launch_us"] * 1e-6) / 1e9
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{cfg.name}.json")
+        if os.path.exists(tfile):
+            with open(tfile) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
+                           "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                           "avg_launch_us": d["avg_launch_us"],
+                           "alg_bytes_per_launch": d["alg_bytes_per_launch"]}
+        out["phases"] = phases
     if not args.no_cpu_baseline:
         cb = cpu_baseline(cfg, args.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -48,6 +48,12 @@ typedef struct LusPbfParams {
     int scorr_n;                         /* Simulation::s_corr_n (integer power) */
     float c_xsph;                        /* Simulation::c_xsph */
     int iterations;                      /* Jacobi solver iterations */
+    /* 1: every float operation in the order of the CPU oracle, IEEE divide and
+     *    sqrt, no fused multiply-add (results bit-identical to the oracle);
+     * 0: fused multiply-add and hardware reciprocal square root in the
+     *    neighbour loops (results within a few ulp of mode 1).  The
+     *    neighbour sets are built with the exact test in both modes. */
+    int exact;
 } LusPbfParams;
 
 /* simulate_sand constants (src/Simulate.cpp:159-163 locals and the
@@ -122,14 +128,23 @@ int lus_pbf_download(LusGpu* ctx, int order, float* pos, float* vel, int* ids);
  * in the last lambda pass of the last step. */
 int lus_pbf_last_max_constraint(LusGpu* ctx, float* out);
 /* Runs warmup + steps substeps; *ms = elapsed of the `steps` (hipEvents on the
- * context stream).  When phase_ms is non-NULL (array of LUS_PBF_NPHASES), a
- * second run of `steps` records an event after every kernel and accumulates
- * per-phase device time; phase_launches receives the launch count per phase. */
-#define LUS_PBF_NPHASES 7
+ * context stream, one before and one after).  When phase_ms is non-NULL
+ * (array of LUS_PBF_NPHASES) the `steps` instead record an event after every
+ * kernel: phase_ms accumulates per-phase device time, phase_launches the
+ * launch count per phase, and *ms their sum. */
+#define LUS_PBF_NPHASES 8
 /* phases: 0 predict+cell key, 1 cell scan, 2 scatter, 3 stable permute,
- *         4 lambda pass, 5 delta-p pass, 6 XSPH + commit */
+ *         4 lambda pass (iterations 2..K), 5 delta-p pass, 6 XSPH + commit,
+ *         7 first lambda pass of the step (builds the neighbour sets) */
 int lus_pbf_bench(LusGpu* ctx, float dt, int warmup, int steps, float* ms,
                   float* phase_ms, int* phase_launches);
+/* Average device time per launch (microseconds) of the neighbour-pass kernels
+ * on the current state: us[0] lambda pass, us[1] delta-p pass, us[2] XSPH
+ * pass, us[3] the first lambda pass of a step (which also builds the
+ * neighbour sets), each launched `reps` times back to back between two
+ * hipEvents on the context stream.  Outputs go to scratch buffers; the scene
+ * state is left unchanged.  Requires at least one step since init. */
+int lus_pbf_time_kernels(LusGpu* ctx, int reps, float* us);
 
 /* --- sand scene: simulate_sand (src/Simulate.cpp:156-325) --- */
 /* n_sand moving particles (pos, vel, attracted may be NULL for zero) and

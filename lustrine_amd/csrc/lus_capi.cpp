@@ -9,6 +9,7 @@
 #include <cstring>
 #include <vector>
 
+#include "lus_device.hpp"
 #include "lus_internal.hpp"
 #include "lustrine_gpu.h"
 
@@ -106,10 +107,14 @@ struct PbfState {
     lus::PbfK k{};
     int n = 0, C = 0;
     Arena mem;
-    float4 *x = nullptr, *x2 = nullptr, *v = nullptr, *v2 = nullptr, *xa = nullptr, *xl = nullptr;
+    float4 *x = nullptr, *x2 = nullptr, *v = nullptr, *v2 = nullptr, *xa = nullptr, *xl = nullptr,
+           *xp = nullptr;
     int *id = nullptr, *id2 = nullptr, *key = nullptr, *slot = nullptr, *tmp = nullptr,
         *key2 = nullptr, *counts = nullptr, *cs = nullptr, *tiles = nullptr, *maxc = nullptr;
-    int nmaxc = 0;
+    unsigned long long* masks = nullptr;  // 9 neighbour-row masks per particle
+    int nmaxc = 0;  // iterations with a block-max table
+    int nblk = 0;   // blocks of one particle-parallel launch
+    bool stepped = false;
 };
 
 struct SandState {
@@ -178,6 +183,7 @@ lus::PbfK pbf_constants(const LusPbfParams& p) {
     k.gY = p.gridY;
     k.gZ = p.gridZ;
     k.gXZ = p.gridX * p.gridZ;
+    k.exact = p.exact ? 1 : 0;
     return k;
 }
 
@@ -196,29 +202,39 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         return hipEventRecord(e, s);
     };
     if (ev) LUS_HIP(mark(-1));
-    LUS_HIP(lus::launch_pbf_key(P.x, P.v, P.n, P.k, dt, P.key, P.slot, P.counts, P.maxc, P.nmaxc, s));
+    LUS_HIP(lus::launch_pbf_key(P.x, P.v, P.n, P.k, dt, P.key, P.slot, P.counts, s));
     LUS_HIP(mark(0));
     LUS_HIP(lus::launch_exclusive_scan(P.counts, P.cs, P.C + 1, P.tiles, 1, s));
     LUS_HIP(mark(1));
     LUS_HIP(lus::launch_scatter(P.key, P.slot, P.cs, P.n, P.tmp, s));
     LUS_HIP(mark(2));
-    LUS_HIP(lus::launch_pbf_permute(P.tmp, P.key, P.cs, P.x, P.v, P.id, P.n, P.k, dt, P.x2, P.xa,
+    LUS_HIP(lus::launch_pbf_permute(P.tmp, P.key, P.cs, P.x, P.v, P.id, P.n, P.k, dt, P.x2, P.xp,
                                     P.id2, P.key2, s));
     LUS_HIP(mark(3));
+    // iterate: xp (predicted) -> lambda -> xl -> delta-p -> xa -> lambda -> xl ...
     const int K = P.p.iterations;
     for (int it = 0; it < K; ++it) {
-        LUS_HIP(lus::launch_pbf_lambda(P.xa, P.key2, P.cs, P.n, P.k, P.xl, P.maxc + it, s));
-        LUS_HIP(mark(4));
+        const float4* cur = it == 0 ? P.xp : P.xa;
+        LUS_HIP(lus::launch_pbf_lambda(cur, P.key2, P.cs, P.n, P.k, P.xl, P.maxc + (size_t)it * P.nblk,
+                                       P.masks, P.xp,
+                                       it == 0 ? 1 : 0, s));
+        LUS_HIP(mark(it == 0 ? 7 : 4));
         const bool last = (it == K - 1);
         LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, P.k, P.xa, P.x2,
-                                       last ? P.v2 : nullptr, dt, s));
+                                       last ? P.v2 : nullptr, dt, P.masks, P.xp, s));
         LUS_HIP(mark(5));
     }
-    if (K == 0) LUS_HIP(lus::launch_pbf_velocity(P.xa, P.x2, P.n, dt, P.v2, s));
-    LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, P.k, P.v, P.x2, s));
+    const float4* fin = K > 0 ? P.xa : P.xp;
+    if (K == 0) {
+        // neighbour sets for XSPH (the lambda values are not used)
+        LUS_HIP(lus::launch_pbf_lambda(P.xp, P.key2, P.cs, P.n, P.k, P.xl, P.maxc, P.masks, P.xp, 1, s));
+        LUS_HIP(lus::launch_pbf_velocity(P.xp, P.x2, P.n, dt, P.v2, s));
+    }
+    LUS_HIP(lus::launch_pbf_xsph(fin, P.v2, P.key2, P.cs, P.n, P.k, P.v, P.x2, P.masks, P.xp, s));
     LUS_HIP(mark(6));
     std::swap(P.x, P.x2);
     std::swap(P.id, P.id2);
+    P.stepped = true;
     return LUS_OK;
 }
 
@@ -334,6 +350,20 @@ struct SortedSet {
     int *key = nullptr, *cs = nullptr, *perm = nullptr;
 };
 
+// solid positions and reference slots reordered by ascending slot
+void slot_order(const float* xyz, const int* index, int n, std::vector<float>* pos,
+                std::vector<int>* idx) {
+    std::vector<int> o((size_t)n);
+    for (int k = 0; k < n; ++k) o[k] = k;
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return index[a] < index[b]; });
+    pos->resize((size_t)3 * n);
+    idx->resize((size_t)n);
+    for (int k = 0; k < n; ++k) {
+        for (int a = 0; a < 3; ++a) (*pos)[3 * (size_t)k + a] = xyz[3 * (size_t)o[k] + a];
+        (*idx)[k] = index[o[k]];
+    }
+}
+
 int sort_points(Arena& m, const LusGrid& g, const float* xyz, int n, SortedSet* out) {
     const int C = g.gridX * g.gridY * g.gridZ;
     float4 *d_pos, *d_sorted;
@@ -382,6 +412,12 @@ int lus_neighbors_fluid(const LusGrid* g, float kernel_radius, int n_sand, const
     SortedSet S, O;
     int r = sort_points(m, *g, sand_pos, n_sand, &S);
     if (r) return r;
+    // solids enter their cells in ascending reference slot (Neighbors.cpp:380-384)
+    std::vector<float> spos;
+    std::vector<int> sidx;
+    slot_order(solid_pos, solid_index, n_solid, &spos, &sidx);
+    solid_pos = spos.data();
+    solid_index = sidx.data();
     r = sort_points(m, *g, solid_pos, n_solid, &O);
     if (r) return r;
     // inverse permutation and sorted solid reference slots (host)
@@ -442,6 +478,7 @@ int lus_pbf_init(LusGpu* c, const LusPbfParams* p, int n, const float* pos, cons
     LUS_HIP(hipStreamSynchronize(c->stream));
     P.mem.release();
     P.ready = false;
+    P.stepped = false;
     P.p = *p;
     P.k = pbf_constants(*p);
     P.n = n;
@@ -454,6 +491,8 @@ int lus_pbf_init(LusGpu* c, const LusPbfParams* p, int n, const float* pos, cons
     LUS_HIP(m.alloc(&P.v2, n));
     LUS_HIP(m.alloc(&P.xa, n));
     LUS_HIP(m.alloc(&P.xl, n));
+    LUS_HIP(m.alloc(&P.xp, n));
+    LUS_HIP(m.alloc(&P.masks, (size_t)9 * n));
     LUS_HIP(m.alloc(&P.id, n));
     LUS_HIP(m.alloc(&P.id2, n));
     LUS_HIP(m.alloc(&P.key, n));
@@ -463,7 +502,8 @@ int lus_pbf_init(LusGpu* c, const LusPbfParams* p, int n, const float* pos, cons
     LUS_HIP(m.alloc(&P.counts, P.C + 1));
     LUS_HIP(m.alloc(&P.cs, P.C + 1));
     LUS_HIP(m.alloc(&P.tiles, lus::scan_tile_count(P.C + 1) + 1));
-    LUS_HIP(m.alloc(&P.maxc, P.nmaxc));
+    P.nblk = lus::grid_for(n);
+    LUS_HIP(m.alloc(&P.maxc, (size_t)P.nmaxc * P.nblk));
     std::vector<float4> hx = to_f4(pos, n), hv = to_f4(vel, n);
     std::vector<int> hid((size_t)n);
     for (int i = 0; i < n; ++i) hid[i] = i;
@@ -471,8 +511,8 @@ int lus_pbf_init(LusGpu* c, const LusPbfParams* p, int n, const float* pos, cons
     LUS_HIP(hipMemcpy(P.v, hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     LUS_HIP(hipMemcpy(P.id, hid.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     LUS_HIP(hipMemset(P.counts, 0, sizeof(int) * (P.C + 1)));
-    std::vector<int> mc((size_t)P.nmaxc, INT32_MIN);
-    LUS_HIP(hipMemcpy(P.maxc, mc.data(), sizeof(int) * P.nmaxc, hipMemcpyHostToDevice));
+    std::vector<int> mc((size_t)P.nmaxc * P.nblk, INT32_MIN);
+    LUS_HIP(hipMemcpy(P.maxc, mc.data(), sizeof(int) * mc.size(), hipMemcpyHostToDevice));
     // the context stream does not order against the null stream used above
     LUS_HIP(hipDeviceSynchronize());
     P.ready = true;
@@ -524,8 +564,10 @@ int lus_pbf_last_max_constraint(LusGpu* c, float* out) {
     PbfState& P = c->pbf;
     LUS_HIP(hipStreamSynchronize(c->stream));
     int last = std::max(P.p.iterations - 1, 0);
-    int m;
-    LUS_HIP(hipMemcpy(&m, P.maxc + last, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> blk((size_t)P.nblk);
+    LUS_HIP(hipMemcpy(blk.data(), P.maxc + (size_t)last * P.nblk, sizeof(int) * P.nblk,
+                      hipMemcpyDeviceToHost));
+    int m = *std::max_element(blk.begin(), blk.end());
     int b = m >= 0 ? m : (m ^ 0x7FFFFFFF);
     float f;
     memcpy(&f, &b, sizeof(f));
@@ -540,19 +582,21 @@ int lus_pbf_bench(LusGpu* c, float dt, int warmup, int steps, float* ms, float* 
     LUS_HIP(hipSetDevice(c->device));
     int r = lus_pbf_step(c, dt, warmup);
     if (r) return r;
-    hipEvent_t e0, e1;
-    LUS_HIP(hipEventCreate(&e0));
-    LUS_HIP(hipEventCreate(&e1));
     LUS_HIP(hipStreamSynchronize(c->stream));
-    LUS_HIP(hipEventRecord(e0, c->stream));
-    r = lus_pbf_step(c, dt, steps);
-    if (r) return r;
-    LUS_HIP(hipEventRecord(e1, c->stream));
-    LUS_HIP(hipEventSynchronize(e1));
-    LUS_HIP(hipEventElapsedTime(ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (phase_ms) {
+    if (!phase_ms) {
+        hipEvent_t e0, e1;
+        LUS_HIP(hipEventCreate(&e0));
+        LUS_HIP(hipEventCreate(&e1));
+        LUS_HIP(hipEventRecord(e0, c->stream));
+        r = lus_pbf_step(c, dt, steps);
+        if (r) return r;
+        LUS_HIP(hipEventRecord(e1, c->stream));
+        LUS_HIP(hipEventSynchronize(e1));
+        LUS_HIP(hipEventElapsedTime(ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    } else {
+        *ms = 0.0f;
         for (int k = 0; k < LUS_PBF_NPHASES; ++k) {
             phase_ms[k] = 0.0f;
             if (phase_launches) phase_launches[k] = 0;
@@ -567,11 +611,51 @@ int lus_pbf_bench(LusGpu* c, float dt, int warmup, int steps, float* ms, float* 
                 float t = 0.0f;
                 LUS_HIP(hipEventElapsedTime(&t, ev[k - 1], ev[k]));
                 phase_ms[ph[k]] += t;
+                *ms += t;
                 if (phase_launches) phase_launches[ph[k]] += 1;
             }
             for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         }
     }
+    return LUS_OK;
+}
+
+int lus_pbf_time_kernels(LusGpu* c, int reps, float* us) {
+    if (!c || !c->pbf.ready || !us || reps <= 0)
+        return set_error(LUS_EINVAL, "lus_pbf_time_kernels: bad arguments");
+    PbfState& P = c->pbf;
+    if (!P.stepped) return set_error(LUS_ESTATE, "lus_pbf_time_kernels: no step yet");
+    LUS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    hipEvent_t e0, e1;
+    LUS_HIP(hipEventCreate(&e0));
+    LUS_HIP(hipEventCreate(&e1));
+    for (int kind = 0; kind < 4; ++kind) {
+        LUS_HIP(hipStreamSynchronize(s));
+        LUS_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) {
+            // xl, xa and x2 are per-step scratch between steps; x, v, id hold the state
+            if (kind == 0)
+                LUS_HIP(lus::launch_pbf_lambda(P.xa, P.key2, P.cs, P.n, P.k, P.xl, P.maxc, P.masks, P.xp,
+                                               0, s));
+            else if (kind == 1)
+                LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, P.k, P.xa, nullptr, nullptr,
+                                               1.0f, P.masks, P.xp, s));
+            else if (kind == 2)
+                LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, P.k, P.xl, P.x2, P.masks,
+                                             P.xp, s));
+            else
+                LUS_HIP(lus::launch_pbf_lambda(P.xp, P.key2, P.cs, P.n, P.k, P.xl, P.maxc, P.masks, P.xp,
+                                               1, s));
+        }
+        LUS_HIP(hipEventRecord(e1, s));
+        LUS_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        LUS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        us[kind] = ms * 1e3f / (float)reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return LUS_OK;
 }
 
@@ -665,6 +749,13 @@ int lus_sand_init(LusGpu* c, const LusSandParams* p, int n_sand, const float* po
     // static solids: stable cell sort once (uniform_grid_cells_static_saved,
     // src/neighbors/Neighbors.cpp:266-272, keeps ascending slot order per cell)
     SortedSet O;
+    // the static cell lists hold solids in ascending reference slot
+    // (Neighbors.cpp:266-272)
+    std::vector<float> spos;
+    std::vector<int> sidx;
+    slot_order(solid_pos, solid_index, n_solid, &spos, &sidx);
+    solid_pos = spos.data();
+    solid_index = sidx.data();
     int rr = sort_points(m, g, solid_pos, n_solid, &O);
     if (rr) return rr;
     S.solid = n_solid ? O.pos : nullptr;

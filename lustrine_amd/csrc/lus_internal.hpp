@@ -20,6 +20,7 @@ struct PbfK {
     float h2, hf, hf2, f, f2, SPf, c_rho, mr0sq, eps, inv_tq3, scorr_k, neg_inv_r0, cW, cs;
     float lo[3], hi[3], g[3];
     int gX, gY, gZ, gXZ, n_scorr;
+    int exact;  // 1: oracle operation order; 0: fused fast path
 };
 
 // simulate_sand constants (src/Simulate.cpp:159-163 and the Simulation fields).
@@ -60,20 +61,23 @@ hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, cons
 
 // PBF substep
 hipError_t launch_pbf_key(const float4* x, const float4* v, int n, PbfK k, float dt, int* key,
-                          int* slot, int* counts, int* maxc, int nmaxc, hipStream_t s);
+                          int* slot, int* counts, hipStream_t s);
 hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_start,
                               const float4* x, const float4* v, const int* id, int n, PbfK k,
-                              float dt, float4* x2, float4* xa, int* id2, int* key2,
+                              float dt, float4* x2, float4* xp, int* id2, int* key2,
                               hipStream_t s);
-hipError_t launch_pbf_lambda(const float4* xa, const int* key2, const int* cs, int n, PbfK k,
-                             float4* xl, int* maxc, hipStream_t s);
-hipError_t launch_pbf_deltap(const float4* xl, const int* key2, const int* cs, int n, PbfK k,
-                             float4* xa, const float4* xold, float4* vout, float dt,
+// masks: 9 * n neighbour-row masks; build = 1 on the first lambda pass of a step
+hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
+                             int* maxc, unsigned long long* masks, const float4* xp, int build,
                              hipStream_t s);
+hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
+                             const float4* xold, float4* vout, float dt, unsigned long long* masks,
+                             const float4* xp, hipStream_t s);
 hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, float dt,
                                float4* vout, hipStream_t s);
-hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, const int* key2, const int* cs,
-                           int n, PbfK k, float4* vout, float4* xout, hipStream_t s);
+hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
+                           PbfK k, float4* vout, float4* xout, unsigned long long* masks,
+                           const float4* xp, hipStream_t s);
 
 // sand (simulate_sand)
 hipError_t launch_sand_predict_key(float4* x, float4* v, int* att, int n, SandK k, float dt,

@@ -20,44 +20,10 @@
 
 #include <climits>
 
+#include "lus_device.hpp"
 #include "lus_internal.hpp"
 
 namespace lus {
-
-constexpr int kBlock = 256;
-constexpr int kXcd = 8;
-
-static inline int grid_for(int n) {
-    int b = (n + kBlock - 1) / kBlock;
-    if (b < 1) b = 1;
-    return ((b + kXcd - 1) / kXcd) * kXcd;
-}
-
-__device__ __forceinline__ int xcd_index() {
-    const int nb = gridDim.x;
-    const int b = blockIdx.x;
-    const int tile = (b % kXcd) * (nb / kXcd) + b / kXcd;
-    return tile * kBlock + threadIdx.x;
-}
-
-__device__ __forceinline__ float rclamp(float x, float lo, float hi) {
-    float t = (x < lo) ? lo : x;
-    return (hi < t) ? hi : t;
-}
-
-// get_cell_id, src/neighbors/Utils.hpp:24-33
-__device__ __forceinline__ int cell_of(float x, float y, float z, float cs, int gX, int gZ) {
-    float px = x / cs;
-    float py = y / cs;
-    float pz = z / cs;
-    return ((int)py) * gX * gZ + ((int)px) * gZ + ((int)pz);
-}
-
-// float -> int preserving order, for atomicMax on floats
-__device__ __forceinline__ int ordered(float f) {
-    int b = __float_as_int(f);
-    return b >= 0 ? b : b ^ 0x7FFFFFFF;
-}
 
 // ------------------------------------------------------------------ scan
 constexpr int kScanV = 8;
@@ -172,8 +138,10 @@ __global__ __launch_bounds__(kBlock) void k_count_keys(const int* __restrict__ k
                                                         int* __restrict__ counts,
                                                         int* __restrict__ slot) {
     const int i = xcd_index();
-    if (i >= n) return;
-    slot[i] = atomicAdd(&counts[key[i]], 1);
+    const bool active = i < n;
+    const int c = active ? key[i] : 0;
+    const int sl = wave_atomic_slot(counts, c, active);
+    if (active) slot[i] = sl;
 }
 
 hipError_t launch_count_keys(const int* key, int n, int* counts, int* slot, hipStream_t s) {
@@ -195,19 +163,6 @@ hipError_t launch_scatter(const int* key, const int* slot, const int* cell_start
     hipLaunchKernelGGL(k_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, key, slot, cell_start, n,
                        tmp);
     return hipGetLastError();
-}
-
-// The atomic slots order a cell's entries arbitrarily.  Entry p of cell c
-// moves to cell_start[c] + #{q in cell c : tmp[q] < tmp[p]}: ascending
-// original index inside the cell, i.e. the unique stable order that
-// Sorting::counting_sort produces (src/neighbors/Sorting.cpp:11-33).
-__device__ __forceinline__ int stable_slot(const int* __restrict__ tmp, const int* __restrict__ cell_start,
-                                           int c, int i) {
-    const int s = cell_start[c];
-    const int e = cell_start[c + 1];
-    int r = 0;
-    for (int q = s; q < e; ++q) r += (tmp[q] < i) ? 1 : 0;
-    return s + r;
 }
 
 __global__ __launch_bounds__(kBlock) void k_stable_rank(const int* __restrict__ tmp,
@@ -328,260 +283,7 @@ hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, cons
     return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ PBF
-__device__ __forceinline__ float4 pbf_predict(float4 x, float4 v, const PbfK& k, float dt) {
-    float vx = v.x + k.g[0] * dt;
-    float vy = v.y + k.g[1] * dt;
-    float vz = v.z + k.g[2] * dt;
-    float4 r;
-    r.x = rclamp(x.x + vx * dt, k.lo[0], k.hi[0]);
-    r.y = rclamp(x.y + vy * dt, k.lo[1], k.hi[1]);
-    r.z = rclamp(x.z + vz * dt, k.lo[2], k.hi[2]);
-    r.w = 0.0f;
-    return r;
-}
-
-// 27-cell stencil as 9 contiguous slot ranges, ascending slot order
-template <class F>
-__device__ __forceinline__ void for_each_candidate(int key, const int* __restrict__ cs, const PbfK& k,
-                                                   F&& f) {
-    const int cy = key / k.gXZ;
-    const int rm = key - cy * k.gXZ;
-    const int cx = rm / k.gZ;
-    const int cz = rm - cx * k.gZ;
-    const int z0 = cz > 0 ? cz - 1 : 0;
-    const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
-    for (int dy = -1; dy <= 1; ++dy) {
-        const int y = cy + dy;
-        if (y < 0 || y >= k.gY) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int x = cx + dx;
-            if (x < 0 || x >= k.gX) continue;
-            const int row = y * k.gXZ + x * k.gZ;
-            const int j0 = cs[row + z0];
-            const int j1 = cs[row + z1 + 1];
-            for (int j = j0; j < j1; ++j) f(j);
-        }
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_pbf_key(const float4* __restrict__ x,
-                                                     const float4* __restrict__ v, int n, PbfK k,
-                                                     float dt, int* __restrict__ key,
-                                                     int* __restrict__ slot,
-                                                     int* __restrict__ counts,
-                                                     int* __restrict__ maxc, int nmaxc) {
-    const int i = xcd_index();
-    if (i < nmaxc) maxc[i] = INT_MIN;
-    if (i >= n) return;
-    const float4 xs = pbf_predict(x[i], v[i], k, dt);
-    const int c = cell_of(xs.x, xs.y, xs.z, k.cs, k.gX, k.gZ);
-    key[i] = c;
-    slot[i] = atomicAdd(&counts[c], 1);
-}
-
-hipError_t launch_pbf_key(const float4* x, const float4* v, int n, PbfK k, float dt, int* key,
-                          int* slot, int* counts, int* maxc, int nmaxc, hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_key, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, n, k, dt, key, slot,
-                       counts, maxc, nmaxc);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(kBlock) void k_pbf_permute(
-    const int* __restrict__ tmp, const int* __restrict__ key, const int* __restrict__ cell_start,
-    const float4* __restrict__ x, const float4* __restrict__ v, const int* __restrict__ id, int n,
-    PbfK k, float dt, float4* __restrict__ x2, float4* __restrict__ xa, int* __restrict__ id2,
-    int* __restrict__ key2) {
-    const int p = xcd_index();
-    if (p >= n) return;
-    const int i = tmp[p];
-    const int c = key[i];
-    const int o = stable_slot(tmp, cell_start, c, i);
-    const float4 xi = x[i];
-    x2[o] = xi;
-    xa[o] = pbf_predict(xi, v[i], k, dt);
-    id2[o] = id[i];
-    key2[o] = c;
-}
-
-hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_start,
-                              const float4* x, const float4* v, const int* id, int n, PbfK k,
-                              float dt, float4* x2, float4* xa, int* id2, int* key2,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_permute, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start,
-                       x, v, id, n, k, dt, x2, xa, id2, key2);
-    return hipGetLastError();
-}
-
-// lambda pass: density constraint and its gradient norm (PBF eq. 1, 8, 11)
-__global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict__ xa,
-                                                        const int* __restrict__ key2,
-                                                        const int* __restrict__ cs, int n, PbfK k,
-                                                        float4* __restrict__ xl,
-                                                        int* __restrict__ maxc) {
-    const int i = xcd_index();
-    float Ci = -1e30f;
-    if (i < n) {
-        const float4 pi = xa[i];
-        float rho = 0.0f, a = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-        for_each_candidate(key2[i], cs, k, [&](int j) {
-            const float4 pj = xa[j];
-            const float dx = pi.x - pj.x;
-            const float dy = pi.y - pj.y;
-            const float dz = pi.z - pj.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 <= k.h2) {
-                const float t = k.hf2 - k.f2 * r2;
-                rho = rho + (t * t) * t;
-                if (r2 > 0.0f) {
-                    const float rl = sqrtf(r2);
-                    const float g = k.hf - k.f * rl;
-                    const float s = ((k.SPf * g) * g) / rl;
-                    a = a + (s * s) * r2;
-                    gx = gx + s * dx;
-                    gy = gy + s * dy;
-                    gz = gz + s * dz;
-                }
-            }
-        });
-        Ci = rho * k.c_rho - 1.0f;
-        const float gg = gx * gx + gy * gy + gz * gz;
-        const float den = k.mr0sq * (a + gg) + k.eps;
-        xl[i] = make_float4(pi.x, pi.y, pi.z, -Ci / den);
-    }
-    // wave max, one atomic per wave
-    int m = ordered(Ci);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        int o = __shfl_xor(m, d, 64);
-        m = o > m ? o : m;
-    }
-    if ((threadIdx.x & 63) == 0) atomicMax(maxc, m);
-}
-
-hipError_t launch_pbf_lambda(const float4* xa, const int* key2, const int* cs, int n, PbfK k,
-                             float4* xl, int* maxc, hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_lambda, dim3(grid_for(n)), dim3(kBlock), 0, s, xa, key2, cs, n, k, xl,
-                       maxc);
-    return hipGetLastError();
-}
-
-// delta-p pass with s_corr (PBF eq. 12-14), Jacobi update, box clamp
-__global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict__ xl,
-                                                        const int* __restrict__ key2,
-                                                        const int* __restrict__ cs, int n, PbfK k,
-                                                        float4* __restrict__ xa,
-                                                        const float4* __restrict__ xold,
-                                                        float4* __restrict__ vout, float dt) {
-    const int i = xcd_index();
-    if (i >= n) return;
-    const float4 pi = xl[i];
-    const float li = pi.w;
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for_each_candidate(key2[i], cs, k, [&](int j) {
-        const float4 pj = xl[j];
-        const float dx = pi.x - pj.x;
-        const float dy = pi.y - pj.y;
-        const float dz = pi.z - pj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 <= k.h2 && r2 > 0.0f) {
-            const float rl = sqrtf(r2);
-            const float g = k.hf - k.f * rl;
-            const float s = ((k.SPf * g) * g) / rl;
-            const float t = k.hf2 - k.f2 * r2;
-            const float ratio = ((t * t) * t) * k.inv_tq3;
-            float pw = ratio;
-            for (int e = 1; e < k.n_scorr; ++e) pw = pw * ratio;
-            const float sc = -k.scorr_k * pw;
-            const float coef = (li + pj.w) + sc;
-            const float c = coef * s;
-            ax = ax + c * dx;
-            ay = ay + c * dy;
-            az = az + c * dz;
-        }
-    });
-    float4 r;
-    r.x = rclamp(pi.x + ax * k.neg_inv_r0, k.lo[0], k.hi[0]);
-    r.y = rclamp(pi.y + ay * k.neg_inv_r0, k.lo[1], k.hi[1]);
-    r.z = rclamp(pi.z + az * k.neg_inv_r0, k.lo[2], k.hi[2]);
-    r.w = 0.0f;
-    xa[i] = r;
-    if (vout) {
-        const float4 xo = xold[i];
-        vout[i] = make_float4((r.x - xo.x) / dt, (r.y - xo.y) / dt, (r.z - xo.z) / dt, 0.0f);
-    }
-}
-
-hipError_t launch_pbf_deltap(const float4* xl, const int* key2, const int* cs, int n, PbfK k,
-                             float4* xa, const float4* xold, float4* vout, float dt,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_deltap, dim3(grid_for(n)), dim3(kBlock), 0, s, xl, key2, cs, n, k, xa,
-                       xold, vout, dt);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(kBlock) void k_pbf_velocity(const float4* __restrict__ xa,
-                                                          const float4* __restrict__ xold, int n,
-                                                          float dt, float4* __restrict__ vout) {
-    const int i = xcd_index();
-    if (i >= n) return;
-    const float4 r = xa[i];
-    const float4 xo = xold[i];
-    vout[i] = make_float4((r.x - xo.x) / dt, (r.y - xo.y) / dt, (r.z - xo.z) / dt, 0.0f);
-}
-
-hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, float dt,
-                               float4* vout, hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_velocity, dim3(grid_for(n)), dim3(kBlock), 0, s, xa, xold, n, dt, vout);
-    return hipGetLastError();
-}
-
-// XSPH viscosity (PBF eq. 17) and commit x = x*
-__global__ __launch_bounds__(kBlock) void k_pbf_xsph(const float4* __restrict__ xa,
-                                                      const float4* __restrict__ vt,
-                                                      const int* __restrict__ key2,
-                                                      const int* __restrict__ cs, int n, PbfK k,
-                                                      float4* __restrict__ vout,
-                                                      float4* __restrict__ xout) {
-    const int i = xcd_index();
-    if (i >= n) return;
-    const float4 pi = xa[i];
-    const float4 vi = vt[i];
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for_each_candidate(key2[i], cs, k, [&](int j) {
-        const float4 pj = xa[j];
-        const float dx = pi.x - pj.x;
-        const float dy = pi.y - pj.y;
-        const float dz = pi.z - pj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 <= k.h2) {
-            const float t = k.hf2 - k.f2 * r2;
-            const float w = (t * t) * t;
-            const float4 vj = vt[j];
-            sx = sx + (vj.x - vi.x) * w;
-            sy = sy + (vj.y - vi.y) * w;
-            sz = sz + (vj.z - vi.z) * w;
-        }
-    });
-    vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
-    xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
-}
-
-hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, const int* key2, const int* cs,
-                           int n, PbfK k, float4* vout, float4* xout, hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_xsph, dim3(grid_for(n)), dim3(kBlock), 0, s, xa, vt, key2, cs, n, k,
-                       vout, xout);
-    return hipGetLastError();
-}
-
 // ------------------------------------------------------------------ sand
-__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
-    const float t0 = ax * bx;
-    const float t1 = ay * by;
-    const float t2 = az * bz;
-    return (t0 + t1) + t2;
-}
-
 // integration loop of simulate_sand, src/Simulate.cpp:188-213
 __global__ __launch_bounds__(kBlock) void k_sand_predict_key(float4* __restrict__ x,
                                                               float4* __restrict__ v,

@@ -1,0 +1,446 @@
+// lus_pbf.hip -- gfx950 kernels of the position-based-fluids substep.
+//
+// Per step (arrays in cell order after the counting sort, one float4 per
+// particle):
+//   k_pbf_key      predict x* = clamp(x + (v + g dt) dt), cell key, histogram
+//   (scan, scatter: lus_kernels.hip)
+//   k_pbf_permute  stable cell order; writes x (old), x* (predicted), id, key
+//   k_pbf_lambda   first call of the step also builds the neighbour sets
+//   k_pbf_deltap   Jacobi position correction with s_corr, box clamp
+//   k_pbf_xsph     XSPH viscosity, commit
+//
+// Neighbour sets.  N(i) = { j : |x*_i - x*_j|^2 <= h^2 } from the predicted
+// positions, once per step.  The 27-cell stencil of a particle is 9 rows of
+// contiguous slots (y-major cells, z contiguous), so N(i) is stored as one
+// 64-bit mask per row, masks[r * n + i], bit t = slot j0_r + t.  The first
+// lambda pass tests every candidate once and writes the masks; the other
+// passes of the step load 9 masks and visit only set bits, in ascending slot
+// order (the order of the CPU oracle's lists).  A particle whose stencil has a
+// row longer than 64 slots is flagged in its key (kOvf) and re-tests its
+// candidates against the stored predicted positions in every pass, which
+// gives the same set in the same order.
+//
+// Arithmetic modes (template kFast, chosen by LusPbfParams::exact):
+//   exact  every expression in the oracle's order, IEEE '/' and sqrtf, no
+//          FMA: bit-identical to the CPU oracle;
+//   fast   fused multiply-adds and the hardware reciprocal square root
+//          (v_rsq_f32) replace sqrt + divide in the neighbour loops.
+// The neighbour-set test is the exact one in both modes.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "lus_device.hpp"
+#include "lus_internal.hpp"
+
+namespace lus {
+
+constexpr int kOvf = 1 << 30;  // keys are < 2^30 (checked at init)
+
+__device__ __forceinline__ float4 pbf_predict(float4 x, float4 v, const PbfK& k, float dt) {
+    float vx = v.x + k.g[0] * dt;
+    float vy = v.y + k.g[1] * dt;
+    float vz = v.z + k.g[2] * dt;
+    float4 r;
+    r.x = rclamp(x.x + vx * dt, k.lo[0], k.hi[0]);
+    r.y = rclamp(x.y + vy * dt, k.lo[1], k.hi[1]);
+    r.z = rclamp(x.z + vz * dt, k.lo[2], k.hi[2]);
+    r.w = 0.0f;
+    return r;
+}
+
+// first slot and length of the 9 stencil rows of cell `key` (0 for rows off the grid)
+__device__ __forceinline__ int stencil_rows(int key, const int* __restrict__ cs, const PbfK& k, int* j0,
+                                            int* ln) {
+    const int cy = key / k.gXZ;
+    const int rm = key - cy * k.gXZ;
+    const int cx = rm / k.gZ;
+    const int cz = rm - cx * k.gZ;
+    const int z0 = cz > 0 ? cz - 1 : 0;
+    const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
+    int maxlen = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int y = cy + r / 3 - 1;
+        const int x = cx + r % 3 - 1;
+        j0[r] = 0;
+        ln[r] = 0;
+        if (y >= 0 && y < k.gY && x >= 0 && x < k.gX) {
+            const int row = y * k.gXZ + x * k.gZ;
+            j0[r] = cs[row + z0];
+            ln[r] = cs[row + z1 + 1] - j0[r];
+        }
+        maxlen = ln[r] > maxlen ? ln[r] : maxlen;
+    }
+    return maxlen;
+}
+
+// the exact neighbour test (no FMA, oracle order)
+__device__ __forceinline__ bool within_h(const float4 a, const float4 b, const PbfK& k) {
+    const float dx = a.x - b.x;
+    const float dy = a.y - b.y;
+    const float dz = a.z - b.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    return r2 <= k.h2;
+}
+
+// Visit N(i) in ascending slot order, calling f(j, cur[j], aux[j]).  build:
+// test the candidates (positions xp) and store the masks; otherwise load them.
+// The data of the next neighbour is loaded before f runs on the current one.
+template <bool kBuild, class F>
+__device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict__ key2,
+                                                  const int* __restrict__ cs,
+                                                  unsigned long long* __restrict__ masks,
+                                                  const float4* __restrict__ xp,
+                                                  const float4* __restrict__ cur,
+                                                  const float4* __restrict__ aux, const PbfK& k,
+                                                  F&& f) {
+    const int kw = key2[i];
+    int j0[9], ln[9];
+    const int maxlen = stencil_rows(kw & ~kOvf, cs, k, j0, ln);
+    const bool ovf = kBuild ? (maxlen > 64) : ((kw & kOvf) != 0);
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (ovf) {
+        if (kBuild) key2[i] = kw | kOvf;
+        const float4 pi = xp[i];
+#pragma unroll
+        for (int r = 0; r < 9; ++r)
+            for (int t = 0; t < ln[r]; ++t) {
+                const int j = j0[r] + t;
+                if (within_h(pi, xp[j], k)) f(j, cur[j], aux ? aux[j] : zero);
+            }
+        return;
+    }
+    unsigned long long m[9];
+    if (kBuild) {
+        const float4 pi = xp[i];
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            unsigned long long mm = 0ull;
+            const float4* __restrict__ base = xp + j0[r];
+            const int L = ln[r];
+            int t = 0;
+            for (; t + 4 <= L; t += 4) {
+                const float4 q0 = base[t], q1 = base[t + 1], q2 = base[t + 2], q3 = base[t + 3];
+                mm |= within_h(pi, q0, k) ? (1ull << t) : 0ull;
+                mm |= within_h(pi, q1, k) ? (2ull << t) : 0ull;
+                mm |= within_h(pi, q2, k) ? (4ull << t) : 0ull;
+                mm |= within_h(pi, q3, k) ? (8ull << t) : 0ull;
+            }
+            for (; t < L; ++t) mm |= within_h(pi, base[t], k) ? (1ull << t) : 0ull;
+            m[r] = mm;
+            masks[(size_t)r * n + i] = mm;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 9; ++r) m[r] = masks[(size_t)r * n + i];
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        unsigned long long mm = m[r];
+        if (!mm) continue;
+        const int base = j0[r];
+        int j = base + __builtin_ctzll(mm);
+        mm &= mm - 1ull;
+        float4 pj = cur[j];
+        float4 aj = aux ? aux[j] : zero;
+        while (true) {
+            const bool more = mm != 0ull;
+            int jn = j;
+            float4 pn = pj, an = aj;
+            if (more) {
+                jn = base + __builtin_ctzll(mm);
+                mm &= mm - 1ull;
+                pn = cur[jn];
+                an = aux ? aux[jn] : zero;
+            }
+            f(j, pj, aj);
+            if (!more) break;
+            j = jn;
+            pj = pn;
+            aj = an;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pbf_key(const float4* __restrict__ x,
+                                                     const float4* __restrict__ v, int n, PbfK k,
+                                                     float dt, int* __restrict__ key,
+                                                     int* __restrict__ slot,
+                                                     int* __restrict__ counts) {
+    const int i = xcd_index();
+    const bool active = i < n;
+    int c = 0;
+    if (active) {
+        const float4 xs = pbf_predict(x[i], v[i], k, dt);
+        c = cell_of(xs.x, xs.y, xs.z, k.cs, k.gX, k.gZ);
+        key[i] = c;
+    }
+    const int sl = wave_atomic_slot(counts, c, active);
+    if (active) slot[i] = sl;
+}
+
+hipError_t launch_pbf_key(const float4* x, const float4* v, int n, PbfK k, float dt, int* key,
+                          int* slot, int* counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_pbf_key, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, n, k, dt, key, slot,
+                       counts);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_pbf_permute(
+    const int* __restrict__ tmp, const int* __restrict__ key, const int* __restrict__ cell_start,
+    const float4* __restrict__ x, const float4* __restrict__ v, const int* __restrict__ id, int n,
+    PbfK k, float dt, float4* __restrict__ x2, float4* __restrict__ xp, int* __restrict__ id2,
+    int* __restrict__ key2) {
+    const int p = xcd_index();
+    if (p >= n) return;
+    const int i = tmp[p];
+    const int c = key[i];
+    const int o = stable_slot(tmp, cell_start, c, i);
+    const float4 xi = x[i];
+    x2[o] = xi;
+    xp[o] = pbf_predict(xi, v[i], k, dt);
+    id2[o] = id[i];
+    key2[o] = c;
+}
+
+hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_start,
+                              const float4* x, const float4* v, const int* id, int n, PbfK k,
+                              float dt, float4* x2, float4* xp, int* id2, int* key2,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_pbf_permute, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start,
+                       x, v, id, n, k, dt, x2, xp, id2, key2);
+    return hipGetLastError();
+}
+
+// s(r) = SPf (hf - f r)^2 / r of the spiky gradient, with r = |d|
+template <bool kFast>
+__device__ __forceinline__ float spiky_s(float r2, const PbfK& k) {
+    if (kFast) {
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float rl = r2 * rinv;
+        const float g = __builtin_fmaf(-k.f, rl, k.hf);
+        return (k.SPf * g) * (g * rinv);
+    } else {
+        const float rl = sqrtf(r2);
+        const float g = k.hf - k.f * rl;
+        return ((k.SPf * g) * g) / rl;
+    }
+}
+
+template <bool kFast>
+__device__ __forceinline__ float dist2(float dx, float dy, float dz) {
+    if (kFast) return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    return dx * dx + dy * dy + dz * dz;
+}
+
+template <bool kFast>
+__device__ __forceinline__ float acc(float sum, float a, float b) {  // sum + a * b
+    if (kFast) return __builtin_fmaf(a, b, sum);
+    return sum + a * b;
+}
+
+template <bool kFast>
+__device__ __forceinline__ float poly_t(float r2, const PbfK& k) {  // hf^2 - f^2 r^2
+    if (kFast) return __builtin_fmaf(-k.f2, r2, k.hf2);
+    return k.hf2 - k.f2 * r2;
+}
+
+// lambda pass: density constraint and its gradient norm (PBF eq. 1, 8, 11)
+template <bool kBuild, bool kFast>
+__global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict__ xa,
+                                                        int* __restrict__ key2,
+                                                        const int* __restrict__ cs, int n, PbfK k,
+                                                        float4* __restrict__ xl,
+                                                        int* __restrict__ maxc,
+                                                        unsigned long long* __restrict__ masks,
+                                                        const float4* __restrict__ xp) {
+    const int i = xcd_index();
+    float Ci = -1e30f;
+    if (i < n) {
+        const float4 pi = xa[i];
+        float rho = 0.0f, a = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        for_each_neighbor<kBuild>(i, n, key2, cs, masks, xp, xa, nullptr, k,
+                                  [&](int, const float4 pj, const float4) {
+            const float dx = pi.x - pj.x;
+            const float dy = pi.y - pj.y;
+            const float dz = pi.z - pj.z;
+            const float r2 = dist2<kFast>(dx, dy, dz);
+            if (r2 <= k.h2) {
+                const float t = poly_t<kFast>(r2, k);
+                rho = acc<kFast>(rho, t * t, t);
+                if (r2 > 0.0f) {
+                    const float s = spiky_s<kFast>(r2, k);
+                    a = acc<kFast>(a, s * s, r2);
+                    gx = acc<kFast>(gx, s, dx);
+                    gy = acc<kFast>(gy, s, dy);
+                    gz = acc<kFast>(gz, s, dz);
+                }
+            }
+        });
+        Ci = rho * k.c_rho - 1.0f;
+        const float gg = gx * gx + gy * gy + gz * gz;
+        const float den = k.mr0sq * (a + gg) + k.eps;
+        xl[i] = make_float4(pi.x, pi.y, pi.z, -Ci / den);
+    }
+    // block max of C, one plain store per block (blockmax[blockIdx.x]); the
+    // consumer reduces the few thousand block values when it needs them
+    __shared__ int wmax[kBlock / 64];
+    int m = ordered(Ci);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        int o = __shfl_xor(m, d, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = wmax[0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; ++w) b = wmax[w] > b ? wmax[w] : b;
+        maxc[blockIdx.x] = b;
+    }
+}
+
+hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
+                             int* maxc, unsigned long long* masks, const float4* xp, int build,
+                             hipStream_t s) {
+    const dim3 g(grid_for(n)), b(kBlock);
+    if (build && k.exact)
+        hipLaunchKernelGGL((k_pbf_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp);
+    else if (build)
+        hipLaunchKernelGGL((k_pbf_lambda<true, true>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp);
+    else if (k.exact)
+        hipLaunchKernelGGL((k_pbf_lambda<false, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp);
+    else
+        hipLaunchKernelGGL((k_pbf_lambda<false, true>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp);
+    return hipGetLastError();
+}
+
+// delta-p pass with s_corr (PBF eq. 12-14), Jacobi update, box clamp
+template <bool kFast>
+__global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict__ xl,
+                                                        int* __restrict__ key2,
+                                                        const int* __restrict__ cs, int n, PbfK k,
+                                                        float4* __restrict__ xa,
+                                                        const float4* __restrict__ xold,
+                                                        float4* __restrict__ vout, float dt,
+                                                        unsigned long long* __restrict__ masks,
+                                                        const float4* __restrict__ xp) {
+    const int i = xcd_index();
+    if (i >= n) return;
+    const float4 pi = xl[i];
+    const float li = pi.w;
+    const bool n4 = k.n_scorr == 4;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k,
+                             [&](int, const float4 pj, const float4) {
+        const float dx = pi.x - pj.x;
+        const float dy = pi.y - pj.y;
+        const float dz = pi.z - pj.z;
+        const float r2 = dist2<kFast>(dx, dy, dz);
+        if (r2 <= k.h2 && r2 > 0.0f) {
+            const float s = spiky_s<kFast>(r2, k);
+            const float t = poly_t<kFast>(r2, k);
+            const float ratio = ((t * t) * t) * k.inv_tq3;
+            float pw;
+            if (n4) {
+                pw = ratio * ratio;
+                pw = pw * ratio;
+                pw = pw * ratio;
+            } else {
+                pw = ratio;
+                for (int e = 1; e < k.n_scorr; ++e) pw = pw * ratio;
+            }
+            const float sc = -k.scorr_k * pw;
+            const float coef = (li + pj.w) + sc;
+            const float c = coef * s;
+            ax = acc<kFast>(ax, c, dx);
+            ay = acc<kFast>(ay, c, dy);
+            az = acc<kFast>(az, c, dz);
+        }
+    });
+    float4 r;
+    r.x = rclamp(pi.x + ax * k.neg_inv_r0, k.lo[0], k.hi[0]);
+    r.y = rclamp(pi.y + ay * k.neg_inv_r0, k.lo[1], k.hi[1]);
+    r.z = rclamp(pi.z + az * k.neg_inv_r0, k.lo[2], k.hi[2]);
+    r.w = 0.0f;
+    xa[i] = r;
+    if (vout) {
+        const float4 xo = xold[i];
+        vout[i] = make_float4((r.x - xo.x) / dt, (r.y - xo.y) / dt, (r.z - xo.z) / dt, 0.0f);
+    }
+}
+
+hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
+                             const float4* xold, float4* vout, float dt, unsigned long long* masks,
+                             const float4* xp, hipStream_t s) {
+    const dim3 g(grid_for(n)), b(kBlock);
+    if (k.exact)
+        hipLaunchKernelGGL((k_pbf_deltap<false>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+    else
+        hipLaunchKernelGGL((k_pbf_deltap<true>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_pbf_velocity(const float4* __restrict__ xa,
+                                                          const float4* __restrict__ xold, int n,
+                                                          float dt, float4* __restrict__ vout) {
+    const int i = xcd_index();
+    if (i >= n) return;
+    const float4 r = xa[i];
+    const float4 xo = xold[i];
+    vout[i] = make_float4((r.x - xo.x) / dt, (r.y - xo.y) / dt, (r.z - xo.z) / dt, 0.0f);
+}
+
+hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, float dt,
+                               float4* vout, hipStream_t s) {
+    hipLaunchKernelGGL(k_pbf_velocity, dim3(grid_for(n)), dim3(kBlock), 0, s, xa, xold, n, dt, vout);
+    return hipGetLastError();
+}
+
+// XSPH viscosity (PBF eq. 17) over N(i) and commit x = x*
+template <bool kFast>
+__global__ __launch_bounds__(kBlock) void k_pbf_xsph(const float4* __restrict__ xa,
+                                                      const float4* __restrict__ vt,
+                                                      int* __restrict__ key2,
+                                                      const int* __restrict__ cs, int n, PbfK k,
+                                                      float4* __restrict__ vout,
+                                                      float4* __restrict__ xout,
+                                                      unsigned long long* __restrict__ masks,
+                                                      const float4* __restrict__ xp) {
+    const int i = xcd_index();
+    if (i >= n) return;
+    const float4 pi = xa[i];
+    const float4 vi = vt[i];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for_each_neighbor<false>(i, n, key2, cs, masks, xp, xa, vt, k,
+                             [&](int, const float4 pj, const float4 vj) {
+        const float dx = pi.x - pj.x;
+        const float dy = pi.y - pj.y;
+        const float dz = pi.z - pj.z;
+        const float r2 = dist2<kFast>(dx, dy, dz);
+        if (r2 <= k.h2) {
+            const float t = poly_t<kFast>(r2, k);
+            const float w = (t * t) * t;
+            sx = acc<kFast>(sx, vj.x - vi.x, w);
+            sy = acc<kFast>(sy, vj.y - vi.y, w);
+            sz = acc<kFast>(sz, vj.z - vi.z, w);
+        }
+    });
+    vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
+    xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
+}
+
+hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
+                           PbfK k, float4* vout, float4* xout, unsigned long long* masks,
+                           const float4* xp, hipStream_t s) {
+    const dim3 g(grid_for(n)), b(kBlock);
+    if (k.exact)
+        hipLaunchKernelGGL((k_pbf_xsph<false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp);
+    else
+        hipLaunchKernelGGL((k_pbf_xsph<true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp);
+    return hipGetLastError();
+}
+
+}  // namespace lus

@@ -13,8 +13,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblustrine_gpu.so")
 
-NPHASES = 7
-PHASES = ("predict_key", "cell_scan", "scatter", "stable_permute", "lambda", "deltap", "xsph")
+NPHASES = 8
+PHASES = ("predict_key", "cell_scan", "scatter", "stable_permute", "lambda", "deltap", "xsph",
+          "lambda_build")
 
 
 class LusError(RuntimeError):
@@ -32,7 +33,7 @@ class LusPbfParams(C.Structure):
         ("lo", C.c_float * 3), ("hi", C.c_float * 3), ("gravity", C.c_float * 3),
         ("mass", C.c_float), ("rest_density", C.c_float), ("epsilon", C.c_float),
         ("scorr_k", C.c_float), ("scorr_dq", C.c_float), ("scorr_n", C.c_int),
-        ("c_xsph", C.c_float), ("iterations", C.c_int),
+        ("c_xsph", C.c_float), ("iterations", C.c_int), ("exact", C.c_int),
     ]
 
 
@@ -69,6 +70,7 @@ _SIGS = {
     "lus_pbf_download": (C.c_int, [C.c_void_p, C.c_int, _f, _f, _i]),
     "lus_pbf_last_max_constraint": (C.c_int, [C.c_void_p, _f]),
     "lus_pbf_bench": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, _f, _f, _i]),
+    "lus_pbf_time_kernels": (C.c_int, [C.c_void_p, C.c_int, _f]),
     "lus_sand_init": (C.c_int, [C.c_void_p, _P(LusSandParams), C.c_int, _f, _f, _i, C.c_int, _f, _i]),
     "lus_sand_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, _f]),
     "lus_sand_download": (C.c_int, [C.c_void_p, _f, _f, _f, _i, _i]),
@@ -173,8 +175,9 @@ def neighbors_fluid(g: LusGrid, kernel_radius, sand_pos, sand_start=0, solid_pos
 
 
 # ------------------------------------------------------------------ params
-def pbf_params(cfg, iterations=None) -> LusPbfParams:
-    """LusPbfParams of a scenes.PbfConfig."""
+def pbf_params(cfg, iterations=None, exact=True) -> LusPbfParams:
+    """LusPbfParams of a scenes.PbfConfig.  exact=True: operation order of
+    the CPU oracle (bit-identical results); False: fused fast path."""
     p = LusPbfParams()
     r = np.float32(cfg.radius)
     p.h = cfg.h
@@ -193,6 +196,7 @@ def pbf_params(cfg, iterations=None) -> LusPbfParams:
     p.scorr_n = cfg.scorr_n
     p.c_xsph = cfg.c_xsph
     p.iterations = cfg.iterations if iterations is None else iterations
+    p.exact = 1 if exact else 0
     return p
 
 
@@ -283,6 +287,13 @@ class Gpu:
                "lus_pbf_bench")
         return ms.value, (dict(zip(PHASES, ph.tolist())) if phases else None), \
             (dict(zip(PHASES, nl.tolist())) if phases else None)
+
+    def pbf_time_kernels(self, reps=20):
+        """Average us per launch of the lambda, delta-p and XSPH kernels."""
+        us = np.zeros(4, np.float32)
+        _check(self._lib.lus_pbf_time_kernels(self._h, reps, _ptr_f(us)), "lus_pbf_time_kernels")
+        return {"lambda": float(us[0]), "deltap": float(us[1]), "xsph": float(us[2]),
+                "lambda_build": float(us[3])}
 
     # --- sand
     def sand_init(self, params: LusSandParams, pos, vel=None, attracted=None, solid_pos=None,
