@@ -122,9 +122,11 @@ def main(argv=None):
         ms, _, _ = g.pbf_bench(dt, args.warmup, args.steps, phases=False)
         maxc = g.pbf_last_max_constraint()
         phase_ms = launches = None
+        copy_gbps = None
         if not args.no_phases:
             g.pbf_init(params, x0, None)
             _, phase_ms, launches = g.pbf_bench(dt, args.warmup, args.steps, phases=True)
+            copy_gbps, _ = g.stream_copy_gbps(1 << 30, 10)
     ms_per_step = ms / args.steps
     value = cfg.n * args.steps / (ms / 1e3)
     step_bytes = (STEP_BYTES_K4 - 4 * 72 + cfg.iterations * 72) * cfg.n
@@ -175,7 +177,8 @@ def main(argv=None):
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
                            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                            "avg_launch_us": d["avg_launch_us"],
-                           "alg_bytes_per_launch": d["alg_bytes_per_launch"]}
+                           "alg_bytes_per_launch": d["alg_bytes_per_launch"],
+                           "measured_copy_GBps": round(copy_gbps, 1)}
         out["phases"] = phases
     if not args.no_cpu_baseline:
         cb = cpu_baseline(cfg, args.cpu_seconds)

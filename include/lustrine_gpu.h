@@ -110,6 +110,11 @@ int lus_neighbors_fluid(const LusGrid* g, float kernel_radius,
 /* ------------------------------------------------------------------ */
 typedef struct LusGpu LusGpu;
 
+/* Measured HBM ceiling: copies `bytes` (multiple of 16) between two device
+ * buffers `reps` times with a float4 copy kernel (k_stream_copy) on the
+ * context stream; *us = average microseconds per copy.  Read + write traffic
+ * per copy is 2 * bytes.  No reference counterpart (measurement support). */
+int  lus_gpu_stream_copy(LusGpu* ctx, size_t bytes, int reps, float* us);
 int  lus_gpu_create(LusGpu** out, int device);
 void lus_gpu_destroy(LusGpu* ctx);
 int  lus_gpu_sync(LusGpu* ctx);

@@ -285,6 +285,33 @@ void lus_gpu_destroy(LusGpu* c) {
     delete c;
 }
 
+int lus_gpu_stream_copy(LusGpu* c, size_t bytes, int reps, float* us) {
+    if (!c || !us || reps <= 0 || bytes == 0 || bytes % 16 != 0)
+        return set_error(LUS_EINVAL, "lus_gpu_stream_copy: bad arguments");
+    LUS_HIP(hipSetDevice(c->device));
+    Arena m;
+    float4 *a, *b;
+    const size_t n4 = bytes / 16;
+    LUS_HIP(m.alloc(&a, n4));
+    LUS_HIP(m.alloc(&b, n4));
+    LUS_HIP(hipMemset(a, 0, bytes));
+    LUS_HIP(hipDeviceSynchronize());
+    LUS_HIP(lus::launch_stream_copy(a, b, n4, c->stream));  // warm
+    hipEvent_t e0, e1;
+    LUS_HIP(hipEventCreate(&e0));
+    LUS_HIP(hipEventCreate(&e1));
+    LUS_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) LUS_HIP(lus::launch_stream_copy(a, b, n4, c->stream));
+    LUS_HIP(hipEventRecord(e1, c->stream));
+    LUS_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    LUS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *us = ms * 1e3f / (float)reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return LUS_OK;
+}
+
 int lus_gpu_sync(LusGpu* c) {
     if (!c) return set_error(LUS_EINVAL, "lus_gpu_sync: NULL context");
     LUS_HIP(hipStreamSynchronize(c->stream));

@@ -51,6 +51,7 @@ hipError_t launch_stable_rank(const int* tmp, const int* key, const int* cell_st
                               int* perm, int* key_sorted, hipStream_t s);
 hipError_t launch_gather4(const float4* src, const int* perm, int n, float4* dst, hipStream_t s);
 hipError_t launch_gather1(const int* src, const int* perm, int n, int* dst, hipStream_t s);
+hipError_t launch_stream_copy(const float4* src, float4* dst, size_t n4, hipStream_t s);
 
 // fluid-path neighbor lists (parity entry point)
 hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, const int* sand_cs,

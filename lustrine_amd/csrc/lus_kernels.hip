@@ -212,6 +212,30 @@ hipError_t launch_gather1(const int* src, const int* perm, int n, int* dst, hipS
     return hipGetLastError();
 }
 
+// plain float4 copy: the HBM ceiling of a read + write stream.  Each thread
+// moves 4 float4 per block-tile (4 loads in flight before the stores).
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const float4* __restrict__ src,
+                                                         float4* __restrict__ dst, size_t n4) {
+    const size_t tile = (size_t)kBlock * 4;
+    for (size_t b = (size_t)blockIdx.x * tile; b < n4; b += (size_t)gridDim.x * tile) {
+        const size_t i0 = b + threadIdx.x;
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u * kBlock < n4) v[u] = src[i0 + u * kBlock];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u * kBlock < n4) dst[i0 + u * kBlock] = v[u];
+    }
+}
+
+hipError_t launch_stream_copy(const float4* src, float4* dst, size_t n4, hipStream_t s) {
+    size_t blocks = (n4 + (size_t)kBlock * 4 - 1) / ((size_t)kBlock * 4);
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(kBlock), 0, s, src, dst, n4);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ fluid lists
 // find_neighbors_uniform_grid (src/neighbors/Neighbors.cpp:366-450): for the
 // particle with original slot i, neighbour cells in (dy, dx, dz) order, each

@@ -49,32 +49,6 @@ __device__ __forceinline__ float4 pbf_predict(float4 x, float4 v, const PbfK& k,
     return r;
 }
 
-// first slot and length of the 9 stencil rows of cell `key` (0 for rows off the grid)
-__device__ __forceinline__ int stencil_rows(int key, const int* __restrict__ cs, const PbfK& k, int* j0,
-                                            int* ln) {
-    const int cy = key / k.gXZ;
-    const int rm = key - cy * k.gXZ;
-    const int cx = rm / k.gZ;
-    const int cz = rm - cx * k.gZ;
-    const int z0 = cz > 0 ? cz - 1 : 0;
-    const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
-    int maxlen = 0;
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        const int y = cy + r / 3 - 1;
-        const int x = cx + r % 3 - 1;
-        j0[r] = 0;
-        ln[r] = 0;
-        if (y >= 0 && y < k.gY && x >= 0 && x < k.gX) {
-            const int row = y * k.gXZ + x * k.gZ;
-            j0[r] = cs[row + z0];
-            ln[r] = cs[row + z1 + 1] - j0[r];
-        }
-        maxlen = ln[r] > maxlen ? ln[r] : maxlen;
-    }
-    return maxlen;
-}
-
 // the exact neighbour test (no FMA, oracle order)
 __device__ __forceinline__ bool within_h(const float4 a, const float4 b, const PbfK& k) {
     const float dx = a.x - b.x;
@@ -84,9 +58,38 @@ __device__ __forceinline__ bool within_h(const float4 a, const float4 b, const P
     return r2 <= k.h2;
 }
 
+__device__ __forceinline__ int low_bit(unsigned int m) { return __builtin_ctz(m); }
+__device__ __forceinline__ int low_bit(unsigned long long m) { return __builtin_ctzll(m); }
+
+// Visit the set bits of one row mask in ascending order, calling
+// f(j, cur[j], aux[j]); the next neighbour's data is loaded before f runs on
+// the current one.
+template <typename M, class F>
+__device__ __forceinline__ void visit_row(M mm, int base, const float4* __restrict__ cur,
+                                          const float4* __restrict__ aux, F& f) {
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (!mm) return;
+    int j = base + low_bit(mm);
+    mm &= mm - (M)1;
+    float4 pa = cur[j];
+    float4 aa = aux ? aux[j] : zero;
+    while (mm) {
+        const int jn = base + low_bit(mm);
+        mm &= mm - (M)1;
+        const float4 pb = cur[jn];
+        const float4 ab = aux ? aux[jn] : zero;
+        f(j, pa, aa);
+        j = jn;
+        pa = pb;
+        aa = ab;
+    }
+    f(j, pa, aa);
+}
+
 // Visit N(i) in ascending slot order, calling f(j, cur[j], aux[j]).  build:
 // test the candidates (positions xp) and store the masks; otherwise load them.
-// The data of the next neighbour is loaded before f runs on the current one.
+// A row whose mask has no bit above 31 in any lane of the wave is walked with
+// 32-bit mask arithmetic.
 template <bool kBuild, class F>
 __device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict__ key2,
                                                   const int* __restrict__ cs,
@@ -96,29 +99,50 @@ __device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict_
                                                   const float4* __restrict__ aux, const PbfK& k,
                                                   F&& f) {
     const int kw = key2[i];
-    int j0[9], ln[9];
-    const int maxlen = stencil_rows(kw & ~kOvf, cs, k, j0, ln);
-    const bool ovf = kBuild ? (maxlen > 64) : ((kw & kOvf) != 0);
-    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (ovf) {
-        if (kBuild) key2[i] = kw | kOvf;
-        const float4 pi = xp[i];
-#pragma unroll
-        for (int r = 0; r < 9; ++r)
-            for (int t = 0; t < ln[r]; ++t) {
-                const int j = j0[r] + t;
-                if (within_h(pi, xp[j], k)) f(j, cur[j], aux ? aux[j] : zero);
-            }
-        return;
-    }
-    unsigned long long m[9];
+    const int key = kw & ~kOvf;
+    const int cy = key / k.gXZ;
+    const int rm = key - cy * k.gXZ;
+    const int cx = rm / k.gZ;
+    const int cz = rm - cx * k.gZ;
+    const int z0 = cz > 0 ? cz - 1 : 0;
+    const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
+    bool ovf;
     if (kBuild) {
-        const float4 pi = xp[i];
+        int maxlen = 0;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
-            unsigned long long mm = 0ull;
-            const float4* __restrict__ base = xp + j0[r];
-            const int L = ln[r];
+            const int y = cy + r / 3 - 1;
+            const int x = cx + r % 3 - 1;
+            if (y >= 0 && y < k.gY && x >= 0 && x < k.gX) {
+                const int row = y * k.gXZ + x * k.gZ;
+                const int l = cs[row + z1 + 1] - cs[row + z0];
+                maxlen = l > maxlen ? l : maxlen;
+            }
+        }
+        ovf = maxlen > 64;
+        if (ovf) key2[i] = kw | kOvf;
+    } else {
+        ovf = (kw & kOvf) != 0;
+    }
+    const float4 pi = (kBuild || ovf) ? xp[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        const int y = cy + r / 3 - 1;
+        const int x = cx + r % 3 - 1;
+        if (y < 0 || y >= k.gY || x < 0 || x >= k.gX) continue;
+        const int row = y * k.gXZ + x * k.gZ;
+        const int j0 = cs[row + z0];
+        const int j1 = cs[row + z1 + 1];
+        if (ovf) {
+            for (int j = j0; j < j1; ++j)
+                if (within_h(pi, xp[j], k)) f(j, cur[j], aux ? aux[j] : make_float4(0.f, 0.f, 0.f, 0.f));
+            continue;
+        }
+        unsigned long long mm;
+        if (kBuild) {
+            mm = 0ull;
+            const float4* __restrict__ base = xp + j0;
+            const int L = j1 - j0;
             int t = 0;
             for (; t + 4 <= L; t += 4) {
                 const float4 q0 = base[t], q1 = base[t + 1], q2 = base[t + 2], q3 = base[t + 3];
@@ -128,38 +152,14 @@ __device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict_
                 mm |= within_h(pi, q3, k) ? (8ull << t) : 0ull;
             }
             for (; t < L; ++t) mm |= within_h(pi, base[t], k) ? (1ull << t) : 0ull;
-            m[r] = mm;
             masks[(size_t)r * n + i] = mm;
+        } else {
+            mm = masks[(size_t)r * n + i];
         }
-    } else {
-#pragma unroll
-        for (int r = 0; r < 9; ++r) m[r] = masks[(size_t)r * n + i];
-    }
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        unsigned long long mm = m[r];
-        if (!mm) continue;
-        const int base = j0[r];
-        int j = base + __builtin_ctzll(mm);
-        mm &= mm - 1ull;
-        float4 pj = cur[j];
-        float4 aj = aux ? aux[j] : zero;
-        while (true) {
-            const bool more = mm != 0ull;
-            int jn = j;
-            float4 pn = pj, an = aj;
-            if (more) {
-                jn = base + __builtin_ctzll(mm);
-                mm &= mm - 1ull;
-                pn = cur[jn];
-                an = aux ? aux[jn] : zero;
-            }
-            f(j, pj, aj);
-            if (!more) break;
-            j = jn;
-            pj = pn;
-            aj = an;
-        }
+        if (__any((int)((mm >> 32) != 0ull)))
+            visit_row<unsigned long long>(mm, j0, cur, aux, f);
+        else
+            visit_row<unsigned int>((unsigned int)mm, j0, cur, aux, f);
     }
 }
 
@@ -317,8 +317,9 @@ hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, 
     return hipGetLastError();
 }
 
-// delta-p pass with s_corr (PBF eq. 12-14), Jacobi update, box clamp
-template <bool kFast>
+// delta-p pass with s_corr (PBF eq. 12-14), Jacobi update, box clamp.
+// kNS: s_corr exponent fixed at compile time (4), or 0 for k.n_scorr.
+template <bool kFast, int kNS>
 __global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict__ xl,
                                                         int* __restrict__ key2,
                                                         const int* __restrict__ cs, int n, PbfK k,
@@ -331,7 +332,6 @@ __global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict_
     if (i >= n) return;
     const float4 pi = xl[i];
     const float li = pi.w;
-    const bool n4 = k.n_scorr == 4;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k,
                              [&](int, const float4 pj, const float4) {
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict_
             const float t = poly_t<kFast>(r2, k);
             const float ratio = ((t * t) * t) * k.inv_tq3;
             float pw;
-            if (n4) {
+            if (kNS == 4) {
                 pw = ratio * ratio;
                 pw = pw * ratio;
                 pw = pw * ratio;
@@ -376,10 +376,15 @@ hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, 
                              const float4* xold, float4* vout, float dt, unsigned long long* masks,
                              const float4* xp, hipStream_t s) {
     const dim3 g(grid_for(n)), b(kBlock);
-    if (k.exact)
-        hipLaunchKernelGGL((k_pbf_deltap<false>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+    const bool n4 = k.n_scorr == 4;
+    if (k.exact && n4)
+        hipLaunchKernelGGL((k_pbf_deltap<false, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+    else if (k.exact)
+        hipLaunchKernelGGL((k_pbf_deltap<false, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+    else if (n4)
+        hipLaunchKernelGGL((k_pbf_deltap<true, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
     else
-        hipLaunchKernelGGL((k_pbf_deltap<true>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+        hipLaunchKernelGGL((k_pbf_deltap<true, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
     return hipGetLastError();
 }
 

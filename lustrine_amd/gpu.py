@@ -62,6 +62,7 @@ _SIGS = {
     "lus_counting_sort": (C.c_int, [_i, _i, C.c_size_t, C.c_size_t, _i]),
     "lus_neighbors_fluid": (C.c_int, [_P(LusGrid), C.c_float, C.c_int, _f, C.c_int, C.c_int, _f, _i,
                                       _l, _i, C.c_long, _l]),
+    "lus_gpu_stream_copy": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, _f]),
     "lus_gpu_create": (C.c_int, [_P(C.c_void_p), C.c_int]),
     "lus_gpu_destroy": (None, [C.c_void_p]),
     "lus_gpu_sync": (C.c_int, [C.c_void_p]),
@@ -253,6 +254,12 @@ class Gpu:
 
     def sync(self):
         _check(self._lib.lus_gpu_sync(self._h), "lus_gpu_sync")
+
+    def stream_copy_gbps(self, nbytes=1 << 30, reps=20):
+        """Measured HBM ceiling: GB/s of read + write of a float4 copy."""
+        us = C.c_float(0)
+        _check(self._lib.lus_gpu_stream_copy(self._h, nbytes, reps, C.byref(us)), "lus_gpu_stream_copy")
+        return 2.0 * nbytes / (us.value * 1e-6) / 1e9, us.value
 
     # --- PBF
     def pbf_init(self, params: LusPbfParams, pos, vel=None):
