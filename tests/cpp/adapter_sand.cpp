@@ -1,0 +1,111 @@
+// Drives lus_adapter::simulate_sand / simulate_pbf through a stand-in with the
+// member names and layout of Lustrine::Simulation (src/Simulation.hpp:135-251)
+// that the adapter reads.  Input/output: raw little-endian files.
+//   adapter_sand.exe sand <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
+//   adapter_sand.exe pbf  <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
+// in:  int32 n_sand, int32 n_solid, int32 total, float32[3*n_sand] sand,
+//      float32[3*n_solid] solids (written downward from slot total-1)
+// out: float32[3*n] positions, float32[3*n] velocities, int32[n] attracted
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lustrine_gpu_simulate.hpp"
+
+namespace stand_in {
+struct vec3 {
+    float x, y, z;
+};
+struct BulletSim {
+    vec3 player_position{0.0f, 0.0f, 0.0f};
+};
+struct Simulation {  // members read by the adapter, with their reference types
+    BulletSim bullet_physics_simulation;
+    float particleRadius, particleDiameter, kernelRadius, kernelFactor = 0.5f;
+    float domainX, domainY, domainZ;
+    float rest_density = 24.0f, mass = 5.0f;
+    vec3 gravity{0.0f, -10.0f, 0.0f};
+    float time_step = 0.01f, relaxation_epsilon = 10.0f;
+    float s_corr_dq = 0.5f, s_corr_k = 1.0f, s_corr_n = 4, c_xsph = 0.1f;
+    vec3* velocities = nullptr;
+    int gridX, gridY, gridZ;
+    float cell_size;
+    bool computed_static_particles = false;
+    vec3* positions = nullptr;
+    vec3* positions_star = nullptr;
+    int ptr_sand_start = -1, ptr_sand_end = -1, ptr_solid_start = -1, ptr_solid_end = -1;
+    int num_sand_particles = 0;
+    bool first_iteration = true;
+    bool attract_flag = false, blow_flag = false;
+    int* attracted = nullptr;
+    float attract_radius = 1.5f, blow_radius = 2.0f, attract_coeff = 1000.0f, blow_coeff = 500.0f;
+};
+}  // namespace stand_in
+
+int main(int argc, char** argv) {
+    if (argc != 9) {
+        fprintf(stderr, "usage: %s sand|pbf in out steps X Y Z scale\n", argv[0]);
+        return 2;
+    }
+    const std::string mode = argv[1];
+    FILE* f = fopen(argv[2], "rb");
+    if (!f) return 2;
+    int hdr[3];
+    if (fread(hdr, sizeof(int), 3, f) != 3) return 2;
+    const int n = hdr[0], ns = hdr[1], total = hdr[2];
+    std::vector<stand_in::vec3> pos((size_t)total), ps((size_t)total), vel((size_t)total);
+    std::vector<int> att((size_t)total, 0);
+    std::vector<float> buf(3 * (size_t)(n + ns));
+    if (fread(buf.data(), sizeof(float), buf.size(), f) != buf.size()) return 2;
+    fclose(f);
+    for (int i = 0; i < n; ++i) pos[i] = {buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]};
+    for (int k = 0; k < ns; ++k) {
+        const float* q = &buf[3 * (size_t)(n + k)];
+        pos[total - 1 - k] = {q[0], q[1], q[2]};  // solids written downward (src/Lustrine.cpp:491-504)
+    }
+    ps = pos;
+    stand_in::Simulation s;
+    const int X = atoi(argv[5]), Y = atoi(argv[6]), Z = atoi(argv[7]);
+    const float scale = (float)atof(argv[8]);
+    s.particleRadius = 0.5f;
+    s.particleDiameter = 1.0f;
+    s.domainX = (float)X;
+    s.domainY = (float)Y;
+    s.domainZ = (float)Z;
+    s.kernelRadius = scale * s.particleRadius;  // src/Lustrine.cpp:545-546
+    s.cell_size = 1.0f * s.kernelRadius;
+    s.gridX = (int)(s.domainX / s.cell_size) + 1;
+    s.gridY = (int)(s.domainY / s.cell_size) + 1;
+    s.gridZ = (int)(s.domainZ / s.cell_size) + 1;
+    s.positions = pos.data();
+    s.positions_star = ps.data();
+    s.velocities = vel.data();
+    s.attracted = att.data();
+    s.ptr_sand_start = 0;
+    s.ptr_sand_end = n;
+    s.num_sand_particles = n;
+    s.ptr_solid_start = ns ? total - ns : total - 1;
+    s.ptr_solid_end = ns ? total : total - 1;
+    const int steps = atoi(argv[4]);
+    for (int t = 0; t < steps; ++t) {
+        const int rc = mode == "pbf" ? lus_adapter::simulate_pbf(&s, 1.0f / 60.0f, 4, 1)
+                                     : lus_adapter::simulate_sand(&s, 0.016f);
+        if (rc) {
+            char msg[512];
+            lus_gpu_last_error(msg, sizeof msg);
+            fprintf(stderr, "step %d failed: %s\n", t, msg);
+            return 1;
+        }
+    }
+    lus_adapter::release(&s);
+    FILE* o = fopen(argv[3], "wb");
+    if (!o) return 2;
+    fwrite(pos.data(), sizeof(float), 3 * (size_t)n, o);
+    fwrite(vel.data(), sizeof(float), 3 * (size_t)n, o);
+    fwrite(att.data(), sizeof(int), (size_t)n, o);
+    fclose(o);
+    return 0;
+}

@@ -1,0 +1,70 @@
+"""The Simulate_fun drop-in (include/lustrine_gpu_simulate.hpp) driven through a
+stand-in with the reference Simulation's member names and layout
+(tests/cpp/adapter_sand.cpp), compared with the oracle's simulate_sand and PBF
+step on the same scene."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from lustrine_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "adapter_sand.exe")
+
+
+def _run(mode, tmp_path, sand, solids, total, box, scale, steps):
+    inp = tmp_path / f"{mode}_in.bin"
+    out = tmp_path / f"{mode}_out.bin"
+    with open(inp, "wb") as f:
+        np.array([sand.shape[0], solids.shape[0], total], np.int32).tofile(f)
+        np.ascontiguousarray(sand, np.float32).tofile(f)
+        np.ascontiguousarray(solids, np.float32).tofile(f)
+    assert os.path.exists(EXE), "build() compiles tests/cpp/adapter_sand.exe"
+    r = subprocess.run([EXE, mode, str(inp), str(out), str(steps), *map(str, box), str(scale)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    n = sand.shape[0]
+    raw = np.fromfile(out, dtype=np.uint8)
+    pos = raw[: 12 * n].view(np.float32).reshape(n, 3)
+    vel = raw[12 * n: 24 * n].view(np.float32).reshape(n, 3)
+    att = raw[24 * n: 28 * n].view(np.int32)
+    return pos, vel, att
+
+
+def test_adapter_simulate_sand_matches_reference(gpu_mod, oracle, tmp_path):
+    box = (16, 16, 16)
+    sand = scenes.jittered_lattice(8, 8, 8, origin=(3.0, 2.0, 3.0), amplitude=0.2, seed=11)
+    solids = scenes.lattice(16, 1, 16)
+    s = oracle.init_constants(*box, 0.5, 1.0, 3.1)
+    st = oracle.SandStream(s, sand, solids)
+    for _ in range(3):
+        st.step(np.float32(0.016))
+    pos, vel, att = _run("sand", tmp_path, sand, solids, st.total, box, 3.1, 3)
+    n = sand.shape[0]
+    assert np.array_equal(pos, st.p[:n])
+    assert np.array_equal(vel, st.v[:n])
+    assert np.array_equal(att, st.att[:n])
+
+
+def test_adapter_simulate_pbf_matches_oracle(gpu_mod, oracle, tmp_path):
+    box = (16, 16, 16)
+    cfg = scenes.small(8, 8, 8, box=box, gravity=(0.0, -10.0, 0.0), c_xsph=0.1)
+    sand = cfg.positions()
+    p = oracle.pbf_params(cfg)
+    p.rest_density = 24.0  # the stand-in keeps the Simulation default (src/Simulation.hpp:157)
+    x, v = sand.copy(), np.zeros_like(sand)
+    ids = np.arange(sand.shape[0], dtype=np.int32)
+    for _ in range(2):
+        oracle.pbf_step(p, x, v, ids, np.float32(1.0 / 60.0))
+    want_x = np.zeros_like(x)
+    want_v = np.zeros_like(v)
+    want_x[ids] = x
+    want_v[ids] = v
+    total = max(sand.shape[0], 16 ** 3)
+    pos, vel, _ = _run("pbf", tmp_path, sand, np.zeros((0, 3), np.float32), total, box, 4.0, 2)
+    assert np.array_equal(pos, want_x)
+    assert np.array_equal(vel, want_v)
