@@ -123,8 +123,23 @@ int  lus_gpu_sync(LusGpu* ctx);
 /* Uploads n particles (host AoS float[3]).  vel may be NULL (zero).
  * Particle identities 0..n-1 are carried through the per-step reordering. */
 int lus_pbf_init(LusGpu* ctx, const LusPbfParams* p, int n, const float* pos, const float* vel);
+/* Many independent scenes in one set of launches.  Scene s has counts[s]
+ * particles (concatenated in pos/vel, scene coordinates) in the box
+ * [0, boxes[3s..3s+2]]; its clamp box is [p->lo, box - p->lo].  p gives the
+ * shared solver constants (gridX/Y/Z and hi are ignored).  The scene grids are
+ * stacked along y one empty cell layer apart in one batch grid, so each scene
+ * computes exactly what it computes alone.  Particle ids run over the
+ * concatenation; lus_pbf_download with order = 1 returns that order. */
+int lus_pbf_batch_init(LusGpu* ctx, const LusPbfParams* p, int nscenes, const int* counts,
+                       const float* boxes, const float* pos, const float* vel);
 /* Enqueues `steps` substeps of length dt on the context stream. */
 int lus_pbf_step(LusGpu* ctx, float dt, int steps);
+/* on = 1: lus_pbf_step replays a hipGraph holding two captured substeps
+ * (captured on first use and again when dt changes); on = 0: plain launches. */
+int lus_pbf_set_graph(LusGpu* ctx, int on);
+/* Runs `steps` substeps with a hipEvent between steps (between step pairs in
+ * graph mode, each step of a pair getting half); ms[k] = step k's time. */
+int lus_pbf_step_latency(LusGpu* ctx, float dt, int steps, float* ms);
 /* Downloads the state.  order = 0: internal cell order (ids[] says which
  * particle each row is); order = 1: original particle order.  Any output
  * pointer may be NULL. */

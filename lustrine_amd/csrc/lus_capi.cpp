@@ -115,6 +115,16 @@ struct PbfState {
     int nmaxc = 0;  // iterations with a block-max table
     int nblk = 0;   // blocks of one particle-parallel launch
     bool stepped = false;
+    // batch of scenes (nullptr scenes: one scene)
+    lus::SceneBox* scenes = nullptr;
+    int *sid = nullptr, *sid2 = nullptr;
+    int nscenes = 0;
+    // hipGraph replay of two captured steps
+    bool use_graph = false;
+    int parity = 0, graph_parity = 0;
+    float graph_dt = 0.0f;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 };
 
 struct SandState {
@@ -201,40 +211,95 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         ev_phase->push_back(phase);
         return hipEventRecord(e, s);
     };
+    // per-step constants: the scene-id buffers of a batch ping-pong like id
+    lus::PbfK k = P.k;
+    if (P.scenes) {
+        k.scenes = P.scenes;
+        k.sid = P.sid;
+        k.sid_out = P.sid2;
+    }
     if (ev) LUS_HIP(mark(-1));
-    LUS_HIP(lus::launch_pbf_key(P.x, P.v, P.n, P.k, dt, P.key, P.slot, P.counts, s));
+    LUS_HIP(lus::launch_pbf_key(P.x, P.v, P.n, k, dt, P.key, P.slot, P.counts, s));
     LUS_HIP(mark(0));
     LUS_HIP(lus::launch_exclusive_scan(P.counts, P.cs, P.C + 1, P.tiles, 1, s));
     LUS_HIP(mark(1));
     LUS_HIP(lus::launch_scatter(P.key, P.slot, P.cs, P.n, P.tmp, s));
     LUS_HIP(mark(2));
-    LUS_HIP(lus::launch_pbf_permute(P.tmp, P.key, P.cs, P.x, P.v, P.id, P.n, P.k, dt, P.x2, P.xp,
+    LUS_HIP(lus::launch_pbf_permute(P.tmp, P.key, P.cs, P.x, P.v, P.id, P.n, k, dt, P.x2, P.xp,
                                     P.id2, P.key2, s));
     LUS_HIP(mark(3));
     // iterate: xp (predicted) -> lambda -> xl -> delta-p -> xa -> lambda -> xl ...
     const int K = P.p.iterations;
     for (int it = 0; it < K; ++it) {
         const float4* cur = it == 0 ? P.xp : P.xa;
-        LUS_HIP(lus::launch_pbf_lambda(cur, P.key2, P.cs, P.n, P.k, P.xl, P.maxc + (size_t)it * P.nblk,
-                                       P.masks, P.xp,
-                                       it == 0 ? 1 : 0, s));
+        LUS_HIP(lus::launch_pbf_lambda(cur, P.key2, P.cs, P.n, k, P.xl, P.maxc + (size_t)it * P.nblk,
+                                       P.masks, P.xp, it == 0 ? 1 : 0, s));
         LUS_HIP(mark(it == 0 ? 7 : 4));
         const bool last = (it == K - 1);
-        LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, P.k, P.xa, P.x2,
+        LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, k, P.xa, P.x2,
                                        last ? P.v2 : nullptr, dt, P.masks, P.xp, s));
         LUS_HIP(mark(5));
     }
     const float4* fin = K > 0 ? P.xa : P.xp;
     if (K == 0) {
         // neighbour sets for XSPH (the lambda values are not used)
-        LUS_HIP(lus::launch_pbf_lambda(P.xp, P.key2, P.cs, P.n, P.k, P.xl, P.maxc, P.masks, P.xp, 1, s));
+        LUS_HIP(lus::launch_pbf_lambda(P.xp, P.key2, P.cs, P.n, k, P.xl, P.maxc, P.masks, P.xp, 1, s));
         LUS_HIP(lus::launch_pbf_velocity(P.xp, P.x2, P.n, dt, P.v2, s));
     }
-    LUS_HIP(lus::launch_pbf_xsph(fin, P.v2, P.key2, P.cs, P.n, P.k, P.v, P.x2, P.masks, P.xp, s));
+    LUS_HIP(lus::launch_pbf_xsph(fin, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp, s));
     LUS_HIP(mark(6));
     std::swap(P.x, P.x2);
     std::swap(P.id, P.id2);
+    std::swap(P.sid, P.sid2);
+    P.parity ^= 1;
     P.stepped = true;
+    return LUS_OK;
+}
+
+int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const float* vel,
+              const std::vector<lus::SceneBox>* scenes, const std::vector<int>* sid);
+
+void pbf_drop_graph(PbfState& P) {
+    if (P.graph_exec) (void)hipGraphExecDestroy(P.graph_exec);
+    if (P.graph) (void)hipGraphDestroy(P.graph);
+    P.graph_exec = nullptr;
+    P.graph = nullptr;
+}
+
+// `steps` substeps; with graphs on, pairs of steps replay one captured graph
+// (two steps restore the ping-pong buffer assignment the capture baked in)
+int pbf_run(LusGpu* c, float dt, int steps) {
+    PbfState& P = c->pbf;
+    if (!P.use_graph) {
+        for (int s = 0; s < steps; ++s) {
+            int r = pbf_enqueue(c, dt, nullptr, nullptr);
+            if (r) return r;
+        }
+        return LUS_OK;
+    }
+    int left = steps;
+    while (left >= 2) {
+        if (!P.graph_exec || P.graph_dt != dt || P.graph_parity != P.parity) {
+            pbf_drop_graph(P);
+            LUS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int r1 = pbf_enqueue(c, dt, nullptr, nullptr);
+            int r2 = r1 ? r1 : pbf_enqueue(c, dt, nullptr, nullptr);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (r1 || r2) {
+                if (g) (void)hipGraphDestroy(g);
+                return r1 ? r1 : r2;
+            }
+            LUS_HIP(e);
+            P.graph = g;
+            LUS_HIP(hipGraphInstantiate(&P.graph_exec, P.graph, nullptr, nullptr, 0));
+            P.graph_dt = dt;
+            P.graph_parity = P.parity;
+        }
+        LUS_HIP(hipGraphLaunch(P.graph_exec, c->stream));
+        left -= 2;
+    }
+    if (left) return pbf_enqueue(c, dt, nullptr, nullptr);
     return LUS_OK;
 }
 
@@ -278,6 +343,7 @@ void lus_gpu_destroy(LusGpu* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    pbf_drop_graph(c->pbf);
     c->pbf.mem.release();
     c->sand.mem.release();
     c->sand.nbr_mem.release();
@@ -500,18 +566,89 @@ int lus_pbf_init(LusGpu* c, const LusPbfParams* p, int n, const float* pos, cons
     if (!points_in_grid(g, corners, 2) || !(p->lo[0] <= p->hi[0]) || !(p->lo[1] <= p->hi[1]) ||
         !(p->lo[2] <= p->hi[2]))
         return set_error(LUS_EINVAL, "lus_pbf_init: clamp box outside the grid");
+    return pbf_setup(c, p, n, pos, vel, nullptr, nullptr);
+}
+
+int lus_pbf_batch_init(LusGpu* c, const LusPbfParams* p, int nscenes, const int* counts,
+                       const float* boxes, const float* pos, const float* vel) {
+    if (!c || !p || nscenes <= 0 || !counts || !boxes || !pos)
+        return set_error(LUS_EINVAL, "lus_pbf_batch_init: bad arguments");
+    if (!(p->h > 0.0f) || !(p->f > 0.0f) || p->cell_size < p->h || p->iterations < 0 ||
+        p->iterations > 1024 || p->scorr_n < 1 || p->scorr_n > 16 || !(p->rest_density > 0.0f) ||
+        !(p->cell_size > 0.0f))
+        return set_error(LUS_EINVAL, "lus_pbf_batch_init: bad solver constants");
+    // scene grids stacked along y with one empty cell layer between scenes
+    std::vector<lus::SceneBox> tab((size_t)nscenes);
+    std::vector<int> sid;
+    long long total = 0;
+    int gX = 0, gY = 0, gZ = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        if (counts[s] <= 0) return set_error(LUS_EINVAL, "lus_pbf_batch_init: scene %d is empty", s);
+        total += counts[s];
+        int dims[3];
+        for (int a = 0; a < 3; ++a) {
+            volatile float q = boxes[3 * s + a] / p->cell_size;
+            dims[a] = (int)q + 1;
+            tab[s].lo[a] = p->lo[a];
+            tab[s].hi[a] = boxes[3 * s + a] - p->lo[a];
+        }
+        LusGrid gs{p->cell_size, dims[0], dims[1], dims[2]};
+        float corners[6] = {tab[s].lo[0], tab[s].lo[1], tab[s].lo[2],
+                            tab[s].hi[0], tab[s].hi[1], tab[s].hi[2]};
+        if (!points_in_grid(gs, corners, 2) || !(tab[s].lo[0] <= tab[s].hi[0]) ||
+            !(tab[s].lo[1] <= tab[s].hi[1]) || !(tab[s].lo[2] <= tab[s].hi[2]))
+            return set_error(LUS_EINVAL, "lus_pbf_batch_init: scene %d clamp box outside its grid", s);
+        tab[s].yoff = gY + (s ? 1 : 0);
+        gY = tab[s].yoff + dims[1];
+        gX = std::max(gX, dims[0]);
+        gZ = std::max(gZ, dims[2]);
+    }
+    if (total >= (1LL << 30)) return set_error(LUS_EINVAL, "lus_pbf_batch_init: too many particles");
+    sid.reserve((size_t)total);
+    for (int s = 0; s < nscenes; ++s) sid.insert(sid.end(), (size_t)counts[s], s);
+    LusPbfParams q = *p;
+    q.gridX = gX;
+    q.gridY = gY;
+    q.gridZ = gZ;
+    for (int a = 0; a < 3; ++a) {
+        q.lo[a] = tab[0].lo[a];
+        q.hi[a] = tab[0].hi[a];
+    }
+    LusGrid g{q.cell_size, gX, gY, gZ};
+    if (!grid_ok(g)) return set_error(LUS_EINVAL, "lus_pbf_batch_init: batch grid too large");
+    return pbf_setup(c, &q, (int)total, pos, vel, &tab, &sid);
+}
+
+}  // extern "C"
+
+namespace {
+int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const float* vel,
+              const std::vector<lus::SceneBox>* scenes, const std::vector<int>* sid) {
     LUS_HIP(hipSetDevice(c->device));
     PbfState& P = c->pbf;
     LUS_HIP(hipStreamSynchronize(c->stream));
+    pbf_drop_graph(P);
     P.mem.release();
     P.ready = false;
     P.stepped = false;
+    P.parity = 0;
+    P.scenes = nullptr;
+    P.sid = P.sid2 = nullptr;
+    P.nscenes = scenes ? (int)scenes->size() : 0;
     P.p = *p;
     P.k = pbf_constants(*p);
     P.n = n;
     P.C = p->gridX * p->gridY * p->gridZ;
     P.nmaxc = std::max(p->iterations, 1);
     Arena& m = P.mem;
+    if (scenes) {
+        LUS_HIP(m.alloc(&P.scenes, scenes->size()));
+        LUS_HIP(m.alloc(&P.sid, n));
+        LUS_HIP(m.alloc(&P.sid2, n));
+        LUS_HIP(hipMemcpy(P.scenes, scenes->data(), sizeof(lus::SceneBox) * scenes->size(),
+                          hipMemcpyHostToDevice));
+        LUS_HIP(hipMemcpy(P.sid, sid->data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    }
     LUS_HIP(m.alloc(&P.x, n));
     LUS_HIP(m.alloc(&P.x2, n));
     LUS_HIP(m.alloc(&P.v, n));
@@ -545,14 +682,54 @@ int lus_pbf_init(LusGpu* c, const LusPbfParams* p, int n, const float* pos, cons
     P.ready = true;
     return LUS_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int lus_pbf_step(LusGpu* c, float dt, int steps) {
     if (!c || !c->pbf.ready) return set_error(LUS_ESTATE, "lus_pbf_step: no PBF scene");
     if (!(dt > 0.0f) || steps < 0) return set_error(LUS_EINVAL, "lus_pbf_step: bad dt/steps");
     LUS_HIP(hipSetDevice(c->device));
-    for (int s = 0; s < steps; ++s) {
-        int r = pbf_enqueue(c, dt, nullptr, nullptr);
+    return pbf_run(c, dt, steps);
+}
+
+int lus_pbf_step_latency(LusGpu* c, float dt, int steps, float* ms) {
+    if (!c || !c->pbf.ready || !ms || steps <= 0 || !(dt > 0.0f))
+        return set_error(LUS_EINVAL, "lus_pbf_step_latency: bad arguments");
+    LUS_HIP(hipSetDevice(c->device));
+    const int per = c->pbf.use_graph ? 2 : 1;
+    const int groups = (steps + per - 1) / per;
+    std::vector<hipEvent_t> ev((size_t)groups + 1);
+    for (auto& e : ev) LUS_HIP(hipEventCreate(&e));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    LUS_HIP(hipEventRecord(ev[0], c->stream));
+    int done = 0;
+    for (int g = 0; g < groups; ++g) {
+        const int k = std::min(per, steps - done);
+        int r = pbf_run(c, dt, k);
         if (r) return r;
+        LUS_HIP(hipEventRecord(ev[g + 1], c->stream));
+        done += k;
+    }
+    LUS_HIP(hipEventSynchronize(ev[groups]));
+    done = 0;
+    for (int g = 0; g < groups; ++g) {
+        const int k = std::min(per, steps - done);
+        float t = 0.0f;
+        LUS_HIP(hipEventElapsedTime(&t, ev[g], ev[g + 1]));
+        for (int q = 0; q < k; ++q) ms[done + q] = t / (float)k;
+        done += k;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return LUS_OK;
+}
+
+int lus_pbf_set_graph(LusGpu* c, int on) {
+    if (!c) return set_error(LUS_EINVAL, "lus_pbf_set_graph: NULL context");
+    c->pbf.use_graph = on != 0;
+    if (!on) {
+        LUS_HIP(hipStreamSynchronize(c->stream));
+        pbf_drop_graph(c->pbf);
     }
     return LUS_OK;
 }

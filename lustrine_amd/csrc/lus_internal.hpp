@@ -16,11 +16,23 @@ namespace lus {
 //   W(r)  = P6 * t^3,          t = hf2 - f2 * r2            (r2 <= h2)
 //   dW(r) = -s * r_vec,        s = SPf * g * g / r, g = hf - f * r
 // and keep the accumulated t^3 unnormalised (c_rho = m * P6 / rho0).
+// One scene of a batch: box clamp bounds (scene coordinates) and the y offset
+// of its cells in the batch grid (scenes are stacked along y, one empty cell
+// layer apart, so no stencil reaches another scene).
+struct SceneBox {
+    float lo[3], hi[3];
+    int yoff;
+    int pad;
+};
+
 struct PbfK {
     float h2, hf, hf2, f, f2, SPf, c_rho, mr0sq, eps, inv_tq3, scorr_k, neg_inv_r0, cW, cs;
     float lo[3], hi[3], g[3];
     int gX, gY, gZ, gXZ, n_scorr;
-    int exact;  // 1: oracle operation order; 0: fused fast path
+    int exact;                  // 1: oracle operation order; 0: fused fast path
+    const SceneBox* scenes;     // batch table, nullptr for a single scene
+    const int* sid;             // scene of each particle (input order), batch only
+    int* sid_out;               // scene of each particle in cell order, batch only
 };
 
 // simulate_sand constants (src/Simulate.cpp:159-163 and the Simulation fields).

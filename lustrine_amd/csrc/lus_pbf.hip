@@ -37,16 +37,54 @@ namespace lus {
 
 constexpr int kOvf = 1 << 30;  // keys are < 2^30 (checked at init)
 
-__device__ __forceinline__ float4 pbf_predict(float4 x, float4 v, const PbfK& k, float dt) {
+// clamp box and cell y offset of a particle: the scene table entry of a batch,
+// or the single scene's bounds
+struct Bounds {
+    float lo[3], hi[3];
+    int yoff;
+};
+
+__device__ __forceinline__ Bounds bounds_of(const PbfK& k, int s) {
+    Bounds b;
+    if (k.scenes) {
+        const SceneBox sb = k.scenes[s];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = sb.lo[a];
+            b.hi[a] = sb.hi[a];
+        }
+        b.yoff = sb.yoff;
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = k.lo[a];
+            b.hi[a] = k.hi[a];
+        }
+        b.yoff = 0;
+    }
+    return b;
+}
+
+__device__ __forceinline__ float4 pbf_predict(float4 x, float4 v, const PbfK& k, const Bounds& b,
+                                              float dt) {
     float vx = v.x + k.g[0] * dt;
     float vy = v.y + k.g[1] * dt;
     float vz = v.z + k.g[2] * dt;
     float4 r;
-    r.x = rclamp(x.x + vx * dt, k.lo[0], k.hi[0]);
-    r.y = rclamp(x.y + vy * dt, k.lo[1], k.hi[1]);
-    r.z = rclamp(x.z + vz * dt, k.lo[2], k.hi[2]);
+    r.x = rclamp(x.x + vx * dt, b.lo[0], b.hi[0]);
+    r.y = rclamp(x.y + vy * dt, b.lo[1], b.hi[1]);
+    r.z = rclamp(x.z + vz * dt, b.lo[2], b.hi[2]);
     r.w = 0.0f;
     return r;
+}
+
+// get_cell_id (src/neighbors/Utils.hpp:24-33) in the (batch) grid: the y cell
+// index is shifted by the scene's offset
+__device__ __forceinline__ int pbf_cell(float4 p, const PbfK& k, int yoff) {
+    const float px = p.x / k.cs;
+    const float py = p.y / k.cs;
+    const float pz = p.z / k.cs;
+    return ((int)py + yoff) * k.gX * k.gZ + ((int)px) * k.gZ + ((int)pz);
 }
 
 // the exact neighbour test (no FMA, oracle order)
@@ -172,8 +210,9 @@ __global__ __launch_bounds__(kBlock) void k_pbf_key(const float4* __restrict__ x
     const bool active = i < n;
     int c = 0;
     if (active) {
-        const float4 xs = pbf_predict(x[i], v[i], k, dt);
-        c = cell_of(xs.x, xs.y, xs.z, k.cs, k.gX, k.gZ);
+        const Bounds b = bounds_of(k, k.scenes ? k.sid[i] : 0);
+        const float4 xs = pbf_predict(x[i], v[i], k, b, dt);
+        c = pbf_cell(xs, k, b.yoff);
         key[i] = c;
     }
     const int sl = wave_atomic_slot(counts, c, active);
@@ -198,10 +237,12 @@ __global__ __launch_bounds__(kBlock) void k_pbf_permute(
     const int c = key[i];
     const int o = stable_slot(tmp, cell_start, c, i);
     const float4 xi = x[i];
+    const int s = k.scenes ? k.sid[i] : 0;
     x2[o] = xi;
-    xp[o] = pbf_predict(xi, v[i], k, dt);
+    xp[o] = pbf_predict(xi, v[i], k, bounds_of(k, s), dt);
     id2[o] = id[i];
     key2[o] = c;
+    if (k.scenes) k.sid_out[o] = s;
 }
 
 hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_start,
@@ -360,10 +401,11 @@ __global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict_
             az = acc<kFast>(az, c, dz);
         }
     });
+    const Bounds b = bounds_of(k, k.scenes ? k.sid_out[i] : 0);
     float4 r;
-    r.x = rclamp(pi.x + ax * k.neg_inv_r0, k.lo[0], k.hi[0]);
-    r.y = rclamp(pi.y + ay * k.neg_inv_r0, k.lo[1], k.hi[1]);
-    r.z = rclamp(pi.z + az * k.neg_inv_r0, k.lo[2], k.hi[2]);
+    r.x = rclamp(pi.x + ax * k.neg_inv_r0, b.lo[0], b.hi[0]);
+    r.y = rclamp(pi.y + ay * k.neg_inv_r0, b.lo[1], b.hi[1]);
+    r.z = rclamp(pi.z + az * k.neg_inv_r0, b.lo[2], b.hi[2]);
     r.w = 0.0f;
     xa[i] = r;
     if (vout) {

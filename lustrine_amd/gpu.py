@@ -67,7 +67,10 @@ _SIGS = {
     "lus_gpu_destroy": (None, [C.c_void_p]),
     "lus_gpu_sync": (C.c_int, [C.c_void_p]),
     "lus_pbf_init": (C.c_int, [C.c_void_p, _P(LusPbfParams), C.c_int, _f, _f]),
+    "lus_pbf_batch_init": (C.c_int, [C.c_void_p, _P(LusPbfParams), C.c_int, _i, _f, _f, _f]),
     "lus_pbf_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
+    "lus_pbf_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "lus_pbf_step_latency": (C.c_int, [C.c_void_p, C.c_float, C.c_int, _f]),
     "lus_pbf_download": (C.c_int, [C.c_void_p, C.c_int, _f, _f, _i]),
     "lus_pbf_last_max_constraint": (C.c_int, [C.c_void_p, _f]),
     "lus_pbf_bench": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, _f, _f, _i]),
@@ -269,8 +272,26 @@ class Gpu:
                "lus_pbf_init")
         self.n = pos.shape[0]
 
+    def pbf_batch_init(self, params: LusPbfParams, counts, boxes, pos, vel=None):
+        counts = np.ascontiguousarray(counts, np.int32)
+        boxes = _f32(boxes, (-1, 3))
+        pos = _f32(pos, (-1, 3))
+        vel = None if vel is None else _f32(vel, (-1, 3))
+        assert pos.shape[0] == int(counts.sum()) and boxes.shape[0] == counts.shape[0]
+        _check(self._lib.lus_pbf_batch_init(self._h, C.byref(params), counts.shape[0], _ptr_i(counts),
+                                            _ptr_f(boxes), _ptr_f(pos), _ptr_f(vel)), "lus_pbf_batch_init")
+        self.n = pos.shape[0]
+
     def pbf_step(self, dt, steps=1):
         _check(self._lib.lus_pbf_step(self._h, dt, steps), "lus_pbf_step")
+
+    def pbf_set_graph(self, on=True):
+        _check(self._lib.lus_pbf_set_graph(self._h, int(bool(on))), "lus_pbf_set_graph")
+
+    def pbf_step_latency(self, dt, steps):
+        ms = np.zeros(steps, np.float32)
+        _check(self._lib.lus_pbf_step_latency(self._h, dt, steps, _ptr_f(ms)), "lus_pbf_step_latency")
+        return ms
 
     def pbf_download(self, order=0):
         pos = np.zeros((self.n, 3), np.float32)

@@ -165,3 +165,43 @@ def cfg2_tail(**kw) -> PbfConfig:
 
 def small(nx=8, ny=8, nz=8, box=(16, 16, 16), **kw) -> PbfConfig:
     return PbfConfig(name=f"pbf_small_{nx}x{ny}x{nz}", lattice=(nx, ny, nz), box=box, **kw)
+
+
+def dam_break_of_size(n_target: int, seed: int, **kw) -> PbfConfig:
+    """A jittered-lattice dam break of about n_target particles: a block twice
+    as wide as tall (ny = (n/4)^(1/3)), in a box twice the block extent per axis."""
+    ny = max(2, int(round((n_target / 4.0) ** (1.0 / 3.0))))
+    nx = max(1, int(round(math.sqrt(n_target / ny))))
+    nz = max(1, int(round(n_target / (nx * ny))))
+    return PbfConfig(name=f"dam_{nx}x{ny}x{nz}", lattice=(nx, ny, nz), box=(2 * nx, 2 * ny, 2 * nz),
+                     seed=seed, **kw)
+
+
+@dataclass
+class BatchConfig:
+    """Independent dam-break scenes stepped together."""
+    name: str
+    scenes: list
+    steps: int = 100
+
+    @property
+    def n(self) -> int:
+        return sum(s.n for s in self.scenes)
+
+    def counts(self) -> np.ndarray:
+        return np.array([s.n for s in self.scenes], np.int32)
+
+    def boxes(self) -> np.ndarray:
+        return np.array([s.box for s in self.scenes], np.float32)
+
+    def positions(self) -> np.ndarray:
+        return np.concatenate([s.positions() for s in self.scenes]).astype(np.float32)
+
+
+def cfg4(n_scenes: int = 64, lo: int = 4096, hi: int = 200000, seed: int = 4, **kw) -> BatchConfig:
+    """64 dam breaks with particle counts log-uniform in [4096, 200000]
+    (numpy PCG64 seeded with `seed`), jitter seed = seed * 1000 + scene."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(lo), math.log(hi), n_scenes))
+    scenes = [dam_break_of_size(int(round(s)), seed * 1000 + k, **kw) for k, s in enumerate(sizes)]
+    return BatchConfig(name=f"pbf_batch_{n_scenes}", scenes=scenes)
