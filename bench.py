@@ -84,15 +84,61 @@ def cpu_baseline(cfg, seconds_target: float = 15.0) -> dict:
     }
 
 
+def bench_batch(args, G, scenes) -> int:
+    """64 independent dam breaks stepped together: aggregate particle-steps/s
+    and the per-scene step latency (every scene of the batch completes its step
+    when the batch step completes, so a scene's latency is the batch step time)."""
+    b = scenes.cfg4()
+    base = b.scenes[0]
+    if args.iterations is not None:
+        for s in b.scenes:
+            s.iterations = args.iterations
+    params = G.pbf_params(base, exact=args.exact)
+    dt = np.float32(base.dt)
+    with G.Gpu(0) as g:
+        g.pbf_batch_init(params, b.counts(), b.boxes(), b.positions())
+        g.pbf_set_graph(not args.no_graph)
+        ms, _, _ = g.pbf_bench(dt, args.warmup, args.steps, phases=False)
+        g.pbf_batch_init(params, b.counts(), b.boxes(), b.positions())
+        g.pbf_set_graph(not args.no_graph)
+        g.pbf_step(dt, args.warmup)
+        lat = g.pbf_step_latency(dt, args.steps)
+    value = b.n * args.steps / (ms / 1e3)
+    out = {
+        "metric": "PBF particle-steps/s, 64 batched scenes (aggregate)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": b.name, "scenes": len(b.scenes), "particles": b.n,
+                   "scene_particles_min": int(b.counts().min()), "scene_particles_max": int(b.counts().max()),
+                   "iterations": base.iterations, "arithmetic": "exact" if args.exact else "fused",
+                   "launch": "plain" if args.no_graph else "hipgraph", "parallelism": "single-gpu"},
+        "scene_step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                  "max": float(lat.max())},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg2", "cfg2_tail"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg2", "cfg2_tail", "cfg4"])
     ap.add_argument("--iterations", type=int, default=None)
     ap.add_argument("--exact", action="store_true",
                     help="oracle operation order (bit-identical results) instead of the fused path")
+    ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--no-phases", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -106,12 +152,14 @@ def main(argv=None):
     from lustrine_amd import gpu as G
     from lustrine_amd import scenes
 
-    cfg = {"cfg0": scenes.cfg0, "cfg2": scenes.cfg2, "cfg2_tail": scenes.cfg2_tail}[args.workload]()
-    if args.iterations is not None:
-        cfg.iterations = args.iterations
     if G.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
+    if args.workload == "cfg4":
+        return bench_batch(args, G, scenes)
+    cfg = {"cfg0": scenes.cfg0, "cfg2": scenes.cfg2, "cfg2_tail": scenes.cfg2_tail}[args.workload]()
+    if args.iterations is not None:
+        cfg.iterations = args.iterations
     x0 = cfg.positions()
     gx, gy, gz = cfg.grid
     ncell = gx * gy * gz
@@ -119,6 +167,7 @@ def main(argv=None):
     dt = np.float32(cfg.dt)
     with G.Gpu(0) as g:
         g.pbf_init(params, x0, None)
+        g.pbf_set_graph(not args.no_graph)
         ms, _, _ = g.pbf_bench(dt, args.warmup, args.steps, phases=False)
         maxc = g.pbf_last_max_constraint()
         phase_ms = launches = None
@@ -147,7 +196,8 @@ def main(argv=None):
         "config": {"workload": cfg.name, "particles": cfg.n, "lattice": list(cfg.lattice),
                    "box": list(cfg.box), "cells": ncell, "h": cfg.h, "dt": cfg.dt,
                    "iterations": cfg.iterations, "c_xsph": cfg.c_xsph,
-                   "arithmetic": "exact" if args.exact else "fused", "parallelism": "single-gpu"},
+                   "arithmetic": "exact" if args.exact else "fused",
+                   "launch": "plain" if args.no_graph else "hipgraph", "parallelism": "single-gpu"},
         "step_roofline": {"alg_bytes_per_step": int(step_bytes),
                           "achieved_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
