@@ -152,10 +152,12 @@ int lus_pbf_last_max_constraint(LusGpu* ctx, float* out);
  * (array of LUS_PBF_NPHASES) the `steps` instead record an event after every
  * kernel: phase_ms accumulates per-phase device time, phase_launches the
  * launch count per phase, and *ms their sum. */
-#define LUS_PBF_NPHASES 8
+#define LUS_PBF_NPHASES 11
 /* phases: 0 predict+cell key, 1 cell scan, 2 scatter, 3 stable permute,
  *         4 lambda pass (iterations 2..K), 5 delta-p pass, 6 XSPH + commit,
- *         7 first lambda pass of the step (builds the neighbour sets) */
+ *         7 first lambda pass of the step (builds the neighbour sets),
+ *         8 warm-start features (build the neighbour sets), 9 warm-start
+ *         network, 10 density decision (learned mode or density check) */
 int lus_pbf_bench(LusGpu* ctx, float dt, int warmup, int steps, float* ms,
                   float* phase_ms, int* phase_launches);
 /* Average device time per launch (microseconds) of the neighbour-pass kernels
@@ -165,6 +167,87 @@ int lus_pbf_bench(LusGpu* ctx, float dt, int warmup, int steps, float* ms,
  * hipEvents on the context stream.  Outputs go to scratch buffers; the scene
  * state is left unchanged.  Requires at least one step since init. */
 int lus_pbf_time_kernels(LusGpu* ctx, int reps, float* us);
+
+/* --- learned warm start of the PBF solve ---
+ * No reference counterpart: the reference solves the density constraint with
+ * one uncorrected pass (src/Simulate.cpp:83-154).  A per-particle network
+ * maps 32 features of the predicted positions to a position correction; a
+ * learned step applies it, runs `learned_iterations` plain Jacobi
+ * iterations, measures e = max_i max(0, rho_i/rho0 - 1) at the final
+ * positions (in the XSPH pass) and, when e > bound (or e is NaN), reruns the
+ * plain step (LusPbfParams::iterations) from the predicted positions.  The
+ * decision stays on the device (no host synchronisation); a rerun gives the
+ * plain step's results bit for bit.
+ *
+ * Features (per particle i, over its neighbour set N(i) at the predicted
+ * positions x*, offsets d = x*_j - x*_i, r = |d|):
+ *   [0]         rho_i/rho0 - 1 (self included)
+ *   [1..3]      (x*_i - x_i)/h
+ *   [4+3b..6+3b] b = 0..7: mean d/h over j != i with min(7, (int)(8 r/h)) = b
+ *   [28]        |N(i)| - 1 over 32
+ *   [29], [30]  mean r/h and mean r^2/h^2 over j != i
+ *   [31]        min(1, distance to the nearest clamp wall / h)
+ * Network: 32 -> 64 (SiLU) -> 64 (SiLU) -> 3, bf16 weights and activations,
+ * fp32 accumulation, output * out_scale = position correction.
+ * Network file (LUS_NET_FLOATS floats): feature mean [32], feature 1/std [32],
+ * parameters [LUS_MLP_PARAMS] = W1[32][64] b1[64] W2[64][64] b2[64] W3[64][3]
+ * b3[3] (W[k][n]: input k -> output n), out_scale [1]. */
+#define LUS_FEATURES 32
+#define LUS_MLP_PARAMS 6467
+#define LUS_NET_MU 0
+#define LUS_NET_INV_SIGMA 32
+#define LUS_NET_PARAMS 64
+#define LUS_NET_OUT_SCALE (LUS_NET_PARAMS + LUS_MLP_PARAMS)
+#define LUS_NET_FLOATS (LUS_NET_OUT_SCALE + 1)
+#define LUS_DENSITY_HISTORY 4096
+
+/* Uploads the network the learned step uses (kept across scene re-inits). */
+int lus_pbf_set_network(LusGpu* ctx, const float* net, int nfloats);
+/* learned = 1: learned steps (needs a network, LusPbfParams::iterations >= 1);
+ * 0: plain steps.  check = 1 in plain mode: the XSPH pass also measures e and
+ * records it (no rerun).  Resets the statistics; drops a captured graph. */
+int lus_pbf_set_mode(LusGpu* ctx, int learned, int learned_iterations, float bound, int check);
+/* Steps measured since the last reset, reruns among them, and the last
+ * min(steps, LUS_DENSITY_HISTORY, cap) values of e (oldest first) in errs;
+ * *nerrs receives their count.  In learned mode e is the learned solution's. */
+int lus_pbf_learned_stats(LusGpu* ctx, long long* steps, long long* reruns, float* errs, int cap,
+                          int* nerrs);
+
+/* Training of the network on the device.  A trainer holds a dataset of
+ * (raw features, target) rows; the target of a recorded particle is the plain
+ * solver's position correction x_K - x* of that step. */
+typedef struct LusMlpTrainer LusMlpTrainer;
+typedef struct LusMlpFitParams {
+    int epochs, batch;           /* batch: rows per Adam step (multiple of 32) */
+    float lr, lr_final;          /* cosine schedule from lr to lr_final */
+    float val_fraction;          /* rows held out for validation (random) */
+    unsigned long long seed;     /* initialisation and shuffling */
+} LusMlpFitParams;
+int  lus_mlp_trainer_create(LusGpu* ctx, long capacity, LusMlpTrainer** out);
+void lus_mlp_trainer_destroy(LusMlpTrainer* t);
+/* Appends host rows: feat [rows][32], target [rows][3]. */
+int  lus_mlp_trainer_add(LusMlpTrainer* t, const float* feat, const float* target, long rows);
+int  lus_mlp_trainer_rows(LusMlpTrainer* t, long* rows);
+int  lus_mlp_trainer_download(LusMlpTrainer* t, long first, long rows, float* feat, float* target);
+/* One plain step of the context's scene that also appends the features and
+ * targets of the particles in cell-order slots p with (p + k) % stride == 0,
+ * k = the number of earlier record calls on this trainer.  *added (nullable)
+ * receives the number of rows appended. */
+int  lus_pbf_record_step(LusGpu* ctx, LusMlpTrainer* t, float dt, int stride, long* added);
+/* Fits a fresh network to the dataset (normalisation from the dataset,
+ * LeCun-uniform initialisation, Adam with b1 0.9, b2 0.999, eps 1e-8, MSE of
+ * target / out_scale with out_scale the RMS target component).  Mean losses
+ * per epoch go to train_loss / val_loss (nullable, `epochs` floats each). */
+int  lus_mlp_fit(LusMlpTrainer* t, const LusMlpFitParams* fp, float* train_loss, float* val_loss);
+int  lus_mlp_trainer_network(LusMlpTrainer* t, float* net);
+/* Loss and gradient of the training kernels on host data: X [rows][32]
+ * (network inputs, rounded to bf16), Y [rows][3]; loss = mean over rows of
+ * sum_c (o_c - Y_c * inv_scale)^2; grad [LUS_MLP_PARAMS]. */
+int  lus_mlp_loss_grad(LusGpu* ctx, const float* params, int rows, const float* X, const float* Y,
+                       float inv_scale, float* loss, float* grad);
+/* The learned step's inference kernel on host features: out [rows][3] =
+ * out_scale * network((feat - mu) * inv_sigma). */
+int  lus_mlp_infer(LusGpu* ctx, const float* net, int rows, const float* feat, float* out);
 
 /* --- sand scene: simulate_sand (src/Simulate.cpp:156-325) --- */
 /* n_sand moving particles (pos, vel, attracted may be NULL for zero) and

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <random>
 #include <vector>
 
 #include "lus_device.hpp"
@@ -125,6 +126,33 @@ struct PbfState {
     float graph_dt = 0.0f;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // learned warm start and density statistics
+    void* feat = nullptr;  // bf16 [n][32] network inputs
+    int* maxd = nullptr;   // per-block max of C at the final positions
+    int* flag = nullptr;   // 1: rerun the plain step
+    int* counters = nullptr;  // [0] measured steps, [1] reruns
+    float* hist = nullptr;    // LUS_DENSITY_HISTORY ring of e
+    bool learned = false, check = false;
+    int learned_iters = 1;
+    float bound = 0.01f;
+};
+
+// the network of the learned step (device copies), kept across scene inits
+struct NetState {
+    bool ready = false;
+    Arena mem;
+    float* params = nullptr;  // fp32 [kMlpParams]
+    void* packed = nullptr;   // bf16 B-operand images
+    float* mu = nullptr;
+    float* inv_sigma = nullptr;
+    float out_scale = 1.0f;
+};
+
+// one plain step that also writes training rows (lus_pbf_record_step)
+struct Record {
+    float* feat;  // fp32 [rows][32]
+    float* tgt;   // fp32 [rows][3]
+    int stride, off;
 };
 
 struct SandState {
@@ -150,7 +178,19 @@ struct LusGpu {
     int device = 0;
     hipStream_t stream = nullptr;
     PbfState pbf;
+    NetState net;
     SandState sand;
+};
+
+struct LusMlpTrainer {
+    LusGpu* ctx = nullptr;
+    long cap = 0, rows = 0;
+    int rec_calls = 0;
+    Arena mem;
+    float* feat = nullptr;  // [cap][32]
+    float* tgt = nullptr;   // [cap][3]
+    bool has_net = false;
+    std::vector<float> net;
 };
 
 namespace {
@@ -198,8 +238,17 @@ lus::PbfK pbf_constants(const LusPbfParams& p) {
 }
 
 // Enqueues one PBF substep.  ev (nullable): LUS_PBF_NPHASES-indexed event log,
-// ev_phase[k] receives the phase of event k; returns the number of events.
-int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<int>* ev_phase) {
+// ev_phase[k] receives the phase of event k.  rec (nullable): a plain step
+// that also writes training rows.
+//
+// Plain step:   key, scan, scatter, permute, K x (lambda, delta-p), XSPH
+// Learned step: key, scan, scatter, permute, features (neighbour sets),
+//               network (x* + correction), L x (lambda, delta-p),
+//               XSPH (+ density maximum), decide, then -- gated on the
+//               decision -- permute, K x (lambda, delta-p), XSPH again: the
+//               plain step from the same sorted state.
+int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<int>* ev_phase,
+                const Record* rec = nullptr) {
     PbfState& P = c->pbf;
     hipStream_t s = c->stream;
     auto mark = [&](int phase) -> hipError_t {
@@ -226,28 +275,84 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
     LUS_HIP(lus::launch_scatter(P.key, P.slot, P.cs, P.n, P.tmp, s));
     LUS_HIP(mark(2));
     LUS_HIP(lus::launch_pbf_permute(P.tmp, P.key, P.cs, P.x, P.v, P.id, P.n, k, dt, P.x2, P.xp,
-                                    P.id2, P.key2, s));
+                                    P.id2, P.key2, nullptr, s));
     LUS_HIP(mark(3));
-    // iterate: xp (predicted) -> lambda -> xl -> delta-p -> xa -> lambda -> xl ...
     const int K = P.p.iterations;
-    for (int it = 0; it < K; ++it) {
-        const float4* cur = it == 0 ? P.xp : P.xa;
-        LUS_HIP(lus::launch_pbf_lambda(cur, P.key2, P.cs, P.n, k, P.xl, P.maxc + (size_t)it * P.nblk,
-                                       P.masks, P.xp, it == 0 ? 1 : 0, s));
-        LUS_HIP(mark(it == 0 ? 7 : 4));
-        const bool last = (it == K - 1);
-        LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, k, P.xa, P.x2,
-                                       last ? P.v2 : nullptr, dt, P.masks, P.xp, s));
-        LUS_HIP(mark(5));
+    const bool learned = P.learned && !rec;
+    // The plain iterations: xp (predicted) -> lambda -> xl -> delta-p -> xa ->
+    // lambda -> xl ...; the last delta-p pass also writes v2 = (x* - x) / dt.
+    // built: the neighbour masks exist already (features pass).
+    auto plain_iterations = [&](bool built, const int* gate) -> int {
+        for (int it = 0; it < K; ++it) {
+            const float4* cur = it == 0 ? P.xp : P.xa;
+            const int build = (it == 0 && !built) ? 1 : 0;
+            LUS_HIP(lus::launch_pbf_lambda(cur, P.key2, P.cs, P.n, k, P.xl, P.maxc + (size_t)it * P.nblk,
+                                           P.masks, P.xp, build, gate, s));
+            LUS_HIP(mark(it == 0 ? 7 : 4));
+            const bool last = (it == K - 1);
+            LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, k, P.xa, P.x2,
+                                           last ? P.v2 : nullptr, dt, P.masks, P.xp, gate, s));
+            LUS_HIP(mark(5));
+        }
+        return LUS_OK;
+    };
+    if (learned) {
+        const NetState& N = c->net;
+        LUS_HIP(lus::launch_pbf_features(P.key2, P.cs, P.n, k, P.masks, P.xp, P.x2, N.mu, N.inv_sigma,
+                                         P.feat, nullptr, 1, 0, s));
+        LUS_HIP(mark(8));
+        LUS_HIP(lus::launch_mlp_forward(P.feat, P.n, N.params, N.packed, N.out_scale, nullptr, P.xp,
+                                        P.xa, k, s));
+        LUS_HIP(mark(9));
+        const int L = P.learned_iters;
+        for (int it = 0; it < L; ++it) {
+            LUS_HIP(lus::launch_pbf_lambda(P.xa, P.key2, P.cs, P.n, k, P.xl, P.maxc, P.masks, P.xp, 0,
+                                           nullptr, s));
+            LUS_HIP(mark(4));
+            LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, k, P.xa, P.x2,
+                                           it == L - 1 ? P.v2 : nullptr, dt, P.masks, P.xp, nullptr, s));
+            LUS_HIP(mark(5));
+        }
+        if (L == 0) LUS_HIP(lus::launch_pbf_velocity(P.xa, P.x2, P.n, dt, P.v2, s));
+        LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp, P.maxd,
+                                     nullptr, s));
+        LUS_HIP(mark(6));
+        LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, P.bound, 1, P.flag, P.counters, P.hist,
+                                       LUS_DENSITY_HISTORY, s));
+        LUS_HIP(mark(10));
+        // the plain step, run only when the decision asks for it (K >= 1)
+        LUS_HIP(lus::launch_pbf_permute(P.tmp, P.key, P.cs, P.x, P.v, P.id, P.n, k, dt, P.x2, P.xp,
+                                        P.id2, P.key2, P.flag, s));
+        LUS_HIP(mark(3));
+        if (int r = plain_iterations(false, P.flag)) return r;
+        LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp, nullptr,
+                                     P.flag, s));
+        LUS_HIP(mark(6));
+    } else {
+        if (rec) {
+            LUS_HIP(lus::launch_pbf_features(P.key2, P.cs, P.n, k, P.masks, P.xp, P.x2, nullptr, nullptr,
+                                             nullptr, rec->feat, rec->stride, rec->off, s));
+            LUS_HIP(mark(8));
+        }
+        if (int r = plain_iterations(rec != nullptr, nullptr)) return r;
+        if (rec) LUS_HIP(lus::launch_pbf_record_target(P.xa, P.xp, P.n, rec->stride, rec->off, rec->tgt, s));
+        const float4* fin = K > 0 ? P.xa : P.xp;
+        if (K == 0) {
+            // neighbour sets for XSPH (the lambda values are not used)
+            if (!rec)
+                LUS_HIP(lus::launch_pbf_lambda(P.xp, P.key2, P.cs, P.n, k, P.xl, P.maxc, P.masks, P.xp, 1,
+                                               nullptr, s));
+            LUS_HIP(lus::launch_pbf_velocity(P.xp, P.x2, P.n, dt, P.v2, s));
+        }
+        LUS_HIP(lus::launch_pbf_xsph(fin, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp,
+                                     P.check ? P.maxd : nullptr, nullptr, s));
+        LUS_HIP(mark(6));
+        if (P.check) {
+            LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, P.bound, 0, P.flag, P.counters, P.hist,
+                                           LUS_DENSITY_HISTORY, s));
+            LUS_HIP(mark(10));
+        }
     }
-    const float4* fin = K > 0 ? P.xa : P.xp;
-    if (K == 0) {
-        // neighbour sets for XSPH (the lambda values are not used)
-        LUS_HIP(lus::launch_pbf_lambda(P.xp, P.key2, P.cs, P.n, k, P.xl, P.maxc, P.masks, P.xp, 1, s));
-        LUS_HIP(lus::launch_pbf_velocity(P.xp, P.x2, P.n, dt, P.v2, s));
-    }
-    LUS_HIP(lus::launch_pbf_xsph(fin, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp, s));
-    LUS_HIP(mark(6));
     std::swap(P.x, P.x2);
     std::swap(P.id, P.id2);
     std::swap(P.sid, P.sid2);
@@ -270,6 +375,8 @@ void pbf_drop_graph(PbfState& P) {
 // (two steps restore the ping-pong buffer assignment the capture baked in)
 int pbf_run(LusGpu* c, float dt, int steps) {
     PbfState& P = c->pbf;
+    if (P.learned && (!c->net.ready || P.p.iterations < 1))
+        return set_error(LUS_ESTATE, "learned PBF step: needs a network and iterations >= 1");
     if (!P.use_graph) {
         for (int s = 0; s < steps; ++s) {
             int r = pbf_enqueue(c, dt, nullptr, nullptr);
@@ -345,6 +452,7 @@ void lus_gpu_destroy(LusGpu* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     pbf_drop_graph(c->pbf);
     c->pbf.mem.release();
+    c->net.mem.release();
     c->sand.mem.release();
     c->sand.nbr_mem.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -668,6 +776,17 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
     LUS_HIP(m.alloc(&P.tiles, lus::scan_tile_count(P.C + 1) + 1));
     P.nblk = lus::grid_for(n);
     LUS_HIP(m.alloc(&P.maxc, (size_t)P.nmaxc * P.nblk));
+    LUS_HIP(m.alloc(&P.maxd, P.nblk));
+    LUS_HIP(m.alloc(&P.flag, 1));
+    LUS_HIP(m.alloc(&P.counters, 2));
+    LUS_HIP(m.alloc(&P.hist, LUS_DENSITY_HISTORY));
+    LUS_HIP(hipMemset(P.counters, 0, 2 * sizeof(int)));
+    LUS_HIP(hipMemset(P.flag, 0, sizeof(int)));
+    {
+        unsigned short* fb = nullptr;  // bf16 storage
+        LUS_HIP(m.alloc(&fb, (size_t)n * lus::kFeat));
+        P.feat = fb;
+    }
     std::vector<float4> hx = to_f4(pos, n), hv = to_f4(vel, n);
     std::vector<int> hid((size_t)n);
     for (int i = 0; i < n; ++i) hid[i] = i;
@@ -841,16 +960,16 @@ int lus_pbf_time_kernels(LusGpu* c, int reps, float* us) {
             // xl, xa and x2 are per-step scratch between steps; x, v, id hold the state
             if (kind == 0)
                 LUS_HIP(lus::launch_pbf_lambda(P.xa, P.key2, P.cs, P.n, P.k, P.xl, P.maxc, P.masks, P.xp,
-                                               0, s));
+                                               0, nullptr, s));
             else if (kind == 1)
                 LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, P.k, P.xa, nullptr, nullptr,
-                                               1.0f, P.masks, P.xp, s));
+                                               1.0f, P.masks, P.xp, nullptr, s));
             else if (kind == 2)
                 LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, P.k, P.xl, P.x2, P.masks,
-                                             P.xp, s));
+                                             P.xp, nullptr, nullptr, s));
             else
                 LUS_HIP(lus::launch_pbf_lambda(P.xp, P.key2, P.cs, P.n, P.k, P.xl, P.maxc, P.masks, P.xp,
-                                               1, s));
+                                               1, nullptr, s));
         }
         LUS_HIP(hipEventRecord(e1, s));
         LUS_HIP(hipEventSynchronize(e1));
@@ -860,6 +979,419 @@ int lus_pbf_time_kernels(LusGpu* c, int reps, float* us) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return LUS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned warm start: network, mode, statistics, training
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kChunk = 256;  // split-K chunk of the weight-gradient products
+
+// device buffers of one forward/backward pass over up to B rows
+struct MlpBufs {
+    Arena mem;
+    int B = 0, chunks = 0;
+    unsigned short *Xb = nullptr, *H1 = nullptr, *H2 = nullptr, *d3 = nullptr, *dZ2 = nullptr,
+                   *dZ1 = nullptr, *pbf = nullptr;  // bf16 storage
+    float *Yb = nullptr, *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *dH2 = nullptr, *dH1 = nullptr,
+          *err = nullptr, *part = nullptr, *grad = nullptr, *acc = nullptr;
+    hipError_t alloc(int rows) {
+        B = rows;
+        chunks = (rows + kChunk - 1) / kChunk;
+        const size_t b = (size_t)rows;
+        hipError_t e;
+        if ((e = mem.alloc(&Xb, b * lus::kMlpIn)) || (e = mem.alloc(&H1, b * lus::kMlpHid)) ||
+            (e = mem.alloc(&H2, b * lus::kMlpHid)) || (e = mem.alloc(&d3, b * lus::kMlpOut)) ||
+            (e = mem.alloc(&dZ2, b * lus::kMlpHid)) || (e = mem.alloc(&dZ1, b * lus::kMlpHid)) ||
+            (e = mem.alloc(&pbf, lus::kMlpParams)) || (e = mem.alloc(&Yb, b * lus::kMlpOut)) ||
+            (e = mem.alloc(&Z1, b * lus::kMlpHid)) || (e = mem.alloc(&Z2, b * lus::kMlpHid)) ||
+            (e = mem.alloc(&Z3, b * lus::kMlpOut)) || (e = mem.alloc(&dH2, b * lus::kMlpHid)) ||
+            (e = mem.alloc(&dH1, b * lus::kMlpHid)) || (e = mem.alloc(&err, b)) ||
+            (e = mem.alloc(&part, (size_t)chunks * (lus::kMlpHid + 1) * lus::kMlpHid)) ||
+            (e = mem.alloc(&grad, lus::kMlpParams)) || (e = mem.alloc(&acc, 1)))
+            return e;
+        return hipSuccess;
+    }
+};
+
+// Forward pass of the training kernels over `rows` rows of b.Xb / b.Yb with
+// fp32 parameters p; err[r] = squared error of row r.  grad: also the
+// backward pass, gradient of the mean loss into b.grad.
+//   Z1 = X W1, H1 = bf16(silu(Z1 + b1)), Z2 = H1 W2, H2 = bf16(silu(Z2 + b2)),
+//   o = H2 W3 + b3;  dW|db = [A^T; 1] dZ per layer (bf16 operands)
+int mlp_pass(hipStream_t s, MlpBufs& b, const float* p, int rows, float inv_scale, bool grad) {
+    using namespace lus;
+    const unsigned short* W1 = b.pbf + kMlpW1;
+    const unsigned short* W2 = b.pbf + kMlpW2;
+    const unsigned short* W3 = b.pbf + kMlpW3;
+    LUS_HIP(launch_to_bf16(p, kMlpParams, b.pbf, s));
+    LUS_HIP(launch_gemm_bf16(b.Xb, kMlpIn, 1, W1, kMlpHid, 1, b.Z1, kMlpHid, rows, kMlpHid, kMlpIn, 32, 0, s));
+    LUS_HIP(launch_bias_silu(b.Z1, p + kMlpB1, rows, kMlpHid, b.H1, s));
+    LUS_HIP(launch_gemm_bf16(b.H1, kMlpHid, 1, W2, kMlpHid, 1, b.Z2, kMlpHid, rows, kMlpHid, kMlpHid, 64, 0, s));
+    LUS_HIP(launch_bias_silu(b.Z2, p + kMlpB2, rows, kMlpHid, b.H2, s));
+    LUS_HIP(launch_gemm_bf16(b.H2, kMlpHid, 1, W3, kMlpOut, 1, b.Z3, kMlpOut, rows, kMlpOut, kMlpHid, 64, 0, s));
+    LUS_HIP(launch_out_loss(b.Z3, p + kMlpB3, b.Yb, rows, inv_scale, b.d3, b.err, s));
+    if (!grad) return LUS_OK;
+    const int ch = (rows + kChunk - 1) / kChunk;
+    // layer 3: [H2^T; 1] d3 -> dW3|db3; dH2 = d3 W3^T
+    LUS_HIP(launch_gemm_bf16(b.H2, 1, kMlpHid, b.d3, kMlpOut, 1, b.part, kMlpOut, kMlpHid + 1, kMlpOut, rows,
+                             kChunk, 1, s));
+    LUS_HIP(launch_sum_partials(b.part, ch, (kMlpHid + 1) * kMlpOut, b.grad + kMlpW3, s));
+    LUS_HIP(launch_gemm_bf16(b.d3, kMlpOut, 1, W3, 1, kMlpOut, b.dH2, kMlpHid, rows, kMlpHid, kMlpOut, 32, 0, s));
+    LUS_HIP(launch_silu_back(b.dH2, b.Z2, (long)rows * kMlpHid, b.dZ2, s));
+    // layer 2
+    LUS_HIP(launch_gemm_bf16(b.H1, 1, kMlpHid, b.dZ2, kMlpHid, 1, b.part, kMlpHid, kMlpHid + 1, kMlpHid, rows,
+                             kChunk, 1, s));
+    LUS_HIP(launch_sum_partials(b.part, ch, (kMlpHid + 1) * kMlpHid, b.grad + kMlpW2, s));
+    LUS_HIP(launch_gemm_bf16(b.dZ2, kMlpHid, 1, W2, 1, kMlpHid, b.dH1, kMlpHid, rows, kMlpHid, kMlpHid, 64, 0, s));
+    LUS_HIP(launch_silu_back(b.dH1, b.Z1, (long)rows * kMlpHid, b.dZ1, s));
+    // layer 1
+    LUS_HIP(launch_gemm_bf16(b.Xb, 1, kMlpIn, b.dZ1, kMlpHid, 1, b.part, kMlpHid, kMlpIn + 1, kMlpHid, rows,
+                             kChunk, 1, s));
+    LUS_HIP(launch_sum_partials(b.part, ch, (kMlpIn + 1) * kMlpHid, b.grad + kMlpW1, s));
+    return LUS_OK;
+}
+
+bool net_layout_ok(int nfloats) { return nfloats == LUS_NET_FLOATS; }
+
+}  // namespace
+
+extern "C" {
+
+int lus_pbf_set_network(LusGpu* c, const float* net, int nfloats) {
+    if (!c || !net || !net_layout_ok(nfloats))
+        return set_error(LUS_EINVAL, "lus_pbf_set_network: need %d floats", LUS_NET_FLOATS);
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    pbf_drop_graph(c->pbf);  // out_scale is a kernel argument of a captured graph
+    NetState& N = c->net;
+    if (!N.params) {
+        unsigned short* pk = nullptr;
+        LUS_HIP(N.mem.alloc(&N.params, lus::kMlpParams));
+        LUS_HIP(N.mem.alloc(&pk, lus::kMlpPackBytes / 2));
+        LUS_HIP(N.mem.alloc(&N.mu, lus::kFeat));
+        LUS_HIP(N.mem.alloc(&N.inv_sigma, lus::kFeat));
+        N.packed = pk;
+    }
+    LUS_HIP(hipMemcpy(N.params, net + LUS_NET_PARAMS, sizeof(float) * lus::kMlpParams, hipMemcpyHostToDevice));
+    LUS_HIP(hipMemcpy(N.mu, net + LUS_NET_MU, sizeof(float) * lus::kFeat, hipMemcpyHostToDevice));
+    LUS_HIP(hipMemcpy(N.inv_sigma, net + LUS_NET_INV_SIGMA, sizeof(float) * lus::kFeat, hipMemcpyHostToDevice));
+    LUS_HIP(lus::launch_mlp_pack(N.params, N.packed, c->stream));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    N.out_scale = net[LUS_NET_OUT_SCALE];
+    N.ready = true;
+    return LUS_OK;
+}
+
+int lus_pbf_set_mode(LusGpu* c, int learned, int learned_iterations, float bound, int check) {
+    if (!c) return set_error(LUS_EINVAL, "lus_pbf_set_mode: NULL context");
+    if (learned && !c->net.ready) return set_error(LUS_ESTATE, "lus_pbf_set_mode: no network uploaded");
+    if (learned_iterations < 0 || learned_iterations > 64)
+        return set_error(LUS_EINVAL, "lus_pbf_set_mode: learned_iterations must be in [0, 64]");
+    PbfState& P = c->pbf;
+    if (learned && P.ready && P.p.iterations < 1)
+        return set_error(LUS_EINVAL, "lus_pbf_set_mode: the plain rerun needs iterations >= 1");
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    pbf_drop_graph(P);
+    P.learned = learned != 0;
+    P.learned_iters = learned_iterations;
+    P.bound = bound;
+    P.check = check != 0;
+    if (P.ready) {
+        LUS_HIP(hipMemset(P.counters, 0, 2 * sizeof(int)));
+        LUS_HIP(hipDeviceSynchronize());
+    }
+    return LUS_OK;
+}
+
+int lus_pbf_learned_stats(LusGpu* c, long long* steps, long long* reruns, float* errs, int cap, int* nerrs) {
+    if (!c || !c->pbf.ready) return set_error(LUS_ESTATE, "lus_pbf_learned_stats: no PBF scene");
+    PbfState& P = c->pbf;
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    int cnt[2];
+    LUS_HIP(hipMemcpy(cnt, P.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (steps) *steps = cnt[0];
+    if (reruns) *reruns = cnt[1];
+    int nh = std::min(cnt[0], LUS_DENSITY_HISTORY);
+    nh = std::min(nh, std::max(cap, 0));
+    if (errs && nh > 0) {
+        std::vector<float> h(LUS_DENSITY_HISTORY);
+        LUS_HIP(hipMemcpy(h.data(), P.hist, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+        for (int q = 0; q < nh; ++q) errs[q] = h[(size_t)((cnt[0] - nh + q) % LUS_DENSITY_HISTORY)];
+    }
+    if (nerrs) *nerrs = errs ? nh : 0;
+    return LUS_OK;
+}
+
+int lus_mlp_trainer_create(LusGpu* c, long capacity, LusMlpTrainer** out) {
+    if (!c || !out || capacity <= 0 || capacity > (1L << 28))
+        return set_error(LUS_EINVAL, "lus_mlp_trainer_create: bad arguments");
+    *out = nullptr;
+    LUS_HIP(hipSetDevice(c->device));
+    LusMlpTrainer* t = new LusMlpTrainer();
+    t->ctx = c;
+    t->cap = capacity;
+    hipError_t e = t->mem.alloc(&t->feat, (size_t)capacity * lus::kFeat);
+    if (e == hipSuccess) e = t->mem.alloc(&t->tgt, (size_t)capacity * 3);
+    if (e != hipSuccess) {
+        delete t;
+        return set_error(LUS_ENOMEM, "lus_mlp_trainer_create: %s", hipGetErrorString(e));
+    }
+    *out = t;
+    return LUS_OK;
+}
+
+void lus_mlp_trainer_destroy(LusMlpTrainer* t) {
+    if (!t) return;
+    (void)hipSetDevice(t->ctx->device);
+    (void)hipStreamSynchronize(t->ctx->stream);
+    t->mem.release();
+    delete t;
+}
+
+int lus_mlp_trainer_add(LusMlpTrainer* t, const float* feat, const float* target, long rows) {
+    if (!t || !feat || !target || rows < 0 || t->rows + rows > t->cap)
+        return set_error(LUS_EINVAL, "lus_mlp_trainer_add: bad arguments or capacity exceeded");
+    LUS_HIP(hipSetDevice(t->ctx->device));
+    LUS_HIP(hipStreamSynchronize(t->ctx->stream));
+    LUS_HIP(hipMemcpy(t->feat + (size_t)t->rows * lus::kFeat, feat, sizeof(float) * lus::kFeat * rows,
+                      hipMemcpyHostToDevice));
+    LUS_HIP(hipMemcpy(t->tgt + (size_t)t->rows * 3, target, sizeof(float) * 3 * rows, hipMemcpyHostToDevice));
+    t->rows += rows;
+    return LUS_OK;
+}
+
+int lus_mlp_trainer_rows(LusMlpTrainer* t, long* rows) {
+    if (!t || !rows) return set_error(LUS_EINVAL, "lus_mlp_trainer_rows: bad arguments");
+    *rows = t->rows;
+    return LUS_OK;
+}
+
+int lus_mlp_trainer_download(LusMlpTrainer* t, long first, long rows, float* feat, float* target) {
+    if (!t || first < 0 || rows < 0 || first + rows > t->rows)
+        return set_error(LUS_EINVAL, "lus_mlp_trainer_download: bad range");
+    LUS_HIP(hipSetDevice(t->ctx->device));
+    LUS_HIP(hipStreamSynchronize(t->ctx->stream));
+    if (feat)
+        LUS_HIP(hipMemcpy(feat, t->feat + (size_t)first * lus::kFeat, sizeof(float) * lus::kFeat * rows,
+                          hipMemcpyDeviceToHost));
+    if (target)
+        LUS_HIP(hipMemcpy(target, t->tgt + (size_t)first * 3, sizeof(float) * 3 * rows, hipMemcpyDeviceToHost));
+    return LUS_OK;
+}
+
+int lus_pbf_record_step(LusGpu* c, LusMlpTrainer* t, float dt, int stride, long* added) {
+    if (!c || !t || t->ctx != c || stride < 1 || !(dt > 0.0f))
+        return set_error(LUS_EINVAL, "lus_pbf_record_step: bad arguments");
+    PbfState& P = c->pbf;
+    if (!P.ready) return set_error(LUS_ESTATE, "lus_pbf_record_step: no PBF scene");
+    if (P.p.iterations < 1) return set_error(LUS_EINVAL, "lus_pbf_record_step: needs iterations >= 1");
+    const int off = t->rec_calls % stride;
+    const int p0 = (stride - off % stride) % stride;
+    const long count = p0 < P.n ? (long)(P.n - 1 - p0) / stride + 1 : 0;
+    if (t->rows + count > t->cap)
+        return set_error(LUS_EINVAL, "lus_pbf_record_step: trainer capacity exceeded (%ld + %ld > %ld)", t->rows,
+                         count, t->cap);
+    LUS_HIP(hipSetDevice(c->device));
+    Record rec{t->feat + (size_t)t->rows * lus::kFeat, t->tgt + (size_t)t->rows * 3, stride, off};
+    int r = pbf_enqueue(c, dt, nullptr, nullptr, &rec);
+    if (r) return r;
+    t->rows += count;
+    t->rec_calls += 1;
+    if (added) *added = count;
+    return LUS_OK;
+}
+
+int lus_mlp_fit(LusMlpTrainer* t, const LusMlpFitParams* fp, float* train_loss, float* val_loss) {
+    if (!t || !fp || fp->epochs < 1 || fp->epochs > 100000 || fp->batch < 32 || fp->batch % 32 ||
+        fp->batch > (1 << 16) || !(fp->lr > 0.0f) || !(fp->lr_final >= 0.0f) || !(fp->val_fraction >= 0.0f) ||
+        !(fp->val_fraction <= 0.5f))
+        return set_error(LUS_EINVAL, "lus_mlp_fit: bad arguments");
+    using namespace lus;
+    const long rows = t->rows;
+    const long nval = (long)((double)rows * fp->val_fraction);
+    const long ntr = rows - nval;
+    const int B = fp->batch;
+    if (ntr < B) return set_error(LUS_EINVAL, "lus_mlp_fit: %ld training rows < batch %d", ntr, B);
+    LusGpu* c = t->ctx;
+    hipStream_t s = c->stream;
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(s));
+    Arena m;
+    // normalisation of the inputs and scale of the targets, from the whole set
+    double* st = nullptr;
+    LUS_HIP(m.alloc(&st, 2 * kFeat + 6));
+    LUS_HIP(launch_col_stats(t->feat, rows, kFeat, st, s));
+    LUS_HIP(launch_col_stats(t->tgt, rows, 3, st + 2 * kFeat, s));
+    std::vector<double> hs(2 * kFeat + 6);
+    LUS_HIP(hipStreamSynchronize(s));
+    LUS_HIP(hipMemcpy(hs.data(), st, sizeof(double) * hs.size(), hipMemcpyDeviceToHost));
+    std::vector<float> mu(kFeat), inv(kFeat);
+    for (int f = 0; f < kFeat; ++f) {
+        const double mean = hs[2 * f] / (double)rows;
+        const double var = std::max(hs[2 * f + 1] / (double)rows - mean * mean, 0.0);
+        mu[f] = (float)mean;
+        inv[f] = (float)(1.0 / std::max(std::sqrt(var), 1e-4));
+    }
+    double ss = 0.0;
+    for (int q = 0; q < 3; ++q) ss += hs[2 * kFeat + 2 * q + 1];
+    float out_scale = (float)std::sqrt(ss / (3.0 * (double)rows));
+    if (!(out_scale > 0.0f)) out_scale = 1.0f;
+    const float inv_scale = 1.0f / out_scale;
+    // initial parameters: LeCun-uniform, last layer scaled by 0.1, zero biases
+    std::mt19937_64 rng(fp->seed);
+    std::vector<float> p((size_t)kMlpParams, 0.0f);
+    auto fill = [&](int off, int fan_in, int count, float gain) {
+        const float a = gain * (float)std::sqrt(3.0 / fan_in);
+        std::uniform_real_distribution<float> u(-a, a);
+        for (int q = 0; q < count; ++q) p[(size_t)off + q] = u(rng);
+    };
+    fill(kMlpW1, kMlpIn, kMlpIn * kMlpHid, 1.0f);
+    fill(kMlpW2, kMlpHid, kMlpHid * kMlpHid, 1.0f);
+    fill(kMlpW3, kMlpHid, kMlpHid * kMlpOut, 0.1f);
+    // random split
+    std::vector<int> order((size_t)rows);
+    for (long r = 0; r < rows; ++r) order[r] = (int)r;
+    std::shuffle(order.begin(), order.end(), rng);
+    std::vector<int> val(order.begin(), order.begin() + nval), tr(order.begin() + nval, order.end());
+    float *dp, *dm, *dv, *dmu, *dinv, *tl, *vl;
+    int *dtr, *dval;
+    LUS_HIP(m.alloc(&dp, kMlpParams));
+    LUS_HIP(m.alloc(&dm, kMlpParams));
+    LUS_HIP(m.alloc(&dv, kMlpParams));
+    LUS_HIP(m.alloc(&dmu, kFeat));
+    LUS_HIP(m.alloc(&dinv, kFeat));
+    LUS_HIP(m.alloc(&tl, fp->epochs));
+    LUS_HIP(m.alloc(&vl, fp->epochs));
+    LUS_HIP(m.alloc(&dtr, ntr));
+    LUS_HIP(m.alloc(&dval, std::max(nval, 1L)));
+    LUS_HIP(hipMemcpy(dp, p.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    LUS_HIP(hipMemset(dm, 0, sizeof(float) * kMlpParams));
+    LUS_HIP(hipMemset(dv, 0, sizeof(float) * kMlpParams));
+    LUS_HIP(hipMemset(tl, 0, sizeof(float) * fp->epochs));
+    LUS_HIP(hipMemset(vl, 0, sizeof(float) * fp->epochs));
+    LUS_HIP(hipMemcpy(dmu, mu.data(), sizeof(float) * kFeat, hipMemcpyHostToDevice));
+    LUS_HIP(hipMemcpy(dinv, inv.data(), sizeof(float) * kFeat, hipMemcpyHostToDevice));
+    if (nval) LUS_HIP(hipMemcpy(dval, val.data(), sizeof(int) * nval, hipMemcpyHostToDevice));
+    MlpBufs b;
+    LUS_HIP(b.alloc(B));
+    LUS_HIP(hipDeviceSynchronize());
+    const long nb = ntr / B;
+    const double total = (double)fp->epochs * (double)nb;
+    long step = 0;
+    for (int e = 0; e < fp->epochs; ++e) {
+        std::shuffle(tr.begin(), tr.end(), rng);
+        LUS_HIP(hipStreamSynchronize(s));
+        LUS_HIP(hipMemcpy(dtr, tr.data(), sizeof(int) * ntr, hipMemcpyHostToDevice));
+        for (long q = 0; q < nb; ++q) {
+            LUS_HIP(launch_normalize_rows(t->feat, dtr + q * B, B, dmu, dinv, b.Xb, s));
+            LUS_HIP(launch_gather_rows(t->tgt, dtr + q * B, B, 3, b.Yb, s));
+            if (int r = mlp_pass(s, b, dp, B, inv_scale, true)) return r;
+            ++step;
+            const double cosw = 0.5 * (1.0 + std::cos(M_PI * (double)(step - 1) / total));
+            const float lr = (float)(fp->lr_final + (fp->lr - fp->lr_final) * cosw);
+            const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)step)));
+            const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)step)));
+            LUS_HIP(launch_adam(dp, b.grad, dm, dv, kMlpParams, lr, 0.9f, 0.999f, 1e-8f, c1, c2, s));
+            LUS_HIP(launch_sum_into(b.err, B, tl + e, s));
+        }
+        for (long q = 0; q < nval; q += B) {
+            const int rr = (int)std::min<long>(B, nval - q);
+            LUS_HIP(launch_normalize_rows(t->feat, dval + q, rr, dmu, dinv, b.Xb, s));
+            LUS_HIP(launch_gather_rows(t->tgt, dval + q, rr, 3, b.Yb, s));
+            if (int r = mlp_pass(s, b, dp, rr, inv_scale, false)) return r;
+            LUS_HIP(launch_sum_into(b.err, rr, vl + e, s));
+        }
+    }
+    LUS_HIP(hipStreamSynchronize(s));
+    std::vector<float> htl(fp->epochs), hvl(fp->epochs);
+    LUS_HIP(hipMemcpy(htl.data(), tl, sizeof(float) * fp->epochs, hipMemcpyDeviceToHost));
+    LUS_HIP(hipMemcpy(hvl.data(), vl, sizeof(float) * fp->epochs, hipMemcpyDeviceToHost));
+    LUS_HIP(hipMemcpy(p.data(), dp, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+    for (int e = 0; e < fp->epochs; ++e) {
+        if (train_loss) train_loss[e] = htl[e] / (float)(nb * B);
+        if (val_loss) val_loss[e] = nval ? hvl[e] / (float)nval : NAN;
+    }
+    t->net.assign((size_t)LUS_NET_FLOATS, 0.0f);
+    std::copy(mu.begin(), mu.end(), t->net.begin() + LUS_NET_MU);
+    std::copy(inv.begin(), inv.end(), t->net.begin() + LUS_NET_INV_SIGMA);
+    std::copy(p.begin(), p.end(), t->net.begin() + LUS_NET_PARAMS);
+    t->net[LUS_NET_OUT_SCALE] = out_scale;
+    t->has_net = true;
+    return LUS_OK;
+}
+
+int lus_mlp_trainer_network(LusMlpTrainer* t, float* net) {
+    if (!t || !net) return set_error(LUS_EINVAL, "lus_mlp_trainer_network: bad arguments");
+    if (!t->has_net) return set_error(LUS_ESTATE, "lus_mlp_trainer_network: no fitted network");
+    std::copy(t->net.begin(), t->net.end(), net);
+    return LUS_OK;
+}
+
+int lus_mlp_loss_grad(LusGpu* c, const float* params, int rows, const float* X, const float* Y, float inv_scale,
+                      float* loss, float* grad) {
+    if (!c || !params || !X || !Y || rows < 1 || rows > (1 << 20))
+        return set_error(LUS_EINVAL, "lus_mlp_loss_grad: bad arguments");
+    using namespace lus;
+    hipStream_t s = c->stream;
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(s));
+    Arena m;
+    float *dp, *dx;
+    LUS_HIP(m.alloc(&dp, kMlpParams));
+    LUS_HIP(m.alloc(&dx, (size_t)rows * kMlpIn));
+    MlpBufs b;
+    LUS_HIP(b.alloc(rows));
+    LUS_HIP(hipMemcpy(dp, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    LUS_HIP(hipMemcpy(dx, X, sizeof(float) * kMlpIn * rows, hipMemcpyHostToDevice));
+    LUS_HIP(hipMemcpy(b.Yb, Y, sizeof(float) * kMlpOut * rows, hipMemcpyHostToDevice));
+    LUS_HIP(hipMemset(b.acc, 0, sizeof(float)));
+    LUS_HIP(hipDeviceSynchronize());
+    LUS_HIP(launch_to_bf16(dx, (long)rows * kMlpIn, b.Xb, s));
+    if (int r = mlp_pass(s, b, dp, rows, inv_scale, true)) return r;
+    LUS_HIP(launch_sum_into(b.err, rows, b.acc, s));
+    LUS_HIP(hipStreamSynchronize(s));
+    float acc = 0.0f;
+    LUS_HIP(hipMemcpy(&acc, b.acc, sizeof(float), hipMemcpyDeviceToHost));
+    if (loss) *loss = acc / (float)rows;
+    if (grad) LUS_HIP(hipMemcpy(grad, b.grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+    return LUS_OK;
+}
+
+int lus_mlp_infer(LusGpu* c, const float* net, int rows, const float* feat, float* out) {
+    if (!c || !net || !feat || !out || rows < 1 || rows > (1 << 24))
+        return set_error(LUS_EINVAL, "lus_mlp_infer: bad arguments");
+    using namespace lus;
+    hipStream_t s = c->stream;
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(s));
+    Arena m;
+    float *dp, *dmu, *dinv, *dx, *dout;
+    unsigned short *pk, *xb;
+    LUS_HIP(m.alloc(&dp, kMlpParams));
+    LUS_HIP(m.alloc(&dmu, kFeat));
+    LUS_HIP(m.alloc(&dinv, kFeat));
+    LUS_HIP(m.alloc(&dx, (size_t)rows * kFeat));
+    LUS_HIP(m.alloc(&dout, (size_t)rows * 3));
+    LUS_HIP(m.alloc(&pk, kMlpPackBytes / 2));
+    LUS_HIP(m.alloc(&xb, (size_t)rows * kFeat));
+    LUS_HIP(hipMemcpy(dp, net + LUS_NET_PARAMS, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    LUS_HIP(hipMemcpy(dmu, net + LUS_NET_MU, sizeof(float) * kFeat, hipMemcpyHostToDevice));
+    LUS_HIP(hipMemcpy(dinv, net + LUS_NET_INV_SIGMA, sizeof(float) * kFeat, hipMemcpyHostToDevice));
+    LUS_HIP(hipMemcpy(dx, feat, sizeof(float) * kFeat * rows, hipMemcpyHostToDevice));
+    LUS_HIP(hipDeviceSynchronize());
+    LUS_HIP(launch_mlp_pack(dp, pk, s));
+    LUS_HIP(launch_normalize_rows(dx, nullptr, rows, dmu, dinv, xb, s));
+    PbfK k{};
+    LUS_HIP(launch_mlp_forward(xb, rows, dp, pk, net[LUS_NET_OUT_SCALE], dout, nullptr, nullptr, k, s));
+    LUS_HIP(hipStreamSynchronize(s));
+    LUS_HIP(hipMemcpy(out, dout, sizeof(float) * 3 * rows, hipMemcpyDeviceToHost));
     return LUS_OK;
 }
 

@@ -72,25 +72,84 @@ hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, cons
                             LusGrid g, float h2, const int* inv_perm, const long* off, int* cnt,
                             int* idx, int fill, hipStream_t s);
 
-// PBF substep
+// PBF substep.  `gate` (nullable): a device int; the kernel does nothing when
+// it holds 0 (the conditional plain rerun of the learned step).
 hipError_t launch_pbf_key(const float4* x, const float4* v, int n, PbfK k, float dt, int* key,
                           int* slot, int* counts, hipStream_t s);
 hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_start,
                               const float4* x, const float4* v, const int* id, int n, PbfK k,
                               float dt, float4* x2, float4* xp, int* id2, int* key2,
-                              hipStream_t s);
+                              const int* gate, hipStream_t s);
 // masks: 9 * n neighbour-row masks; build = 1 on the first lambda pass of a step
 hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
                              int* maxc, unsigned long long* masks, const float4* xp, int build,
-                             hipStream_t s);
+                             const int* gate, hipStream_t s);
 hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
                              const float4* xold, float4* vout, float dt, unsigned long long* masks,
-                             const float4* xp, hipStream_t s);
+                             const float4* xp, const int* gate, hipStream_t s);
 hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, float dt,
                                float4* vout, hipStream_t s);
+// XSPH + commit; maxd (nullable): per-block max of the density constraint
+// C_i = rho_i / rho0 - 1 at the final positions (one int per block, ordered())
 hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
                            PbfK k, float4* vout, float4* xout, unsigned long long* masks,
-                           const float4* xp, hipStream_t s);
+                           const float4* xp, int* maxd, const int* gate, hipStream_t s);
+
+// ---- learned warm start ----
+// Per-particle features of the predicted positions (build pass: also writes
+// the neighbour masks, like the first lambda pass).  Exactly one of
+//   fbf:  bf16 [n][32], (raw - mu) * inv_sigma       (mu/inv_sigma: 32 floats each)
+//   frec: fp32 [rows][32] raw, only slots p with (p + rec_off) % rec_stride == 0,
+//         row (p - p0) / rec_stride, p0 the first such slot
+// is non-null.
+constexpr int kFeat = 32;
+hipError_t launch_pbf_features(int* key2, const int* cs, int n, PbfK k, unsigned long long* masks,
+                               const float4* xp, const float4* xold, const float* mu,
+                               const float* inv_sigma, void* fbf, float* frec, int rec_stride,
+                               int rec_off, hipStream_t s);
+// training target of the recorded slots: tgt[row] = xa[p] - xp[p]
+hipError_t launch_pbf_record_target(const float4* xa, const float4* xp, int n, int rec_stride,
+                                    int rec_off, float* tgt, hipStream_t s);
+// reduce the XSPH block maxima: e = max(0, max C); hist[*counter % cap] = e;
+// flag = learned && !(e <= bound); counters[0] += 1, counters[1] += flag
+hipError_t launch_pbf_decide(const int* maxd, int nblk, float bound, int learned, int* flag,
+                             int* counters, float* hist, int cap, hipStream_t s);
+
+// MLP 32 -> 64 (SiLU) -> 64 (SiLU) -> 3, parameters (fp32) in this order:
+constexpr int kMlpIn = 32, kMlpHid = 64, kMlpOut = 3;
+constexpr int kMlpW1 = 0;                              // [32][64]
+constexpr int kMlpB1 = kMlpW1 + kMlpIn * kMlpHid;      // [64]
+constexpr int kMlpW2 = kMlpB1 + kMlpHid;               // [64][64]
+constexpr int kMlpB2 = kMlpW2 + kMlpHid * kMlpHid;     // [64]
+constexpr int kMlpW3 = kMlpB2 + kMlpHid;               // [64][3]
+constexpr int kMlpB3 = kMlpW3 + kMlpHid * kMlpOut;     // [3]
+constexpr int kMlpParams = kMlpB3 + kMlpOut;           // 6467
+// bf16 B-operand images of W1, W2, W3 (bytes)
+constexpr int kMlpPackBytes = (4 + 8 + 2) * 64 * 16;
+
+hipError_t launch_mlp_pack(const float* params, void* packed, hipStream_t s);
+// out (nullable): [n][3] = out_scale * mlp(X); xa (nullable): clamp(xp + that)
+hipError_t launch_mlp_forward(const void* X, int n, const float* params, const void* packed,
+                              float out_scale, float* out, const float4* xp, float4* xa,
+                              const PbfK& k, hipStream_t s);
+// training kernels (row-major fp32 / bf16 matrices)
+hipError_t launch_gemm_bf16(const void* A, long sam, long sak, const void* B, long sbk, long sbn,
+                            float* C, int ldc, int M, int N, int K, int kchunk, int ones_row,
+                            hipStream_t s);
+hipError_t launch_sum_partials(const float* part, int chunks, int MN, float* out, hipStream_t s);
+hipError_t launch_bias_silu(float* z, const float* bias, int rows, int N, void* h, hipStream_t s);
+hipError_t launch_out_loss(const float* z3, const float* b3, const float* y, int rows,
+                           float inv_scale, void* dbf, float* err, hipStream_t s);
+hipError_t launch_silu_back(const float* dh, const float* z, long tot, void* dzb, hipStream_t s);
+hipError_t launch_to_bf16(const float* a, long tot, void* b, hipStream_t s);
+hipError_t launch_normalize_rows(const float* x, const int* rows, long nrows, const float* mu,
+                                 const float* inv_sigma, void* xb, hipStream_t s);
+hipError_t launch_gather_rows(const float* y, const int* rows, long nrows, int w, float* out,
+                              hipStream_t s);
+hipError_t launch_sum_into(const float* v, int n, float* acc, hipStream_t s);
+hipError_t launch_col_stats(const float* x, long rows, int w, double* out, hipStream_t s);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, int np, float lr, float b1,
+                       float b2, float eps, float c1, float c2, hipStream_t s);
 
 // sand (simulate_sand)
 hipError_t launch_sand_predict_key(float4* x, float4* v, int* att, int n, SandK k, float dt,

@@ -230,7 +230,8 @@ __global__ __launch_bounds__(kBlock) void k_pbf_permute(
     const int* __restrict__ tmp, const int* __restrict__ key, const int* __restrict__ cell_start,
     const float4* __restrict__ x, const float4* __restrict__ v, const int* __restrict__ id, int n,
     PbfK k, float dt, float4* __restrict__ x2, float4* __restrict__ xp, int* __restrict__ id2,
-    int* __restrict__ key2) {
+    int* __restrict__ key2, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int p = xcd_index();
     if (p >= n) return;
     const int i = tmp[p];
@@ -248,9 +249,9 @@ __global__ __launch_bounds__(kBlock) void k_pbf_permute(
 hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_start,
                               const float4* x, const float4* v, const int* id, int n, PbfK k,
                               float dt, float4* x2, float4* xp, int* id2, int* key2,
-                              hipStream_t s) {
+                              const int* gate, hipStream_t s) {
     hipLaunchKernelGGL(k_pbf_permute, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start,
-                       x, v, id, n, k, dt, x2, xp, id2, key2);
+                       x, v, id, n, k, dt, x2, xp, id2, key2, gate);
     return hipGetLastError();
 }
 
@@ -287,6 +288,27 @@ __device__ __forceinline__ float poly_t(float r2, const PbfK& k) {  // hf^2 - f^
     return k.hf2 - k.f2 * r2;
 }
 
+// Block max of C, one plain store per block (out[blockIdx.x]); the consumer
+// reduces the few thousand block values when it needs them.  Every thread of
+// the block must call it.
+__device__ __forceinline__ void block_max_store(float c, int* __restrict__ out) {
+    __shared__ int wmax[kBlock / 64];
+    int m = ordered(c);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        int o = __shfl_xor(m, d, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = wmax[0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; ++w) b = wmax[w] > b ? wmax[w] : b;
+        out[blockIdx.x] = b;
+    }
+}
+
 // lambda pass: density constraint and its gradient norm (PBF eq. 1, 8, 11)
 template <bool kBuild, bool kFast>
 __global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict__ xa,
@@ -295,7 +317,9 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict_
                                                         float4* __restrict__ xl,
                                                         int* __restrict__ maxc,
                                                         unsigned long long* __restrict__ masks,
-                                                        const float4* __restrict__ xp) {
+                                                        const float4* __restrict__ xp,
+                                                        const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;  // uniform: before any barrier
     const int i = xcd_index();
     float Ci = -1e30f;
     if (i < n) {
@@ -324,37 +348,21 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict_
         const float den = k.mr0sq * (a + gg) + k.eps;
         xl[i] = make_float4(pi.x, pi.y, pi.z, -Ci / den);
     }
-    // block max of C, one plain store per block (blockmax[blockIdx.x]); the
-    // consumer reduces the few thousand block values when it needs them
-    __shared__ int wmax[kBlock / 64];
-    int m = ordered(Ci);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        int o = __shfl_xor(m, d, 64);
-        m = o > m ? o : m;
-    }
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int b = wmax[0];
-#pragma unroll
-        for (int w = 1; w < kBlock / 64; ++w) b = wmax[w] > b ? wmax[w] : b;
-        maxc[blockIdx.x] = b;
-    }
+    block_max_store(Ci, maxc);
 }
 
 hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
                              int* maxc, unsigned long long* masks, const float4* xp, int build,
-                             hipStream_t s) {
+                             const int* gate, hipStream_t s) {
     const dim3 g(grid_for(n)), b(kBlock);
     if (build && k.exact)
-        hipLaunchKernelGGL((k_pbf_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp);
+        hipLaunchKernelGGL((k_pbf_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
     else if (build)
-        hipLaunchKernelGGL((k_pbf_lambda<true, true>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp);
+        hipLaunchKernelGGL((k_pbf_lambda<true, true>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
     else if (k.exact)
-        hipLaunchKernelGGL((k_pbf_lambda<false, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp);
+        hipLaunchKernelGGL((k_pbf_lambda<false, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
     else
-        hipLaunchKernelGGL((k_pbf_lambda<false, true>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp);
+        hipLaunchKernelGGL((k_pbf_lambda<false, true>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
     return hipGetLastError();
 }
 
@@ -368,7 +376,9 @@ __global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict_
                                                         const float4* __restrict__ xold,
                                                         float4* __restrict__ vout, float dt,
                                                         unsigned long long* __restrict__ masks,
-                                                        const float4* __restrict__ xp) {
+                                                        const float4* __restrict__ xp,
+                                                        const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int i = xcd_index();
     if (i >= n) return;
     const float4 pi = xl[i];
@@ -416,17 +426,17 @@ __global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict_
 
 hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
                              const float4* xold, float4* vout, float dt, unsigned long long* masks,
-                             const float4* xp, hipStream_t s) {
+                             const float4* xp, const int* gate, hipStream_t s) {
     const dim3 g(grid_for(n)), b(kBlock);
     const bool n4 = k.n_scorr == 4;
     if (k.exact && n4)
-        hipLaunchKernelGGL((k_pbf_deltap<false, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+        hipLaunchKernelGGL((k_pbf_deltap<false, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
     else if (k.exact)
-        hipLaunchKernelGGL((k_pbf_deltap<false, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+        hipLaunchKernelGGL((k_pbf_deltap<false, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
     else if (n4)
-        hipLaunchKernelGGL((k_pbf_deltap<true, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+        hipLaunchKernelGGL((k_pbf_deltap<true, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
     else
-        hipLaunchKernelGGL((k_pbf_deltap<true, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+        hipLaunchKernelGGL((k_pbf_deltap<true, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
     return hipGetLastError();
 }
 
@@ -446,8 +456,11 @@ hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, floa
     return hipGetLastError();
 }
 
-// XSPH viscosity (PBF eq. 17) over N(i) and commit x = x*
-template <bool kFast>
+// XSPH viscosity (PBF eq. 17) over N(i) and commit x = x*.  kCheck: the same
+// kernel sums give the density at the final positions; its constraint's block
+// maximum goes to maxd (the learned step's fallback test and the density
+// error statistics).
+template <bool kFast, bool kCheck>
 __global__ __launch_bounds__(kBlock) void k_pbf_xsph(const float4* __restrict__ xa,
                                                       const float4* __restrict__ vt,
                                                       int* __restrict__ key2,
@@ -455,38 +468,225 @@ __global__ __launch_bounds__(kBlock) void k_pbf_xsph(const float4* __restrict__ 
                                                       float4* __restrict__ vout,
                                                       float4* __restrict__ xout,
                                                       unsigned long long* __restrict__ masks,
-                                                      const float4* __restrict__ xp) {
+                                                      const float4* __restrict__ xp,
+                                                      int* __restrict__ maxd,
+                                                      const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;  // uniform: before any barrier
     const int i = xcd_index();
-    if (i >= n) return;
-    const float4 pi = xa[i];
-    const float4 vi = vt[i];
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for_each_neighbor<false>(i, n, key2, cs, masks, xp, xa, vt, k,
-                             [&](int, const float4 pj, const float4 vj) {
-        const float dx = pi.x - pj.x;
-        const float dy = pi.y - pj.y;
-        const float dz = pi.z - pj.z;
-        const float r2 = dist2<kFast>(dx, dy, dz);
-        if (r2 <= k.h2) {
-            const float t = poly_t<kFast>(r2, k);
-            const float w = (t * t) * t;
-            sx = acc<kFast>(sx, vj.x - vi.x, w);
-            sy = acc<kFast>(sy, vj.y - vi.y, w);
-            sz = acc<kFast>(sz, vj.z - vi.z, w);
-        }
-    });
-    vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
-    xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
+    float Ci = -1e30f;
+    if (i < n) {
+        const float4 pi = xa[i];
+        const float4 vi = vt[i];
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f, rho = 0.0f;
+        for_each_neighbor<false>(i, n, key2, cs, masks, xp, xa, vt, k,
+                                 [&](int, const float4 pj, const float4 vj) {
+            const float dx = pi.x - pj.x;
+            const float dy = pi.y - pj.y;
+            const float dz = pi.z - pj.z;
+            const float r2 = dist2<kFast>(dx, dy, dz);
+            if (r2 <= k.h2) {
+                const float t = poly_t<kFast>(r2, k);
+                const float w = (t * t) * t;
+                sx = acc<kFast>(sx, vj.x - vi.x, w);
+                sy = acc<kFast>(sy, vj.y - vi.y, w);
+                sz = acc<kFast>(sz, vj.z - vi.z, w);
+                if (kCheck) rho += w;
+            }
+        });
+        vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
+        xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
+        Ci = rho * k.c_rho - 1.0f;
+        Ci = (Ci == Ci) ? Ci : __builtin_inff();  // a NaN must fail the density test
+    }
+    if (kCheck) block_max_store(Ci, maxd);
 }
 
 hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
                            PbfK k, float4* vout, float4* xout, unsigned long long* masks,
-                           const float4* xp, hipStream_t s) {
+                           const float4* xp, int* maxd, const int* gate, hipStream_t s) {
     const dim3 g(grid_for(n)), b(kBlock);
-    if (k.exact)
-        hipLaunchKernelGGL((k_pbf_xsph<false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp);
+    if (k.exact && maxd)
+        hipLaunchKernelGGL((k_pbf_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+    else if (k.exact)
+        hipLaunchKernelGGL((k_pbf_xsph<false, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+    else if (maxd)
+        hipLaunchKernelGGL((k_pbf_xsph<true, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
     else
-        hipLaunchKernelGGL((k_pbf_xsph<true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp);
+        hipLaunchKernelGGL((k_pbf_xsph<true, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// learned warm start: features, training targets, fallback decision
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+// Features of particle i at the predicted positions x*, over N(i):
+//   f[0]        C_i = rho_i / rho0 - 1 (poly6, self included)
+//   f[1..3]     (x*_i - x_i) / h, the predicted displacement of the step
+//   f[4+3b..]   b = 0..7: mean offset (x*_j - x*_i) / h over j != i with
+//               b = min(7, (int)(8 |d| / h)) (0 when the bin is empty)
+//   f[28]       |N(i) \ i| / 32
+//   f[29]       mean |d| / h over N(i) \ i
+//   f[30]       mean |d|^2 / h^2 over N(i) \ i
+//   f[31]       min(1, distance to the nearest wall of the clamp box / h)
+template <bool kFast, bool kRecord>
+__global__ __launch_bounds__(kBlock) void k_pbf_features(
+    int* __restrict__ key2, const int* __restrict__ cs, int n, PbfK k,
+    unsigned long long* __restrict__ masks, const float4* __restrict__ xp,
+    const float4* __restrict__ xold, const float* __restrict__ mu, const float* __restrict__ inv_sigma,
+    bf16x8_t* __restrict__ fbf, float* __restrict__ frec, int rec_stride, int rec_off, float inv_h) {
+    const int i = xcd_index();
+    if (i >= n) return;
+    const float4 pi = xp[i];
+    float rho = 0.0f, cnt = 0.0f, sr = 0.0f, sr2 = 0.0f;
+    float bx[8], by[8], bz[8], bn[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) bx[b] = by[b] = bz[b] = bn[b] = 0.0f;
+    for_each_neighbor<true>(i, n, key2, cs, masks, xp, xp, nullptr, k,
+                            [&](int j, const float4 pj, const float4) {
+        const float dx = pj.x - pi.x;
+        const float dy = pj.y - pi.y;
+        const float dz = pj.z - pi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float t = poly_t<kFast>(r2, k);
+        rho = acc<kFast>(rho, t * t, t);
+        if (j != i) {
+            const float q = sqrtf(r2) * inv_h;
+            int b = (int)(q * 8.0f);
+            b = b < 7 ? b : 7;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float on = (b == u) ? 1.0f : 0.0f;
+                bx[u] += on * dx;
+                by[u] += on * dy;
+                bz[u] += on * dz;
+                bn[u] += on;
+            }
+            cnt += 1.0f;
+            sr += q;
+            sr2 += q * q;
+        }
+    });
+    float f[kFeat];
+    f[0] = rho * k.c_rho - 1.0f;
+    const float4 xo = xold[i];
+    f[1] = (pi.x - xo.x) * inv_h;
+    f[2] = (pi.y - xo.y) * inv_h;
+    f[3] = (pi.z - xo.z) * inv_h;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const float s = bn[b] > 0.0f ? inv_h / bn[b] : 0.0f;
+        f[4 + 3 * b] = bx[b] * s;
+        f[5 + 3 * b] = by[b] * s;
+        f[6 + 3 * b] = bz[b] * s;
+    }
+    f[28] = cnt * (1.0f / 32.0f);
+    f[29] = cnt > 0.0f ? sr / cnt : 0.0f;
+    f[30] = cnt > 0.0f ? sr2 / cnt : 0.0f;
+    const Bounds bb = bounds_of(k, k.scenes ? k.sid_out[i] : 0);
+    float wall = pi.x - bb.lo[0];
+    wall = fminf(wall, bb.hi[0] - pi.x);
+    wall = fminf(wall, pi.y - bb.lo[1]);
+    wall = fminf(wall, bb.hi[1] - pi.y);
+    wall = fminf(wall, pi.z - bb.lo[2]);
+    wall = fminf(wall, bb.hi[2] - pi.z);
+    f[31] = fminf(1.0f, wall * inv_h);
+    if (kRecord) {
+        const int p0 = (rec_stride - rec_off % rec_stride) % rec_stride;
+        if (i < p0 || (i - p0) % rec_stride != 0) return;
+        float4* row = reinterpret_cast<float4*>(frec + (size_t)((i - p0) / rec_stride) * kFeat);
+#pragma unroll
+        for (int q = 0; q < kFeat / 4; ++q) row[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < kFeat / 8; ++q) {
+            bf16x8_t v;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (__bf16)((f[8 * q + e] - mu[8 * q + e]) * inv_sigma[8 * q + e]);
+            fbf[(size_t)i * (kFeat / 8) + q] = v;
+        }
+    }
+}
+
+hipError_t launch_pbf_features(int* key2, const int* cs, int n, PbfK k, unsigned long long* masks,
+                               const float4* xp, const float4* xold, const float* mu,
+                               const float* inv_sigma, void* fbf, float* frec, int rec_stride,
+                               int rec_off, hipStream_t s) {
+    const dim3 g(grid_for(n)), b(kBlock);
+    const float inv_h = 1.0f / sqrtf(k.h2);
+    if (frec) {
+        if (rec_stride < 1) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_pbf_features<false, true>), g, b, 0, s, key2, cs, n, k, masks, xp, xold, mu,
+                           inv_sigma, (bf16x8_t*)fbf, frec, rec_stride, rec_off, inv_h);
+    } else if (k.exact) {
+        hipLaunchKernelGGL((k_pbf_features<false, false>), g, b, 0, s, key2, cs, n, k, masks, xp, xold, mu,
+                           inv_sigma, (bf16x8_t*)fbf, frec, 1, 0, inv_h);
+    } else {
+        hipLaunchKernelGGL((k_pbf_features<true, false>), g, b, 0, s, key2, cs, n, k, masks, xp, xold, mu,
+                           inv_sigma, (bf16x8_t*)fbf, frec, 1, 0, inv_h);
+    }
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_pbf_record_target(const float4* __restrict__ xa,
+                                                               const float4* __restrict__ xp, int n,
+                                                               int rec_stride, int rec_off,
+                                                               float* __restrict__ tgt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p0 = (rec_stride - rec_off % rec_stride) % rec_stride;
+    const int p = p0 + i * rec_stride;
+    if (p >= n) return;
+    const float4 a = xa[p], b = xp[p];
+    tgt[(size_t)i * 3 + 0] = a.x - b.x;
+    tgt[(size_t)i * 3 + 1] = a.y - b.y;
+    tgt[(size_t)i * 3 + 2] = a.z - b.z;
+}
+
+hipError_t launch_pbf_record_target(const float4* xa, const float4* xp, int n, int rec_stride,
+                                    int rec_off, float* tgt, hipStream_t s) {
+    if (rec_stride < 1) return hipErrorInvalidValue;
+    const int rows = n / rec_stride + 1;
+    hipLaunchKernelGGL(k_pbf_record_target, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, xa, xp,
+                       n, rec_stride, rec_off, tgt);
+    return hipGetLastError();
+}
+
+// one block: reduce the block maxima of the XSPH pass, record, decide
+__global__ __launch_bounds__(kBlock) void k_pbf_decide(const int* __restrict__ maxd, int nblk, float bound,
+                                                       int learned, int* __restrict__ flag,
+                                                       int* __restrict__ counters, float* __restrict__ hist,
+                                                       int cap) {
+    __shared__ int wmax[kBlock / 64];
+    int m = INT_MIN;
+    for (int b = threadIdx.x; b < nblk; b += kBlock) m = maxd[b] > m ? maxd[b] : m;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        int o = __shfl_xor(m, d, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = wmax[0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; ++w) b = wmax[w] > b ? wmax[w] : b;
+        const int bits = b >= 0 ? b : (b ^ 0x7FFFFFFF);
+        float c = __int_as_float(bits);
+        const float e = c > 0.0f ? c : (c == c ? 0.0f : c);  // max(0, C), NaN kept
+        const int fb = (learned && !(e <= bound)) ? 1 : 0;
+        const int step = counters[0];
+        hist[step % cap] = e;
+        counters[0] = step + 1;
+        counters[1] = counters[1] + fb;
+        *flag = fb;
+    }
+}
+
+hipError_t launch_pbf_decide(const int* maxd, int nblk, float bound, int learned, int* flag,
+                             int* counters, float* hist, int cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_pbf_decide, dim3(1), dim3(kBlock), 0, s, maxd, nblk, bound, learned, flag, counters,
+                       hist, cap);
     return hipGetLastError();
 }
 

@@ -13,9 +13,17 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblustrine_gpu.so")
 
-NPHASES = 8
+NPHASES = 11
 PHASES = ("predict_key", "cell_scan", "scatter", "stable_permute", "lambda", "deltap", "xsph",
-          "lambda_build")
+          "lambda_build", "features", "mlp", "decide")
+
+# learned warm start: network file layout (include/lustrine_gpu.h)
+FEATURES = 32
+MLP_PARAMS = 6467
+NET_MU, NET_INV_SIGMA, NET_PARAMS = 0, 32, 64
+NET_OUT_SCALE = NET_PARAMS + MLP_PARAMS
+NET_FLOATS = NET_OUT_SCALE + 1
+DENSITY_HISTORY = 4096
 
 
 class LusError(RuntimeError):
@@ -55,6 +63,11 @@ _f = _P(C.c_float)
 _i = _P(C.c_int)
 _l = _P(C.c_long)
 
+class LusMlpFitParams(C.Structure):
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float), ("lr_final", C.c_float),
+                ("val_fraction", C.c_float), ("seed", C.c_ulonglong)]
+
+
 _SIGS = {
     "lus_gpu_last_error": (C.c_int, [C.c_char_p, C.c_int]),
     "lus_gpu_device_count": (C.c_int, []),
@@ -75,6 +88,19 @@ _SIGS = {
     "lus_pbf_last_max_constraint": (C.c_int, [C.c_void_p, _f]),
     "lus_pbf_bench": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, _f, _f, _i]),
     "lus_pbf_time_kernels": (C.c_int, [C.c_void_p, C.c_int, _f]),
+    "lus_pbf_set_network": (C.c_int, [C.c_void_p, _f, C.c_int]),
+    "lus_pbf_set_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int]),
+    "lus_pbf_learned_stats": (C.c_int, [C.c_void_p, _P(C.c_longlong), _P(C.c_longlong), _f, C.c_int, _i]),
+    "lus_mlp_trainer_create": (C.c_int, [C.c_void_p, C.c_long, _P(C.c_void_p)]),
+    "lus_mlp_trainer_destroy": (None, [C.c_void_p]),
+    "lus_mlp_trainer_add": (C.c_int, [C.c_void_p, _f, _f, C.c_long]),
+    "lus_mlp_trainer_rows": (C.c_int, [C.c_void_p, _l]),
+    "lus_mlp_trainer_download": (C.c_int, [C.c_void_p, C.c_long, C.c_long, _f, _f]),
+    "lus_pbf_record_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_int, _l]),
+    "lus_mlp_fit": (C.c_int, [C.c_void_p, _P(LusMlpFitParams), _f, _f]),
+    "lus_mlp_trainer_network": (C.c_int, [C.c_void_p, _f]),
+    "lus_mlp_loss_grad": (C.c_int, [C.c_void_p, _f, C.c_int, _f, _f, C.c_float, _f, _f]),
+    "lus_mlp_infer": (C.c_int, [C.c_void_p, _f, C.c_int, _f, _f]),
     "lus_sand_init": (C.c_int, [C.c_void_p, _P(LusSandParams), C.c_int, _f, _f, _i, C.c_int, _f, _i]),
     "lus_sand_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, _f]),
     "lus_sand_download": (C.c_int, [C.c_void_p, _f, _f, _f, _i, _i]),
@@ -323,6 +349,45 @@ class Gpu:
         return {"lambda": float(us[0]), "deltap": float(us[1]), "xsph": float(us[2]),
                 "lambda_build": float(us[3])}
 
+    # --- learned warm start
+    def pbf_set_network(self, net):
+        net = _f32(net, (-1,))
+        _check(self._lib.lus_pbf_set_network(self._h, _ptr_f(net), net.shape[0]), "lus_pbf_set_network")
+
+    def pbf_set_mode(self, learned=False, learned_iterations=1, bound=0.01, check=False):
+        _check(self._lib.lus_pbf_set_mode(self._h, int(bool(learned)), learned_iterations, bound,
+                                          int(bool(check))), "lus_pbf_set_mode")
+
+    def pbf_learned_stats(self):
+        """(measured steps, plain reruns, per-step max density error e, oldest first)."""
+        steps, reruns = C.c_longlong(0), C.c_longlong(0)
+        errs = np.zeros(DENSITY_HISTORY, np.float32)
+        ne = C.c_int(0)
+        _check(self._lib.lus_pbf_learned_stats(self._h, C.byref(steps), C.byref(reruns), _ptr_f(errs),
+                                               DENSITY_HISTORY, C.byref(ne)), "lus_pbf_learned_stats")
+        return steps.value, reruns.value, errs[: ne.value].copy()
+
+    def mlp_loss_grad(self, params, X, Y, inv_scale=1.0):
+        params = _f32(params, (MLP_PARAMS,))
+        X = _f32(X, (-1, FEATURES))
+        Y = _f32(Y, (-1, 3))
+        loss = C.c_float(0)
+        grad = np.zeros(MLP_PARAMS, np.float32)
+        _check(self._lib.lus_mlp_loss_grad(self._h, _ptr_f(params), X.shape[0], _ptr_f(X), _ptr_f(Y),
+                                           inv_scale, C.byref(loss), _ptr_f(grad)), "lus_mlp_loss_grad")
+        return loss.value, grad
+
+    def mlp_infer(self, net, feat):
+        net = _f32(net, (NET_FLOATS,))
+        feat = _f32(feat, (-1, FEATURES))
+        out = np.zeros((feat.shape[0], 3), np.float32)
+        _check(self._lib.lus_mlp_infer(self._h, _ptr_f(net), feat.shape[0], _ptr_f(feat), _ptr_f(out)),
+               "lus_mlp_infer")
+        return out
+
+    def trainer(self, capacity):
+        return Trainer(self, capacity)
+
     # --- sand
     def sand_init(self, params: LusSandParams, pos, vel=None, attracted=None, solid_pos=None,
                   solid_index=None):
@@ -360,3 +425,70 @@ class Gpu:
         _check(self._lib.lus_sand_neighbors(self._h, _ptr_l(off), _ptr_i(idx), idx.shape[0], C.byref(total)),
                "lus_sand_neighbors")
         return off, idx[: total.value]
+
+
+class Trainer:
+    """Device-resident training set of the warm-start network (LusMlpTrainer)."""
+
+    def __init__(self, gpu: Gpu, capacity: int):
+        self._lib = gpu._lib
+        self.gpu = gpu
+        h = C.c_void_p()
+        _check(self._lib.lus_mlp_trainer_create(gpu._h, capacity, C.byref(h)), "lus_mlp_trainer_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.lus_mlp_trainer_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def rows(self) -> int:
+        r = C.c_long(0)
+        _check(self._lib.lus_mlp_trainer_rows(self._h, C.byref(r)), "lus_mlp_trainer_rows")
+        return r.value
+
+    def add(self, feat, target):
+        feat = _f32(feat, (-1, FEATURES))
+        target = _f32(target, (-1, 3))
+        assert feat.shape[0] == target.shape[0]
+        _check(self._lib.lus_mlp_trainer_add(self._h, _ptr_f(feat), _ptr_f(target), feat.shape[0]),
+               "lus_mlp_trainer_add")
+
+    def download(self, first=0, rows=None):
+        rows = self.rows - first if rows is None else rows
+        feat = np.zeros((rows, FEATURES), np.float32)
+        tgt = np.zeros((rows, 3), np.float32)
+        _check(self._lib.lus_mlp_trainer_download(self._h, first, rows, _ptr_f(feat), _ptr_f(tgt)),
+               "lus_mlp_trainer_download")
+        return feat, tgt
+
+    def record_step(self, dt, stride=1) -> int:
+        added = C.c_long(0)
+        _check(self._lib.lus_pbf_record_step(self.gpu._h, self._h, dt, stride, C.byref(added)),
+               "lus_pbf_record_step")
+        return added.value
+
+    def fit(self, epochs=20, batch=4096, lr=2e-3, lr_final=1e-4, val_fraction=0.05, seed=1):
+        fp = LusMlpFitParams(epochs, batch, lr, lr_final, val_fraction, seed)
+        tl = np.zeros(epochs, np.float32)
+        vl = np.zeros(epochs, np.float32)
+        _check(self._lib.lus_mlp_fit(self._h, C.byref(fp), _ptr_f(tl), _ptr_f(vl)), "lus_mlp_fit")
+        return tl, vl
+
+    def network(self) -> np.ndarray:
+        net = np.zeros(NET_FLOATS, np.float32)
+        _check(self._lib.lus_mlp_trainer_network(self._h, _ptr_f(net)), "lus_mlp_trainer_network")
+        return net

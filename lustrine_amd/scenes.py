@@ -177,6 +177,18 @@ def dam_break_of_size(n_target: int, seed: int, **kw) -> PbfConfig:
                      seed=seed, **kw)
 
 
+def dam_break_aspect(n_target: int, aspect: float, seed: int, **kw) -> PbfConfig:
+    """A dam break of about n_target particles whose block (and box, twice the
+    block per axis) has width / depth = aspect; height (n/4)^(1/3) as in
+    dam_break_of_size."""
+    ny = max(2, int(round((n_target / 4.0) ** (1.0 / 3.0))))
+    area = n_target / ny
+    nx = max(1, int(round(math.sqrt(area * aspect))))
+    nz = max(1, int(round(area / nx)))
+    return PbfConfig(name=f"dam_{nx}x{ny}x{nz}", lattice=(nx, ny, nz), box=(2 * nx, 2 * ny, 2 * nz),
+                     seed=seed, **kw)
+
+
 @dataclass
 class BatchConfig:
     """Independent dam-break scenes stepped together."""
@@ -205,3 +217,13 @@ def cfg4(n_scenes: int = 64, lo: int = 4096, hi: int = 200000, seed: int = 4, **
     sizes = np.exp(rng.uniform(math.log(lo), math.log(hi), n_scenes))
     scenes = [dam_break_of_size(int(round(s)), seed * 1000 + k, **kw) for k, s in enumerate(sizes)]
     return BatchConfig(name=f"pbf_batch_{n_scenes}", scenes=scenes)
+
+
+def training_batch(n_scenes: int = 64, n_per_scene: int = 8192, seed: int = 3, **kw) -> BatchConfig:
+    """The warm-start network's training scenes: dam breaks of n_per_scene
+    particles with box aspect ratio width / depth uniform in [0.5, 2] (numpy
+    PCG64 seeded with `seed`), jitter seed = seed * 1000 + scene, 200 steps."""
+    rng = np.random.default_rng(seed)
+    aspects = rng.uniform(0.5, 2.0, n_scenes)
+    scenes = [dam_break_aspect(n_per_scene, float(a), seed * 1000 + k, **kw) for k, a in enumerate(aspects)]
+    return BatchConfig(name=f"pbf_train_{n_scenes}", scenes=scenes, steps=200)

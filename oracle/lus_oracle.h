@@ -125,11 +125,12 @@ typedef struct OrcPbfParams {
 /* One substep over n fluid particles.  x, v: [3n] in/out, id: [n] in/out
  * (the carried original index).  State comes back in cell order.
  * max_c (optional): max over particles of the density constraint C_i from the
- * last lambda pass. */
+ * last lambda pass.  final_err (optional): max(0, max_i C_i) at the final
+ * positions over the step's neighbour sets (summed in the XSPH loop). */
 int orc_pbf_step_f32(const OrcPbfParams* p, int n, float* x, float* v, int* id,
-                     float dt, float* max_c);
+                     float dt, float* max_c, float* final_err);
 int orc_pbf_step_f64(const OrcPbfParams* p, int n, double* x, double* v, int* id,
-                     double dt, double* max_c);
+                     double dt, double* max_c, double* final_err);
 
 /* Density error of a state: max_i max(0, rho_i/rho0 - 1) with the grid built
  * from x itself (no predict).  Used to judge the learned warm start. */

@@ -115,10 +115,10 @@ def lib():
             [C.c_int] * 4 + [C.c_float]
         L.orc_pbf_step_f32.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_float),
                                        C.POINTER(C.c_float), C.POINTER(C.c_int), C.c_float,
-                                       C.POINTER(C.c_float)]
+                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.orc_pbf_step_f64.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_double,
-                                       C.POINTER(C.c_double)]
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_pbf_density_error_f64.restype = C.c_double
         L.orc_pbf_density_error_f64.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_double)]
     return _lib
@@ -269,18 +269,19 @@ def pbf_params(cfg, iterations=None) -> OrcPbfParams:
     return p
 
 
-def pbf_step(params: OrcPbfParams, x, v, ids, dt, fp64=False):
-    """One substep in place; returns max density constraint of the last lambda pass."""
+def pbf_step(params: OrcPbfParams, x, v, ids, dt, fp64=False, final_err=False):
+    """One substep in place; returns max density constraint of the last lambda
+    pass, and with final_err=True also max(0, max C) at the final positions."""
     n = x.shape[0]
     if fp64:
         assert x.dtype == np.float64 and v.dtype == np.float64
-        mc = C.c_double(0)
-        lib().orc_pbf_step_f64(C.byref(params), n, _dp(x), _dp(v), _ip(ids), dt, C.byref(mc))
+        mc, fe = C.c_double(0), C.c_double(0)
+        lib().orc_pbf_step_f64(C.byref(params), n, _dp(x), _dp(v), _ip(ids), dt, C.byref(mc), C.byref(fe))
     else:
         assert x.dtype == np.float32 and v.dtype == np.float32
-        mc = C.c_float(0)
-        lib().orc_pbf_step_f32(C.byref(params), n, _fp(x), _fp(v), _ip(ids), dt, C.byref(mc))
-    return mc.value
+        mc, fe = C.c_float(0), C.c_float(0)
+        lib().orc_pbf_step_f32(C.byref(params), n, _fp(x), _fp(v), _ip(ids), dt, C.byref(mc), C.byref(fe))
+    return (mc.value, fe.value) if final_err else mc.value
 
 
 def density_error(params, x):
