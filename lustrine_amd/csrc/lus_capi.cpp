@@ -174,16 +174,23 @@ struct SandState {
     bool stepped = false;
 };
 
+struct LusMlpTrainer;
+
 struct LusGpu {
     int device = 0;
     hipStream_t stream = nullptr;
     PbfState pbf;
     NetState net;
     SandState sand;
+    std::vector<LusMlpTrainer*> trainers;  // live trainers, detached on destroy
 };
 
+// A trainer belongs to one context.  Destroying the context first frees the
+// trainer's device memory and detaches it (ctx = nullptr): later calls on it
+// fail with LUS_ESTATE, lus_mlp_trainer_destroy still releases the handle.
 struct LusMlpTrainer {
     LusGpu* ctx = nullptr;
+    int device = 0;
     long cap = 0, rows = 0;
     int rec_calls = 0;
     Arena mem;
@@ -245,8 +252,8 @@ lus::PbfK pbf_constants(const LusPbfParams& p) {
 // Learned step: key, scan, scatter, permute, features (neighbour sets),
 //               network (x* + correction), L x (lambda, delta-p),
 //               XSPH (+ density maximum), decide, then -- gated on the
-//               decision -- permute, K x (lambda, delta-p), XSPH again: the
-//               plain step from the same sorted state.
+//               decision -- restore x2, K x (lambda, delta-p), XSPH again:
+//               the plain step from the same sorted state.
 int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<int>* ev_phase,
                 const Record* rec = nullptr) {
     PbfState& P = c->pbf;
@@ -320,11 +327,13 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, P.bound, 1, P.flag, P.counters, P.hist,
                                        LUS_DENSITY_HISTORY, s));
         LUS_HIP(mark(10));
-        // the plain step, run only when the decision asks for it (K >= 1)
-        LUS_HIP(lus::launch_pbf_permute(P.tmp, P.key, P.cs, P.x, P.v, P.id, P.n, k, dt, P.x2, P.xp,
-                                        P.id2, P.key2, P.flag, s));
+        // The plain step, run only when the decision asks for it (K >= 1).
+        // x*, the neighbour masks and the overflow flags in key2 are what the
+        // plain step's first lambda pass builds; only x2 (old positions, the
+        // velocity reference) was overwritten by the learned commit.
+        LUS_HIP(lus::launch_pbf_restore_old(P.tmp, P.key, P.cs, P.x, P.n, P.x2, P.flag, s));
         LUS_HIP(mark(3));
-        if (int r = plain_iterations(false, P.flag)) return r;
+        if (int r = plain_iterations(true, P.flag)) return r;
         LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp, nullptr,
                                      P.flag, s));
         LUS_HIP(mark(6));
@@ -451,6 +460,13 @@ void lus_gpu_destroy(LusGpu* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     pbf_drop_graph(c->pbf);
+    for (LusMlpTrainer* t : c->trainers) {  // detach: the handles stay valid for destroy
+        t->mem.release();
+        t->ctx = nullptr;
+        t->rows = 0;
+        t->cap = 0;
+    }
+    c->trainers.clear();
     c->pbf.mem.release();
     c->net.mem.release();
     c->sand.mem.release();
@@ -1136,6 +1152,7 @@ int lus_mlp_trainer_create(LusGpu* c, long capacity, LusMlpTrainer** out) {
     LUS_HIP(hipSetDevice(c->device));
     LusMlpTrainer* t = new LusMlpTrainer();
     t->ctx = c;
+    t->device = c->device;
     t->cap = capacity;
     hipError_t e = t->mem.alloc(&t->feat, (size_t)capacity * lus::kFeat);
     if (e == hipSuccess) e = t->mem.alloc(&t->tgt, (size_t)capacity * 3);
@@ -1143,14 +1160,19 @@ int lus_mlp_trainer_create(LusGpu* c, long capacity, LusMlpTrainer** out) {
         delete t;
         return set_error(LUS_ENOMEM, "lus_mlp_trainer_create: %s", hipGetErrorString(e));
     }
+    c->trainers.push_back(t);
     *out = t;
     return LUS_OK;
 }
 
 void lus_mlp_trainer_destroy(LusMlpTrainer* t) {
     if (!t) return;
-    (void)hipSetDevice(t->ctx->device);
-    (void)hipStreamSynchronize(t->ctx->stream);
+    (void)hipSetDevice(t->device);
+    if (t->ctx) {
+        (void)hipStreamSynchronize(t->ctx->stream);
+        auto& v = t->ctx->trainers;
+        v.erase(std::remove(v.begin(), v.end(), t), v.end());
+    }
     t->mem.release();
     delete t;
 }
@@ -1158,6 +1180,7 @@ void lus_mlp_trainer_destroy(LusMlpTrainer* t) {
 int lus_mlp_trainer_add(LusMlpTrainer* t, const float* feat, const float* target, long rows) {
     if (!t || !feat || !target || rows < 0 || t->rows + rows > t->cap)
         return set_error(LUS_EINVAL, "lus_mlp_trainer_add: bad arguments or capacity exceeded");
+    if (!t->ctx) return set_error(LUS_ESTATE, "lus_mlp_trainer_add: the trainer's context was destroyed");
     LUS_HIP(hipSetDevice(t->ctx->device));
     LUS_HIP(hipStreamSynchronize(t->ctx->stream));
     LUS_HIP(hipMemcpy(t->feat + (size_t)t->rows * lus::kFeat, feat, sizeof(float) * lus::kFeat * rows,
@@ -1176,6 +1199,7 @@ int lus_mlp_trainer_rows(LusMlpTrainer* t, long* rows) {
 int lus_mlp_trainer_download(LusMlpTrainer* t, long first, long rows, float* feat, float* target) {
     if (!t || first < 0 || rows < 0 || first + rows > t->rows)
         return set_error(LUS_EINVAL, "lus_mlp_trainer_download: bad range");
+    if (!t->ctx) return set_error(LUS_ESTATE, "lus_mlp_trainer_download: the trainer's context was destroyed");
     LUS_HIP(hipSetDevice(t->ctx->device));
     LUS_HIP(hipStreamSynchronize(t->ctx->stream));
     if (feat)
@@ -1214,6 +1238,7 @@ int lus_mlp_fit(LusMlpTrainer* t, const LusMlpFitParams* fp, float* train_loss, 
         !(fp->val_fraction <= 0.5f))
         return set_error(LUS_EINVAL, "lus_mlp_fit: bad arguments");
     using namespace lus;
+    if (!t->ctx) return set_error(LUS_ESTATE, "lus_mlp_fit: the trainer's context was destroyed");
     const long rows = t->rows;
     const long nval = (long)((double)rows * fp->val_fraction);
     const long ntr = rows - nval;

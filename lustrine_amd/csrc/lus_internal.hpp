@@ -80,6 +80,9 @@ hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_st
                               const float4* x, const float4* v, const int* id, int n, PbfK k,
                               float dt, float4* x2, float4* xp, int* id2, int* key2,
                               const int* gate, hipStream_t s);
+// x2 only (old positions in cell order), as launch_pbf_permute writes it
+hipError_t launch_pbf_restore_old(const int* tmp, const int* key, const int* cell_start,
+                                  const float4* x, int n, float4* x2, const int* gate, hipStream_t s);
 // masks: 9 * n neighbour-row masks; build = 1 on the first lambda pass of a step
 hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
                              int* maxc, unsigned long long* masks, const float4* xp, int build,

@@ -255,6 +255,28 @@ hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_st
     return hipGetLastError();
 }
 
+// x2 = the step's old positions in cell order again (the permute's x2 output
+// alone), for the plain rerun of a learned step
+__global__ __launch_bounds__(kBlock) void k_pbf_restore_old(const int* __restrict__ tmp,
+                                                            const int* __restrict__ key,
+                                                            const int* __restrict__ cell_start,
+                                                            const float4* __restrict__ x, int n,
+                                                            float4* __restrict__ x2,
+                                                            const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int p = xcd_index();
+    if (p >= n) return;
+    const int i = tmp[p];
+    x2[stable_slot(tmp, cell_start, key[i], i)] = x[i];
+}
+
+hipError_t launch_pbf_restore_old(const int* tmp, const int* key, const int* cell_start,
+                                  const float4* x, int n, float4* x2, const int* gate, hipStream_t s) {
+    hipLaunchKernelGGL(k_pbf_restore_old, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start, x, n,
+                       x2, gate);
+    return hipGetLastError();
+}
+
 // s(r) = SPf (hf - f r)^2 / r of the spiky gradient, with r = |d|
 template <bool kFast>
 __device__ __forceinline__ float spiky_s(float r2, const PbfK& k) {
@@ -496,7 +518,10 @@ __global__ __launch_bounds__(kBlock) void k_pbf_xsph(const float4* __restrict__ 
         vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
         xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
         Ci = rho * k.c_rho - 1.0f;
-        Ci = (Ci == Ci) ? Ci : __builtin_inff();  // a NaN must fail the density test
+        // a non-finite position (no pair passes the distance test) or a NaN
+        // must fail the density test
+        const bool finite = __builtin_isfinite(pi.x) && __builtin_isfinite(pi.y) && __builtin_isfinite(pi.z);
+        Ci = (finite && Ci == Ci) ? Ci : __builtin_inff();
     }
     if (kCheck) block_max_store(Ci, maxd);
 }
@@ -675,9 +700,9 @@ __global__ __launch_bounds__(kBlock) void k_pbf_decide(const int* __restrict__ m
         float c = __int_as_float(bits);
         const float e = c > 0.0f ? c : (c == c ? 0.0f : c);  // max(0, C), NaN kept
         const int fb = (learned && !(e <= bound)) ? 1 : 0;
-        const int step = counters[0];
-        hist[step % cap] = e;
-        counters[0] = step + 1;
+        const unsigned step = (unsigned)counters[0];
+        hist[step % (unsigned)cap] = e;
+        counters[0] = (int)(step + 1u);
         counters[1] = counters[1] + fb;
         *flag = fb;
     }

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -263,9 +264,12 @@ class Gpu:
         self._h = h
         self.n = 0
         self.n_sand = 0
+        self._trainers = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
+            for t in list(getattr(self, "_trainers", ())):  # a trainer lives inside its context
+                t.close()
             self._lib.lus_gpu_destroy(self._h)
             self._h = None
 
@@ -436,6 +440,7 @@ class Trainer:
         h = C.c_void_p()
         _check(self._lib.lus_mlp_trainer_create(gpu._h, capacity, C.byref(h)), "lus_mlp_trainer_create")
         self._h = h
+        gpu._trainers.add(self)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
