@@ -129,12 +129,119 @@ def bench_batch(args, G, scenes) -> int:
     return 0
 
 
+PLAIN_SWEEP = (1, 2, 3, 4, 6, 8, 12, 16)
+
+
+def measure_mode(g, G, cfg, x0, dt, warmup, steps, iterations, learned_iters=None, bound=float("inf"),
+                 exact=False, graph=True):
+    """Throughput and per-step density error e_t = max_i max(0, rho_i/rho0 - 1)
+    at the final positions of one solver configuration over the same window
+    (warmup untimed steps, then `steps` steps).  learned_iters None: plain
+    solve with `iterations`; else the learned step with that many plain
+    iterations after the network and the plain rerun (`iterations`) above
+    `bound`.  Two runs from the same initial state: a timed one (plain: no
+    density measurement) and one that records e_t."""
+    params = G.pbf_params(cfg, iterations=iterations, exact=exact)
+    learned = learned_iters is not None
+    g.pbf_init(params, x0, None)
+    g.pbf_set_mode(learned, learned_iters or 0, bound, check=False)
+    g.pbf_set_graph(graph)
+    ms, _, _ = g.pbf_bench(dt, warmup, steps, phases=False)
+    g.pbf_init(params, x0, None)
+    g.pbf_set_mode(learned, learned_iters or 0, bound, check=True)
+    g.pbf_set_graph(graph)
+    g.pbf_step(dt, warmup)
+    _, r0, _ = g.pbf_learned_stats()
+    g.pbf_step(dt, steps)
+    _, r1, errs, errm = g.pbf_learned_stats(with_mean=True)
+    _, vel, _ = g.pbf_download(order=0)
+    e = errs[-steps:].astype(np.float64)
+    em = errm[-steps:].astype(np.float64)
+    return {"iterations": iterations, "learned_iterations": learned_iters,
+            "bound": None if not learned or not np.isfinite(bound) else bound,
+            "ms_per_step": ms / steps, "particle_steps_per_s": cfg.n * steps / (ms / 1e3),
+            "err_mean": float(e.mean()), "err_p50": float(np.percentile(e, 50)), "err_max": float(e.max()),
+            "avg_compression_mean": float(em.mean()),
+            "mean_speed_end": float(np.linalg.norm(vel.astype(np.float64), axis=1).mean()),
+            "rerun_fraction": (r1 - r0) / steps if learned else None}
+
+
+def equal_error_speedup(plain, learned_row, key="err_mean"):
+    """Learned throughput over the throughput of the cheapest plain K whose
+    error is at most the learned error (plain rows sorted by K).  None when no
+    plain K reaches it."""
+    target = learned_row[key]
+    ok = [r for r in plain if r[key] <= target]
+    if not ok:
+        return None, None
+    best = max(ok, key=lambda r: r["particle_steps_per_s"])
+    return learned_row["particle_steps_per_s"] / best["particle_steps_per_s"], best["iterations"]
+
+
+def bench_learned(args, G, scenes) -> int:
+    """Learned warm start vs the plain solver on the 1M dam break: an
+    error-vs-throughput curve (plain K over PLAIN_SWEEP, learned with one plain
+    iteration), and the learned step with the 1% rerun bound."""
+    from lustrine_amd import learned as L
+    cfg = scenes.cfg2()
+    net_path = args.network or L.DEFAULT_NETWORK
+    net, meta = L.load(net_path)
+    x0 = cfg.positions()
+    dt = np.float32(cfg.dt)
+    with G.Gpu(0) as g:
+        g.pbf_set_network(net)
+        plain = [measure_mode(g, G, cfg, x0, dt, args.warmup, args.steps, k) for k in PLAIN_SWEEP]
+        lrn = [measure_mode(g, G, cfg, x0, dt, args.warmup, args.steps, cfg.iterations, learned_iters=li)
+               for li in (1, 2)]
+        # rerun bound 1% (every step of this scene exceeds it) and a guard at
+        # the plain K=4 solve's worst per-step error
+        guard = next(r for r in plain if r["iterations"] == cfg.iterations)["err_max"]
+        bounded = [measure_mode(g, G, cfg, x0, dt, args.warmup, args.steps, cfg.iterations, learned_iters=1,
+                                bound=b) for b in (0.01, guard)]
+    head = lrn[0]
+    sp_mean, k_mean = equal_error_speedup(plain, head, "err_mean")
+    sp_max, k_max = equal_error_speedup(plain, head, "err_max")
+    sp_avg, k_avg = equal_error_speedup(plain, head, "avg_compression_mean")
+    out = {
+        "metric": METRIC,
+        "value": head["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+bf16",
+        "data": "synthetic",
+        "config": {"workload": "pbf_dam_break_1m_learned", "particles": cfg.n, "learned_iterations": 1,
+                   "rerun_iterations": cfg.iterations, "network": os.path.relpath(net_path, ROOT),
+                   "network_training": {k: meta.get(k) for k in ("scenes", "particles_per_scene", "steps",
+                                                                   "teacher_iterations", "rows", "epochs")},
+                   "error": "per step: max_i max(0, rho_i/rho0 - 1) at the final positions; "
+                            "err_mean / err_max over the timed window", "parallelism": "single-gpu"},
+        "equal_error": {"by_err_mean": {"speedup": sp_mean, "plain_iterations": k_mean},
+                        "by_err_max": {"speedup": sp_max, "plain_iterations": k_max},
+                        "by_avg_compression": {"speedup": sp_avg, "plain_iterations": k_avg},
+                        "note": "speedup None: no plain K in the sweep reaches the learned error; "
+                                "the learned throughput over the largest K's is then a lower bound"},
+        "plain_sweep": plain,
+        "learned": lrn,
+        "learned_bounded": bounded,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg2", "cfg2_tail", "cfg4"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg2", "cfg2_tail", "cfg3", "cfg4"])
+    ap.add_argument("--network", default=None, help="warm-start network file (cfg3)")
     ap.add_argument("--iterations", type=int, default=None)
     ap.add_argument("--exact", action="store_true",
                     help="oracle operation order (bit-identical results) instead of the fused path")
@@ -157,6 +264,8 @@ def main(argv=None):
         return 1
     if args.workload == "cfg4":
         return bench_batch(args, G, scenes)
+    if args.workload == "cfg3":
+        return bench_learned(args, G, scenes)
     cfg = {"cfg0": scenes.cfg0, "cfg2": scenes.cfg2, "cfg2_tail": scenes.cfg2_tail}[args.workload]()
     if args.iterations is not None:
         cfg.iterations = args.iterations

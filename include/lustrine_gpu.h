@@ -207,11 +207,13 @@ int lus_pbf_set_network(LusGpu* ctx, const float* net, int nfloats);
  * 0: plain steps.  check = 1 in plain mode: the XSPH pass also measures e and
  * records it (no rerun).  Resets the statistics; drops a captured graph. */
 int lus_pbf_set_mode(LusGpu* ctx, int learned, int learned_iterations, float bound, int check);
-/* Steps measured since the last reset, reruns among them, and the last
- * min(steps, LUS_DENSITY_HISTORY, cap) values of e (oldest first) in errs;
- * *nerrs receives their count.  In learned mode e is the learned solution's. */
-int lus_pbf_learned_stats(LusGpu* ctx, long long* steps, long long* reruns, float* errs, int cap,
-                          int* nerrs);
+/* Steps measured since the last reset, reruns among them, and for the last
+ * min(steps, LUS_DENSITY_HISTORY, cap) steps (oldest first) e in errs and the
+ * mean over particles of max(0, rho_i/rho0 - 1) in errs_mean (either may be
+ * NULL); *nerrs receives their count.  In learned mode the values are the
+ * learned solution's (before a rerun). */
+int lus_pbf_learned_stats(LusGpu* ctx, long long* steps, long long* reruns, float* errs,
+                          float* errs_mean, int cap, int* nerrs);
 
 /* Training of the network on the device.  A trainer holds a dataset of
  * (raw features, target) rows; the target of a recorded particle is the plain

@@ -128,10 +128,10 @@ struct PbfState {
     hipGraphExec_t graph_exec = nullptr;
     // learned warm start and density statistics
     void* feat = nullptr;  // bf16 [n][32] network inputs
-    int* maxd = nullptr;   // per-block max of C at the final positions
+    int* maxd = nullptr;   // per-block max of C, then per-block sum of max(0, C)
     int* flag = nullptr;   // 1: rerun the plain step
     int* counters = nullptr;  // [0] measured steps, [1] reruns
-    float* hist = nullptr;    // LUS_DENSITY_HISTORY ring of e
+    float* hist = nullptr;    // rings of e (max) and of mean max(0, C)
     bool learned = false, check = false;
     int learned_iters = 1;
     float bound = 0.01f;
@@ -324,7 +324,7 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp, P.maxd,
                                      nullptr, s));
         LUS_HIP(mark(6));
-        LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, P.bound, 1, P.flag, P.counters, P.hist,
+        LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, P.n, P.bound, 1, P.flag, P.counters, P.hist,
                                        LUS_DENSITY_HISTORY, s));
         LUS_HIP(mark(10));
         // The plain step, run only when the decision asks for it (K >= 1).
@@ -357,7 +357,7 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
                                      P.check ? P.maxd : nullptr, nullptr, s));
         LUS_HIP(mark(6));
         if (P.check) {
-            LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, P.bound, 0, P.flag, P.counters, P.hist,
+            LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, P.n, P.bound, 0, P.flag, P.counters, P.hist,
                                            LUS_DENSITY_HISTORY, s));
             LUS_HIP(mark(10));
         }
@@ -792,10 +792,10 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
     LUS_HIP(m.alloc(&P.tiles, lus::scan_tile_count(P.C + 1) + 1));
     P.nblk = lus::grid_for(n);
     LUS_HIP(m.alloc(&P.maxc, (size_t)P.nmaxc * P.nblk));
-    LUS_HIP(m.alloc(&P.maxd, P.nblk));
+    LUS_HIP(m.alloc(&P.maxd, 2 * (size_t)P.nblk));
     LUS_HIP(m.alloc(&P.flag, 1));
     LUS_HIP(m.alloc(&P.counters, 2));
-    LUS_HIP(m.alloc(&P.hist, LUS_DENSITY_HISTORY));
+    LUS_HIP(m.alloc(&P.hist, 2 * LUS_DENSITY_HISTORY));
     LUS_HIP(hipMemset(P.counters, 0, 2 * sizeof(int)));
     LUS_HIP(hipMemset(P.flag, 0, sizeof(int)));
     {
@@ -1125,7 +1125,8 @@ int lus_pbf_set_mode(LusGpu* c, int learned, int learned_iterations, float bound
     return LUS_OK;
 }
 
-int lus_pbf_learned_stats(LusGpu* c, long long* steps, long long* reruns, float* errs, int cap, int* nerrs) {
+int lus_pbf_learned_stats(LusGpu* c, long long* steps, long long* reruns, float* errs, float* errs_mean,
+                          int cap, int* nerrs) {
     if (!c || !c->pbf.ready) return set_error(LUS_ESTATE, "lus_pbf_learned_stats: no PBF scene");
     PbfState& P = c->pbf;
     LUS_HIP(hipSetDevice(c->device));
@@ -1136,12 +1137,16 @@ int lus_pbf_learned_stats(LusGpu* c, long long* steps, long long* reruns, float*
     if (reruns) *reruns = cnt[1];
     int nh = std::min(cnt[0], LUS_DENSITY_HISTORY);
     nh = std::min(nh, std::max(cap, 0));
-    if (errs && nh > 0) {
-        std::vector<float> h(LUS_DENSITY_HISTORY);
+    if ((errs || errs_mean) && nh > 0) {
+        std::vector<float> h(2 * LUS_DENSITY_HISTORY);
         LUS_HIP(hipMemcpy(h.data(), P.hist, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
-        for (int q = 0; q < nh; ++q) errs[q] = h[(size_t)((cnt[0] - nh + q) % LUS_DENSITY_HISTORY)];
+        for (int q = 0; q < nh; ++q) {
+            const size_t r = (size_t)((unsigned)(cnt[0] - nh + q) % LUS_DENSITY_HISTORY);
+            if (errs) errs[q] = h[r];
+            if (errs_mean) errs_mean[q] = h[LUS_DENSITY_HISTORY + r];
+        }
     }
-    if (nerrs) *nerrs = errs ? nh : 0;
+    if (nerrs) *nerrs = (errs || errs_mean) ? nh : 0;
     return LUS_OK;
 }
 

@@ -92,8 +92,9 @@ hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, 
                              const float4* xp, const int* gate, hipStream_t s);
 hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, float dt,
                                float4* vout, hipStream_t s);
-// XSPH + commit; maxd (nullable): per-block max of the density constraint
-// C_i = rho_i / rho0 - 1 at the final positions (one int per block, ordered())
+// XSPH + commit; maxd (nullable, 2 * blocks ints): per-block max of the
+// density constraint C_i = rho_i / rho0 - 1 at the final positions
+// (ordered() ints), then per-block sums of max(0, C_i) (float bits)
 hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
                            PbfK k, float4* vout, float4* xout, unsigned long long* masks,
                            const float4* xp, int* maxd, const int* gate, hipStream_t s);
@@ -113,9 +114,10 @@ hipError_t launch_pbf_features(int* key2, const int* cs, int n, PbfK k, unsigned
 // training target of the recorded slots: tgt[row] = xa[p] - xp[p]
 hipError_t launch_pbf_record_target(const float4* xa, const float4* xp, int n, int rec_stride,
                                     int rec_off, float* tgt, hipStream_t s);
-// reduce the XSPH block maxima: e = max(0, max C); hist[*counter % cap] = e;
+// reduce the XSPH block maxima and sums (maxd: 2 * nblk): e = max(0, max C),
+// hist[step % cap] = e, hist[cap + step % cap] = mean_i max(0, C_i);
 // flag = learned && !(e <= bound); counters[0] += 1, counters[1] += flag
-hipError_t launch_pbf_decide(const int* maxd, int nblk, float bound, int learned, int* flag,
+hipError_t launch_pbf_decide(const int* maxd, int nblk, int n, float bound, int learned, int* flag,
                              int* counters, float* hist, int cap, hipStream_t s);
 
 // MLP 32 -> 64 (SiLU) -> 64 (SiLU) -> 3, parameters (fp32) in this order:

@@ -331,6 +331,23 @@ __device__ __forceinline__ void block_max_store(float c, int* __restrict__ out) 
     }
 }
 
+// block_max_store plus the block sum of max(0, c) (fixed reduction tree), as
+// float bits at out[gridDim.x + blockIdx.x]
+__device__ __forceinline__ void block_max_sum_store(float c, int* __restrict__ out) {
+    __shared__ float wsum[kBlock / 64];
+    float s = c > 0.0f ? c : 0.0f;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    block_max_store(c, out);  // contains the barrier that publishes wsum
+    if (threadIdx.x == 0) {
+        float b = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) b += wsum[w];
+        out[gridDim.x + blockIdx.x] = __float_as_int(b);
+    }
+}
+
 // lambda pass: density constraint and its gradient norm (PBF eq. 1, 8, 11)
 template <bool kBuild, bool kFast>
 __global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict__ xa,
@@ -523,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_xsph(const float4* __restrict__ 
         const bool finite = __builtin_isfinite(pi.x) && __builtin_isfinite(pi.y) && __builtin_isfinite(pi.z);
         Ci = (finite && Ci == Ci) ? Ci : __builtin_inff();
     }
-    if (kCheck) block_max_store(Ci, maxd);
+    if (kCheck) block_max_sum_store(Ci, maxd);
 }
 
 hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
@@ -677,41 +694,57 @@ hipError_t launch_pbf_record_target(const float4* xa, const float4* xp, int n, i
     return hipGetLastError();
 }
 
-// one block: reduce the block maxima of the XSPH pass, record, decide
-__global__ __launch_bounds__(kBlock) void k_pbf_decide(const int* __restrict__ maxd, int nblk, float bound,
-                                                       int learned, int* __restrict__ flag,
+// One block: reduce the XSPH pass's block maxima and block sums, record
+// e = max(0, max C) in hist[step % cap] and mean max(0, C) in
+// hist[cap + step % cap], decide the rerun.
+__global__ __launch_bounds__(kBlock) void k_pbf_decide(const int* __restrict__ maxd, int nblk, int n,
+                                                       float bound, int learned, int* __restrict__ flag,
                                                        int* __restrict__ counters, float* __restrict__ hist,
                                                        int cap) {
     __shared__ int wmax[kBlock / 64];
+    __shared__ float wsum[kBlock / 64];
     int m = INT_MIN;
-    for (int b = threadIdx.x; b < nblk; b += kBlock) m = maxd[b] > m ? maxd[b] : m;
+    float s = 0.0f;
+    for (int b = threadIdx.x; b < nblk; b += kBlock) {
+        m = maxd[b] > m ? maxd[b] : m;
+        s += __int_as_float(maxd[nblk + b]);
+    }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         int o = __shfl_xor(m, d, 64);
         m = o > m ? o : m;
+        s += __shfl_xor(s, d, 64);
     }
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    if ((threadIdx.x & 63) == 0) {
+        wmax[threadIdx.x >> 6] = m;
+        wsum[threadIdx.x >> 6] = s;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         int b = wmax[0];
+        float tot = wsum[0];
 #pragma unroll
-        for (int w = 1; w < kBlock / 64; ++w) b = wmax[w] > b ? wmax[w] : b;
+        for (int w = 1; w < kBlock / 64; ++w) {
+            b = wmax[w] > b ? wmax[w] : b;
+            tot += wsum[w];
+        }
         const int bits = b >= 0 ? b : (b ^ 0x7FFFFFFF);
         float c = __int_as_float(bits);
         const float e = c > 0.0f ? c : (c == c ? 0.0f : c);  // max(0, C), NaN kept
         const int fb = (learned && !(e <= bound)) ? 1 : 0;
         const unsigned step = (unsigned)counters[0];
         hist[step % (unsigned)cap] = e;
+        hist[cap + step % (unsigned)cap] = tot / (float)n;
         counters[0] = (int)(step + 1u);
         counters[1] = counters[1] + fb;
         *flag = fb;
     }
 }
 
-hipError_t launch_pbf_decide(const int* maxd, int nblk, float bound, int learned, int* flag,
+hipError_t launch_pbf_decide(const int* maxd, int nblk, int n, float bound, int learned, int* flag,
                              int* counters, float* hist, int cap, hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_decide, dim3(1), dim3(kBlock), 0, s, maxd, nblk, bound, learned, flag, counters,
-                       hist, cap);
+    hipLaunchKernelGGL(k_pbf_decide, dim3(1), dim3(kBlock), 0, s, maxd, nblk, n, bound, learned, flag,
+                       counters, hist, cap);
     return hipGetLastError();
 }
 

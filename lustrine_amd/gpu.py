@@ -91,7 +91,7 @@ _SIGS = {
     "lus_pbf_time_kernels": (C.c_int, [C.c_void_p, C.c_int, _f]),
     "lus_pbf_set_network": (C.c_int, [C.c_void_p, _f, C.c_int]),
     "lus_pbf_set_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int]),
-    "lus_pbf_learned_stats": (C.c_int, [C.c_void_p, _P(C.c_longlong), _P(C.c_longlong), _f, C.c_int, _i]),
+    "lus_pbf_learned_stats": (C.c_int, [C.c_void_p, _P(C.c_longlong), _P(C.c_longlong), _f, _f, C.c_int, _i]),
     "lus_mlp_trainer_create": (C.c_int, [C.c_void_p, C.c_long, _P(C.c_void_p)]),
     "lus_mlp_trainer_destroy": (None, [C.c_void_p]),
     "lus_mlp_trainer_add": (C.c_int, [C.c_void_p, _f, _f, C.c_long]),
@@ -362,14 +362,18 @@ class Gpu:
         _check(self._lib.lus_pbf_set_mode(self._h, int(bool(learned)), learned_iterations, bound,
                                           int(bool(check))), "lus_pbf_set_mode")
 
-    def pbf_learned_stats(self):
-        """(measured steps, plain reruns, per-step max density error e, oldest first)."""
+    def pbf_learned_stats(self, with_mean=False):
+        """(measured steps, plain reruns, per-step max density error e, oldest
+        first[, per-step mean compression mean_i max(0, rho_i/rho0 - 1)])."""
         steps, reruns = C.c_longlong(0), C.c_longlong(0)
         errs = np.zeros(DENSITY_HISTORY, np.float32)
+        errm = np.zeros(DENSITY_HISTORY, np.float32)
         ne = C.c_int(0)
         _check(self._lib.lus_pbf_learned_stats(self._h, C.byref(steps), C.byref(reruns), _ptr_f(errs),
-                                               DENSITY_HISTORY, C.byref(ne)), "lus_pbf_learned_stats")
-        return steps.value, reruns.value, errs[: ne.value].copy()
+                                               _ptr_f(errm), DENSITY_HISTORY, C.byref(ne)),
+               "lus_pbf_learned_stats")
+        out = (steps.value, reruns.value, errs[: ne.value].copy())
+        return out + (errm[: ne.value].copy(),) if with_mean else out
 
     def mlp_loss_grad(self, params, X, Y, inv_scale=1.0):
         params = _f32(params, (MLP_PARAMS,))

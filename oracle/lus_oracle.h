@@ -125,8 +125,9 @@ typedef struct OrcPbfParams {
 /* One substep over n fluid particles.  x, v: [3n] in/out, id: [n] in/out
  * (the carried original index).  State comes back in cell order.
  * max_c (optional): max over particles of the density constraint C_i from the
- * last lambda pass.  final_err (optional): max(0, max_i C_i) at the final
- * positions over the step's neighbour sets (summed in the XSPH loop). */
+ * last lambda pass.  final_err (optional, 2 values): max(0, max_i C_i) and
+ * mean_i max(0, C_i) at the final positions over the step's neighbour sets
+ * (summed in the XSPH loop). */
 int orc_pbf_step_f32(const OrcPbfParams* p, int n, float* x, float* v, int* id,
                      float dt, float* max_c, float* final_err);
 int orc_pbf_step_f64(const OrcPbfParams* p, int n, double* x, double* v, int* id,

@@ -271,17 +271,18 @@ def pbf_params(cfg, iterations=None) -> OrcPbfParams:
 
 def pbf_step(params: OrcPbfParams, x, v, ids, dt, fp64=False, final_err=False):
     """One substep in place; returns max density constraint of the last lambda
-    pass, and with final_err=True also max(0, max C) at the final positions."""
+    pass, and with final_err=True also (max(0, max C), mean max(0, C)) at the
+    final positions."""
     n = x.shape[0]
     if fp64:
         assert x.dtype == np.float64 and v.dtype == np.float64
-        mc, fe = C.c_double(0), C.c_double(0)
-        lib().orc_pbf_step_f64(C.byref(params), n, _dp(x), _dp(v), _ip(ids), dt, C.byref(mc), C.byref(fe))
+        mc, fe = C.c_double(0), (C.c_double * 2)()
+        lib().orc_pbf_step_f64(C.byref(params), n, _dp(x), _dp(v), _ip(ids), dt, C.byref(mc), fe)
     else:
         assert x.dtype == np.float32 and v.dtype == np.float32
-        mc, fe = C.c_float(0), C.c_float(0)
-        lib().orc_pbf_step_f32(C.byref(params), n, _fp(x), _fp(v), _ip(ids), dt, C.byref(mc), C.byref(fe))
-    return (mc.value, fe.value) if final_err else mc.value
+        mc, fe = C.c_float(0), (C.c_float * 2)()
+        lib().orc_pbf_step_f32(C.byref(params), n, _fp(x), _fp(v), _ip(ids), dt, C.byref(mc), fe)
+    return (mc.value, fe[0], fe[1]) if final_err else mc.value
 
 
 def density_error(params, x):

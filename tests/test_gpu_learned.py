@@ -212,14 +212,17 @@ def test_density_check_matches_oracle(gpu_mod, oracle):
     steps = 6
     x, v = x0.copy(), np.zeros_like(x0)
     ids = np.arange(cfg.n, dtype=np.int32)
-    want = []
+    want, want_mean = [], []
     for _ in range(steps):
-        _, e = oracle.pbf_step(oracle.pbf_params(cfg), x, v, ids, np.float32(cfg.dt), final_err=True)
+        _, e, em = oracle.pbf_step(oracle.pbf_params(cfg), x, v, ids, np.float32(cfg.dt), final_err=True)
         want.append(e)
+        want_mean.append(em)
     with gpu_mod.Gpu(0) as g:
         g.pbf_init(gpu_mod.pbf_params(cfg, exact=True), x0)
         g.pbf_set_mode(learned=False, check=True)
         g.pbf_step(np.float32(cfg.dt), steps)
-        n, reruns, errs = g.pbf_learned_stats()
+        n, reruns, errs, errm = g.pbf_learned_stats(with_mean=True)
     assert n == steps and reruns == 0
     assert np.array_equal(errs, np.array(want, np.float32))
+    assert max(want_mean) > 0
+    assert np.allclose(errm, want_mean, rtol=1e-5, atol=1e-9)
