@@ -102,20 +102,26 @@ __device__ __forceinline__ int low_bit(unsigned long long m) { return __builtin_
 // Visit the set bits of one row mask in ascending order, calling
 // f(j, cur[j], aux[j]); the next neighbour's data is loaded before f runs on
 // the current one.
+// aux[j], or zeros without aux.  Both arms are values: a conditional between
+// aux[j] and a named zero vector would select between two addresses and put
+// the zero vector in scratch memory.
+__device__ __forceinline__ float4 aux_at(const float4* __restrict__ aux, int j) {
+    return aux ? aux[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 template <typename M, class F>
 __device__ __forceinline__ void visit_row(M mm, int base, const float4* __restrict__ cur,
                                           const float4* __restrict__ aux, F& f) {
-    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (!mm) return;
     int j = base + low_bit(mm);
     mm &= mm - (M)1;
     float4 pa = cur[j];
-    float4 aa = aux ? aux[j] : zero;
+    float4 aa = aux_at(aux, j);
     while (mm) {
         const int jn = base + low_bit(mm);
         mm &= mm - (M)1;
         const float4 pb = cur[jn];
-        const float4 ab = aux ? aux[jn] : zero;
+        const float4 ab = aux_at(aux, jn);
         f(j, pa, aa);
         j = jn;
         pa = pb;
