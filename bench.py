@@ -41,6 +41,10 @@ PHASE_BYTES = {
     "lambda_build": 28,
     "deltap": 44,             # read x*, lambda, range; write x*
     "xsph": 88,               # read x, x*, v, range; write v, x
+    "features": 104,          # read x*, x, range; write 32 bf16 features
+    "mlp": 96,                # read 32 bf16 features, x*; write x*
+    "decide": None,           # one block over the block maxima
+    "tiles": None,            # tile list: two cell-start reads per chunk, one int4 per tile
 }
 STEP_BYTES_K4 = 580           # per particle-step at 4 iterations
 
@@ -321,11 +325,11 @@ def main(argv=None):
             if nl == 0:
                 continue
             avg_us = t / nl * 1e3
-            b = pb[name] * cfg.n
-            phases[name] = {"total_ms": round(t, 4), "launches": nl, "avg_launch_us": round(avg_us, 2),
-                            "alg_bytes_per_launch": int(b),
-                            "alg_GBps": round(b / (avg_us * 1e-6) / 1e9, 1)}
-        dom = max(phases, key=lambda k: phases[k]["total_ms"])
+            phases[name] = {"total_ms": round(t, 4), "launches": nl, "avg_launch_us": round(avg_us, 2)}
+            if pb.get(name) is not None:
+                b = pb[name] * cfg.n
+                phases[name].update(alg_bytes_per_launch=int(b), alg_GBps=round(b / (avg_us * 1e-6) / 1e9, 1))
+        dom = max((p for p in phases if "alg_bytes_per_launch" in phases[p]), key=lambda k: phases[k]["total_ms"])
         d = phases[dom]
         achieved = d["alg_bytes_per_launch"] / (d["avg_launch_us"] * 1e-6) / 1e9
         traffic = None

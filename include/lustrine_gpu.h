@@ -137,6 +137,13 @@ int lus_pbf_step(LusGpu* ctx, float dt, int steps);
 /* on = 1: lus_pbf_step replays a hipGraph holding two captured substeps
  * (captured on first use and again when dt changes); on = 0: plain launches. */
 int lus_pbf_set_graph(LusGpu* ctx, int on);
+/* Neighbour passes: tiled = 1 runs one block per tile of at most 128
+ * consecutive slots of one (y, x) cell column, cells cut into chunks of zt
+ * (1..32) cells along z, with the tile's 27-cell neighbourhood staged in LDS;
+ * tiled = 0 (default, faster on gfx950) runs one thread per slot with global
+ * loads.  Same results bit for bit.  No reference counterpart (launch
+ * configuration). */
+int lus_pbf_set_tiling(LusGpu* ctx, int tiled, int zt);
 /* Runs `steps` substeps with a hipEvent between steps (between step pairs in
  * graph mode, each step of a pair getting half); ms[k] = step k's time. */
 int lus_pbf_step_latency(LusGpu* ctx, float dt, int steps, float* ms);
@@ -152,12 +159,13 @@ int lus_pbf_last_max_constraint(LusGpu* ctx, float* out);
  * (array of LUS_PBF_NPHASES) the `steps` instead record an event after every
  * kernel: phase_ms accumulates per-phase device time, phase_launches the
  * launch count per phase, and *ms their sum. */
-#define LUS_PBF_NPHASES 11
+#define LUS_PBF_NPHASES 12
 /* phases: 0 predict+cell key, 1 cell scan, 2 scatter, 3 stable permute,
  *         4 lambda pass (iterations 2..K), 5 delta-p pass, 6 XSPH + commit,
  *         7 first lambda pass of the step (builds the neighbour sets),
  *         8 warm-start features (build the neighbour sets), 9 warm-start
- *         network, 10 density decision (learned mode or density check) */
+ *         network, 10 density decision (learned mode or density check),
+ *         11 tile list of the tiled neighbour passes */
 int lus_pbf_bench(LusGpu* ctx, float dt, int warmup, int steps, float* ms,
                   float* phase_ms, int* phase_launches);
 /* Average device time per launch (microseconds) of the neighbour-pass kernels

@@ -135,6 +135,15 @@ struct PbfState {
     bool learned = false, check = false;
     int learned_iters = 1;
     float bound = 0.01f;
+    // tiled neighbour passes (LDS staging of each tile's 9 stencil rows);
+    // off by default: measured slower than one thread per slot on gfx950
+    bool tiled = false;
+    int zt = 16;           // cells per chunk along z
+    int *tcount = nullptr, *toff = nullptr, *tscan = nullptr;
+    int4* tl = nullptr;    // tiles
+    int tgrid = 0;         // blocks of a tiled launch
+    int lcap = 0, lcap_x = 0;  // staged float4 per array (XSPH stages two arrays)
+    int nred = 0;          // block-reduction entries the last step wrote
 };
 
 // the network of the learned step (device copies), kept across scene inits
@@ -284,6 +293,18 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
     LUS_HIP(lus::launch_pbf_permute(P.tmp, P.key, P.cs, P.x, P.v, P.id, P.n, k, dt, P.x2, P.xp,
                                     P.id2, P.key2, nullptr, s));
     LUS_HIP(mark(3));
+    // tiles of the step (cell columns cut into chunks of P.zt cells), or flat launches
+    lus::TileArgs ta, tx;
+    if (P.tiled) {
+        LUS_HIP(lus::launch_tiles(P.cs, P.key2, k, P.zt, P.tcount, P.toff, P.tscan, P.tl, s));
+        LUS_HIP(mark(11));
+        ta.tiles = tx.tiles = P.tl;
+        ta.ntiles = tx.ntiles = P.toff + lus::tile_chunks(k, P.zt);
+        ta.grid = tx.grid = P.tgrid;
+        ta.lcap = P.lcap;
+        tx.lcap = P.lcap_x;
+    }
+    const int nred = P.tiled ? P.tgrid : P.nblk;  // block-reduction entries written
     const int K = P.p.iterations;
     const bool learned = P.learned && !rec;
     // The plain iterations: xp (predicted) -> lambda -> xl -> delta-p -> xa ->
@@ -294,11 +315,11 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
             const float4* cur = it == 0 ? P.xp : P.xa;
             const int build = (it == 0 && !built) ? 1 : 0;
             LUS_HIP(lus::launch_pbf_lambda(cur, P.key2, P.cs, P.n, k, P.xl, P.maxc + (size_t)it * P.nblk,
-                                           P.masks, P.xp, build, gate, s));
+                                           P.masks, P.xp, build, gate, ta, s));
             LUS_HIP(mark(it == 0 ? 7 : 4));
             const bool last = (it == K - 1);
             LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, k, P.xa, P.x2,
-                                           last ? P.v2 : nullptr, dt, P.masks, P.xp, gate, s));
+                                           last ? P.v2 : nullptr, dt, P.masks, P.xp, gate, ta, s));
             LUS_HIP(mark(5));
         }
         return LUS_OK;
@@ -306,7 +327,7 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
     if (learned) {
         const NetState& N = c->net;
         LUS_HIP(lus::launch_pbf_features(P.key2, P.cs, P.n, k, P.masks, P.xp, P.x2, N.mu, N.inv_sigma,
-                                         P.feat, nullptr, 1, 0, s));
+                                         P.feat, nullptr, 1, 0, ta, s));
         LUS_HIP(mark(8));
         LUS_HIP(lus::launch_mlp_forward(P.feat, P.n, N.params, N.packed, N.out_scale, nullptr, P.xp,
                                         P.xa, k, s));
@@ -314,17 +335,17 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         const int L = P.learned_iters;
         for (int it = 0; it < L; ++it) {
             LUS_HIP(lus::launch_pbf_lambda(P.xa, P.key2, P.cs, P.n, k, P.xl, P.maxc, P.masks, P.xp, 0,
-                                           nullptr, s));
+                                           nullptr, ta, s));
             LUS_HIP(mark(4));
             LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, k, P.xa, P.x2,
-                                           it == L - 1 ? P.v2 : nullptr, dt, P.masks, P.xp, nullptr, s));
+                                           it == L - 1 ? P.v2 : nullptr, dt, P.masks, P.xp, nullptr, ta, s));
             LUS_HIP(mark(5));
         }
         if (L == 0) LUS_HIP(lus::launch_pbf_velocity(P.xa, P.x2, P.n, dt, P.v2, s));
         LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp, P.maxd,
-                                     nullptr, s));
+                                     nullptr, tx, s));
         LUS_HIP(mark(6));
-        LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, P.n, P.bound, 1, P.flag, P.counters, P.hist,
+        LUS_HIP(lus::launch_pbf_decide(P.maxd, nred, P.n, P.bound, 1, P.flag, P.counters, P.hist,
                                        LUS_DENSITY_HISTORY, s));
         LUS_HIP(mark(10));
         // The plain step, run only when the decision asks for it (K >= 1).
@@ -335,12 +356,12 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         LUS_HIP(mark(3));
         if (int r = plain_iterations(true, P.flag)) return r;
         LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp, nullptr,
-                                     P.flag, s));
+                                     P.flag, tx, s));
         LUS_HIP(mark(6));
     } else {
         if (rec) {
             LUS_HIP(lus::launch_pbf_features(P.key2, P.cs, P.n, k, P.masks, P.xp, P.x2, nullptr, nullptr,
-                                             nullptr, rec->feat, rec->stride, rec->off, s));
+                                             nullptr, rec->feat, rec->stride, rec->off, ta, s));
             LUS_HIP(mark(8));
         }
         if (int r = plain_iterations(rec != nullptr, nullptr)) return r;
@@ -350,18 +371,19 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
             // neighbour sets for XSPH (the lambda values are not used)
             if (!rec)
                 LUS_HIP(lus::launch_pbf_lambda(P.xp, P.key2, P.cs, P.n, k, P.xl, P.maxc, P.masks, P.xp, 1,
-                                               nullptr, s));
+                                               nullptr, ta, s));
             LUS_HIP(lus::launch_pbf_velocity(P.xp, P.x2, P.n, dt, P.v2, s));
         }
         LUS_HIP(lus::launch_pbf_xsph(fin, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp,
-                                     P.check ? P.maxd : nullptr, nullptr, s));
+                                     P.check ? P.maxd : nullptr, nullptr, tx, s));
         LUS_HIP(mark(6));
         if (P.check) {
-            LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, P.n, P.bound, 0, P.flag, P.counters, P.hist,
+            LUS_HIP(lus::launch_pbf_decide(P.maxd, nred, P.n, P.bound, 0, P.flag, P.counters, P.hist,
                                            LUS_DENSITY_HISTORY, s));
             LUS_HIP(mark(10));
         }
     }
+    P.nred = nred;
     std::swap(P.x, P.x2);
     std::swap(P.id, P.id2);
     std::swap(P.sid, P.sid2);
@@ -803,6 +825,24 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
         LUS_HIP(m.alloc(&fb, (size_t)n * lus::kFeat));
         P.feat = fb;
     }
+    // tile buffers sized for the finest chunking (one cell per chunk), so the
+    // chunk length can change without reallocation
+    {
+        const int nch = lus::tile_chunks(P.k, 1);
+        LUS_HIP(m.alloc(&P.tcount, (size_t)nch + 1));
+        LUS_HIP(m.alloc(&P.toff, (size_t)nch + 1));
+        LUS_HIP(m.alloc(&P.tscan, (size_t)lus::scan_tile_count(nch + 1) + 1));
+        LUS_HIP(m.alloc(&P.tl, (size_t)n / lus::kTileT + 1 + nch));
+        LUS_HIP(hipMemset(P.tcount, 0, sizeof(int) * ((size_t)nch + 1)));
+        int maxsh = 0;
+        LUS_HIP(hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+        const int avail = maxsh - 2048;  // static shared memory of the tiled kernels
+        P.lcap = std::min(2048, avail / 16);
+        P.lcap_x = std::min(1536, avail / 32);
+        P.tgrid = std::min(2048, P.nblk);
+        if (P.lcap < 256 || P.lcap_x < 256) P.tiled = false;  // too little LDS to stage a tile
+    }
+    P.nred = P.nblk;
     std::vector<float4> hx = to_f4(pos, n), hv = to_f4(vel, n);
     std::vector<int> hid((size_t)n);
     for (int i = 0; i < n; ++i) hid[i] = i;
@@ -869,6 +909,21 @@ int lus_pbf_set_graph(LusGpu* c, int on) {
     return LUS_OK;
 }
 
+int lus_pbf_set_tiling(LusGpu* c, int tiled, int zt) {
+    if (!c) return set_error(LUS_EINVAL, "lus_pbf_set_tiling: NULL context");
+    if (tiled && (zt < 1 || zt > lus::kTileZMax))
+        return set_error(LUS_EINVAL, "lus_pbf_set_tiling: zt must be in [1, %d]", lus::kTileZMax);
+    PbfState& P = c->pbf;
+    if (tiled && P.ready && (P.lcap < 256 || P.lcap_x < 256))
+        return set_error(LUS_EINVAL, "lus_pbf_set_tiling: not enough LDS per block on this device");
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    pbf_drop_graph(P);
+    P.tiled = tiled != 0;
+    if (tiled) P.zt = zt;
+    return LUS_OK;
+}
+
 int lus_pbf_download(LusGpu* c, int order, float* pos, float* vel, int* ids) {
     if (!c || !c->pbf.ready) return set_error(LUS_ESTATE, "lus_pbf_download: no PBF scene");
     PbfState& P = c->pbf;
@@ -903,8 +958,8 @@ int lus_pbf_last_max_constraint(LusGpu* c, float* out) {
     PbfState& P = c->pbf;
     LUS_HIP(hipStreamSynchronize(c->stream));
     int last = std::max(P.p.iterations - 1, 0);
-    std::vector<int> blk((size_t)P.nblk);
-    LUS_HIP(hipMemcpy(blk.data(), P.maxc + (size_t)last * P.nblk, sizeof(int) * P.nblk,
+    std::vector<int> blk((size_t)P.nred);  // entries the last step's launches wrote
+    LUS_HIP(hipMemcpy(blk.data(), P.maxc + (size_t)last * P.nblk, sizeof(int) * P.nred,
                       hipMemcpyDeviceToHost));
     int m = *std::max_element(blk.begin(), blk.end());
     int b = m >= 0 ? m : (m ^ 0x7FFFFFFF);
@@ -969,6 +1024,15 @@ int lus_pbf_time_kernels(LusGpu* c, int reps, float* us) {
     hipEvent_t e0, e1;
     LUS_HIP(hipEventCreate(&e0));
     LUS_HIP(hipEventCreate(&e1));
+    // the last step's tiles still describe the current cell order
+    lus::TileArgs ta, tx;
+    if (P.tiled) {
+        ta.tiles = tx.tiles = P.tl;
+        ta.ntiles = tx.ntiles = P.toff + lus::tile_chunks(P.k, P.zt);
+        ta.grid = tx.grid = P.tgrid;
+        ta.lcap = P.lcap;
+        tx.lcap = P.lcap_x;
+    }
     for (int kind = 0; kind < 4; ++kind) {
         LUS_HIP(hipStreamSynchronize(s));
         LUS_HIP(hipEventRecord(e0, s));
@@ -976,16 +1040,16 @@ int lus_pbf_time_kernels(LusGpu* c, int reps, float* us) {
             // xl, xa and x2 are per-step scratch between steps; x, v, id hold the state
             if (kind == 0)
                 LUS_HIP(lus::launch_pbf_lambda(P.xa, P.key2, P.cs, P.n, P.k, P.xl, P.maxc, P.masks, P.xp,
-                                               0, nullptr, s));
+                                               0, nullptr, ta, s));
             else if (kind == 1)
                 LUS_HIP(lus::launch_pbf_deltap(P.xl, P.key2, P.cs, P.n, P.k, P.xa, nullptr, nullptr,
-                                               1.0f, P.masks, P.xp, nullptr, s));
+                                               1.0f, P.masks, P.xp, nullptr, ta, s));
             else if (kind == 2)
                 LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, P.k, P.xl, P.x2, P.masks,
-                                             P.xp, nullptr, nullptr, s));
+                                             P.xp, nullptr, nullptr, tx, s));
             else
                 LUS_HIP(lus::launch_pbf_lambda(P.xp, P.key2, P.cs, P.n, P.k, P.xl, P.maxc, P.masks, P.xp,
-                                               1, nullptr, s));
+                                               1, nullptr, ta, s));
         }
         LUS_HIP(hipEventRecord(e1, s));
         LUS_HIP(hipEventSynchronize(e1));

@@ -72,8 +72,27 @@ hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, cons
                             LusGrid g, float h2, const int* inv_perm, const long* off, int* cnt,
                             int* idx, int fill, hipStream_t s);
 
+// Tiled neighbour passes (lus_pbf.hip): tiles of at most kTileT slots of one
+// (y, x) cell column, built per step from cells in chunks of zt <= kTileZMax
+// along z; each block stages its tile's 9 stencil rows in LDS.
+constexpr int kTileT = 128;
+constexpr int kTileZMax = 32;
+struct TileArgs {
+    const int4* tiles = nullptr;  // nullptr: one thread per slot, global loads
+    const int* ntiles = nullptr;  // device tile count
+    int grid = 0;                 // blocks of kTileT threads, a multiple of 8
+    int lcap = 0;                 // staged float4 per staged array
+};
+int tile_chunks(const PbfK& k, int zt);
+// tcount: tile_chunks + 1 ints (last one 0); toff: the same; scan_tiles:
+// scan_tile_count(tile_chunks + 1) + 1; tiles: n / kTileT + tile_chunks int4.
+// *ntiles = toff[tile_chunks].
+hipError_t launch_tiles(const int* cs, const int* key2, PbfK k, int zt, int* tcount, int* toff, int* scan_tiles,
+                        int4* tiles, hipStream_t s);
+
 // PBF substep.  `gate` (nullable): a device int; the kernel does nothing when
-// it holds 0 (the conditional plain rerun of the learned step).
+// it holds 0 (the conditional plain rerun of the learned step).  `ta`: tiled
+// launch (ta.tiles non-null) or one thread per slot.
 hipError_t launch_pbf_key(const float4* x, const float4* v, int n, PbfK k, float dt, int* key,
                           int* slot, int* counts, hipStream_t s);
 hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_start,
@@ -86,10 +105,10 @@ hipError_t launch_pbf_restore_old(const int* tmp, const int* key, const int* cel
 // masks: 9 * n neighbour-row masks; build = 1 on the first lambda pass of a step
 hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
                              int* maxc, unsigned long long* masks, const float4* xp, int build,
-                             const int* gate, hipStream_t s);
+                             const int* gate, const TileArgs& ta, hipStream_t s);
 hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
                              const float4* xold, float4* vout, float dt, unsigned long long* masks,
-                             const float4* xp, const int* gate, hipStream_t s);
+                             const float4* xp, const int* gate, const TileArgs& ta, hipStream_t s);
 hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, float dt,
                                float4* vout, hipStream_t s);
 // XSPH + commit; maxd (nullable, 2 * blocks ints): per-block max of the
@@ -97,7 +116,7 @@ hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, floa
 // (ordered() ints), then per-block sums of max(0, C_i) (float bits)
 hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
                            PbfK k, float4* vout, float4* xout, unsigned long long* masks,
-                           const float4* xp, int* maxd, const int* gate, hipStream_t s);
+                           const float4* xp, int* maxd, const int* gate, const TileArgs& ta, hipStream_t s);
 
 // ---- learned warm start ----
 // Per-particle features of the predicted positions (build pass: also writes
@@ -110,7 +129,7 @@ constexpr int kFeat = 32;
 hipError_t launch_pbf_features(int* key2, const int* cs, int n, PbfK k, unsigned long long* masks,
                                const float4* xp, const float4* xold, const float* mu,
                                const float* inv_sigma, void* fbf, float* frec, int rec_stride,
-                               int rec_off, hipStream_t s);
+                               int rec_off, const TileArgs& ta, hipStream_t s);
 // training target of the recorded slots: tgt[row] = xa[p] - xp[p]
 hipError_t launch_pbf_record_target(const float4* xa, const float4* xp, int n, int rec_stride,
                                     int rec_off, float* tgt, hipStream_t s);

@@ -99,9 +99,6 @@ __device__ __forceinline__ bool within_h(const float4 a, const float4 b, const P
 __device__ __forceinline__ int low_bit(unsigned int m) { return __builtin_ctz(m); }
 __device__ __forceinline__ int low_bit(unsigned long long m) { return __builtin_ctzll(m); }
 
-// Visit the set bits of one row mask in ascending order, calling
-// f(j, cur[j], aux[j]); the next neighbour's data is loaded before f runs on
-// the current one.
 // aux[j], or zeros without aux.  Both arms are values: a conditional between
 // aux[j] and a named zero vector would select between two addresses and put
 // the zero vector in scratch memory.
@@ -109,19 +106,23 @@ __device__ __forceinline__ float4 aux_at(const float4* __restrict__ aux, int j) 
     return aux ? aux[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// Visit the set bits of one row mask in ascending order, calling
+// f(j, cur[j + delta], aux[j + delta]) for slot j = base + bit; the next
+// neighbour's data is loaded before f runs on the current one.  delta maps a
+// slot to its copy in a staged (LDS) array, 0 for the global arrays.
 template <typename M, class F>
-__device__ __forceinline__ void visit_row(M mm, int base, const float4* __restrict__ cur,
+__device__ __forceinline__ void visit_row(M mm, int base, int delta, const float4* __restrict__ cur,
                                           const float4* __restrict__ aux, F& f) {
     if (!mm) return;
     int j = base + low_bit(mm);
     mm &= mm - (M)1;
-    float4 pa = cur[j];
-    float4 aa = aux_at(aux, j);
+    float4 pa = cur[j + delta];
+    float4 aa = aux_at(aux, j + delta);
     while (mm) {
         const int jn = base + low_bit(mm);
         mm &= mm - (M)1;
-        const float4 pb = cur[jn];
-        const float4 ab = aux_at(aux, jn);
+        const float4 pb = cur[jn + delta];
+        const float4 ab = aux_at(aux, jn + delta);
         f(j, pa, aa);
         j = jn;
         pa = pb;
@@ -201,11 +202,222 @@ __device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict_
             mm = masks[(size_t)r * n + i];
         }
         if (__any((int)((mm >> 32) != 0ull)))
-            visit_row<unsigned long long>(mm, j0, cur, aux, f);
+            visit_row<unsigned long long>(mm, j0, 0, cur, aux, f);
         else
-            visit_row<unsigned int>((unsigned int)mm, j0, cur, aux, f);
+            visit_row<unsigned int>((unsigned int)mm, j0, 0, cur, aux, f);
     }
 }
+
+// ---------------------------------------------------------------------------
+// Tiled neighbour passes.  A tile is at most kTileT consecutive slots of one
+// (y, x) cell column, cells z in [za, zb].  Every stencil row of its particles
+// lies in the 9 rows (y +- 1, x +- 1) over cells [za - 1, zb + 1]: the block
+// stages those 9 slot ranges of the pass's input (and their cell starts) in
+// LDS once, and the particles read their neighbours from there.  A tile whose
+// staged ranges exceed the LDS capacity reads from global memory instead.
+// The visiting order and the arithmetic are those of for_each_neighbor, so
+// both paths give the same bits.
+// ---------------------------------------------------------------------------
+struct TileShared {
+    int cs[9][kTileZMax + 3];  // cell starts of row r, cells zlo .. zhi + 1
+    int rb[9];                 // first staged slot of row r
+    int lo[10];                // LDS offset of row r's copy; lo[9] = staged count
+    int y, x, zlo, zhi, i0, i1, staged;
+};
+
+// XCD-aware first tile of this block; tiles advance by gridDim.x (a multiple of 8)
+__device__ __forceinline__ int tile_first() {
+    const int G = gridDim.x, b = blockIdx.x;
+    return (b % kXcd) * (G / kXcd) + b / kXcd;
+}
+
+// Stage tile `tl` (all threads of the block call it; ends with a barrier).
+__device__ __forceinline__ void tile_stage(TileShared& S, int4 tl, const int* __restrict__ cs,
+                                           const PbfK& k, const float4* __restrict__ cur,
+                                           const float4* __restrict__ aux, float4* __restrict__ lds,
+                                           float4* __restrict__ lds_aux, int lcap) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        const int y = tl.z / k.gXZ;
+        const int x = (tl.z - y * k.gXZ) / k.gZ;
+        const int za = tl.w & 0xFFFF, zb = tl.w >> 16;
+        S.i0 = tl.x;
+        S.i1 = tl.y;
+        S.y = y;
+        S.x = x;
+        S.zlo = za > 0 ? za - 1 : 0;
+        S.zhi = zb + 1 < k.gZ ? zb + 1 : k.gZ - 1;
+    }
+    __syncthreads();
+    const int w = S.zhi - S.zlo + 2;
+    for (int t = tid; t < 9 * w; t += blockDim.x) {
+        const int r = t / w, q = t - r * w;
+        const int yy = S.y + r / 3 - 1, xx = S.x + r % 3 - 1;
+        const bool valid = yy >= 0 && yy < k.gY && xx >= 0 && xx < k.gX;
+        S.cs[r][q] = valid ? cs[yy * k.gXZ + xx * k.gZ + S.zlo + q] : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int r = 0; r < 9; ++r) {
+            S.rb[r] = S.cs[r][0];
+            S.lo[r] = acc;
+            acc += S.cs[r][w - 1] - S.cs[r][0];
+        }
+        S.lo[9] = acc;
+        S.staged = acc <= lcap ? 1 : 0;
+    }
+    __syncthreads();
+    if (S.staged) {
+        const int tot = S.lo[9];
+        for (int t = tid; t < tot; t += blockDim.x) {
+            int r = 0;
+            while (r < 8 && t >= S.lo[r + 1]) ++r;
+            const int j = S.rb[r] + (t - S.lo[r]);
+            lds[t] = cur[j];
+            if (aux) lds_aux[t] = aux[j];
+        }
+    }
+    __syncthreads();
+}
+
+// for_each_neighbor for particle i of the staged tile S.  kStaged: read the
+// neighbours from the LDS copies (src = lds, delta per row), otherwise from
+// the global arrays (src = cur, delta 0); separate instantiations keep the
+// LDS reads in the LDS address space.
+template <bool kBuild, bool kStaged, class F>
+__device__ __forceinline__ void for_each_neighbor_tiled_impl(const TileShared& S, int i, int n,
+                                                             int* __restrict__ key2,
+                                                             unsigned long long* __restrict__ masks,
+                                                             const float4* __restrict__ xp,
+                                                             const float4* __restrict__ src,
+                                                             const float4* __restrict__ src_aux, const PbfK& k,
+                                                             F& f) {
+    const int kw = key2[i];
+    const int key = kw & ~kOvf;
+    const int cz = key - (key / k.gZ) * k.gZ;
+    const int q0 = (cz > 0 ? cz - 1 : 0) - S.zlo;
+    const int q1 = (cz + 1 < k.gZ ? cz + 1 : k.gZ - 1) + 1 - S.zlo;
+    bool ovf;
+    if (kBuild) {
+        int maxlen = 0;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const int l = S.cs[r][q1] - S.cs[r][q0];  // 0 for rows outside the grid
+            maxlen = l > maxlen ? l : maxlen;
+        }
+        ovf = maxlen > 64;
+        if (ovf) key2[i] = kw | kOvf;
+    } else {
+        ovf = (kw & kOvf) != 0;
+    }
+    const float4 pi = (kBuild || ovf) ? xp[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        const int y = S.y + r / 3 - 1;
+        const int x = S.x + r % 3 - 1;
+        if (y < 0 || y >= k.gY || x < 0 || x >= k.gX) continue;
+        const int j0 = S.cs[r][q0];
+        const int j1 = S.cs[r][q1];
+        const int delta = kStaged ? S.lo[r] - S.rb[r] : 0;
+        if (ovf) {
+            for (int j = j0; j < j1; ++j)
+                if (within_h(pi, xp[j], k)) f(j, src[j + delta], aux_at(src_aux, j + delta));
+            continue;
+        }
+        unsigned long long mm;
+        if (kBuild) {
+            // candidates from the staged copy of x* (build passes stage x*)
+            mm = 0ull;
+            const float4* __restrict__ base = src + j0 + delta;
+            const int L = j1 - j0;
+            int t = 0;
+            for (; t + 4 <= L; t += 4) {
+                const float4 c0 = base[t], c1 = base[t + 1], c2 = base[t + 2], c3 = base[t + 3];
+                mm |= within_h(pi, c0, k) ? (1ull << t) : 0ull;
+                mm |= within_h(pi, c1, k) ? (2ull << t) : 0ull;
+                mm |= within_h(pi, c2, k) ? (4ull << t) : 0ull;
+                mm |= within_h(pi, c3, k) ? (8ull << t) : 0ull;
+            }
+            for (; t < L; ++t) mm |= within_h(pi, base[t], k) ? (1ull << t) : 0ull;
+            masks[(size_t)r * n + i] = mm;
+        } else {
+            mm = masks[(size_t)r * n + i];
+        }
+        if (__any((int)((mm >> 32) != 0ull)))
+            visit_row<unsigned long long>(mm, j0, delta, src, src_aux, f);
+        else
+            visit_row<unsigned int>((unsigned int)mm, j0, delta, src, src_aux, f);
+    }
+}
+
+template <bool kBuild, class F>
+__device__ __forceinline__ void for_each_neighbor_tiled(const TileShared& S, int i, int n, int* __restrict__ key2,
+                                                        unsigned long long* __restrict__ masks,
+                                                        const float4* __restrict__ xp,
+                                                        const float4* __restrict__ cur,
+                                                        const float4* __restrict__ aux,
+                                                        const float4* __restrict__ lds,
+                                                        const float4* __restrict__ lds_aux, const PbfK& k,
+                                                        F&& f) {
+    if (S.staged)
+        for_each_neighbor_tiled_impl<kBuild, true>(S, i, n, key2, masks, xp, lds, lds_aux, k, f);
+    else
+        for_each_neighbor_tiled_impl<kBuild, false>(S, i, n, key2, masks, xp, cur, aux, k, f);
+}
+
+// Tiles of the step: chunk q = column * nzc + c covers cells
+// [c * zt, min((c + 1) * zt, gZ)) of column (y, x); its particles are cut into
+// ceil(count / kTileT) tiles of consecutive slots.
+__global__ __launch_bounds__(kBlock) void k_tile_count(const int* __restrict__ cs, PbfK k, int zt, int nzc,
+                                                       int nchunks, int* __restrict__ tcount) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nchunks) return;
+    const int col = q / nzc, c = q - col * nzc;
+    const int y = col / k.gX, x = col - y * k.gX;
+    const int row = y * k.gXZ + x * k.gZ;
+    const int z0 = c * zt;
+    const int z1 = z0 + zt < k.gZ ? z0 + zt : k.gZ;
+    const int cnt = cs[row + z1] - cs[row + z0];
+    tcount[q] = (cnt + kTileT - 1) / kTileT;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tile_write(const int* __restrict__ cs, const int* __restrict__ key2,
+                                                       PbfK k, int zt, int nzc, int nchunks,
+                                                       const int* __restrict__ toff, int4* __restrict__ tiles) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nchunks) return;
+    const int col = q / nzc, c = q - col * nzc;
+    const int y = col / k.gX, x = col - y * k.gX;
+    const int row = y * k.gXZ + x * k.gZ;
+    const int z0 = c * zt;
+    const int z1 = z0 + zt < k.gZ ? z0 + zt : k.gZ;
+    const int s = cs[row + z0], e = cs[row + z1];
+    int out = toff[q];
+    for (int sb = s; sb < e; sb += kTileT, ++out) {
+        const int se = sb + kTileT < e ? sb + kTileT : e;
+        const int ka = key2[sb] & ~kOvf, kb = key2[se - 1] & ~kOvf;
+        const int za = ka - (ka / k.gZ) * k.gZ, zb = kb - (kb / k.gZ) * k.gZ;
+        tiles[out] = make_int4(sb, se, row, za | (zb << 16));
+    }
+}
+
+hipError_t launch_tiles(const int* cs, const int* key2, PbfK k, int zt, int* tcount, int* toff, int* scan_tiles,
+                        int4* tiles, hipStream_t s) {
+    if (zt < 1 || zt > kTileZMax) return hipErrorInvalidValue;
+    const int nzc = (k.gZ + zt - 1) / zt;
+    const int nchunks = k.gX * k.gY * nzc;
+    const dim3 g((nchunks + kBlock - 1) / kBlock), b(kBlock);
+    hipLaunchKernelGGL(k_tile_count, g, b, 0, s, cs, k, zt, nzc, nchunks, tcount);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = launch_exclusive_scan(tcount, toff, nchunks + 1, scan_tiles, 1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tile_write, g, b, 0, s, cs, key2, k, zt, nzc, nchunks, toff, tiles);
+    return hipGetLastError();
+}
+
+int tile_chunks(const PbfK& k, int zt) { return k.gX * k.gY * ((k.gZ + zt - 1) / zt); }
 
 __global__ __launch_bounds__(kBlock) void k_pbf_key(const float4* __restrict__ x,
                                                      const float4* __restrict__ v, int n, PbfK k,
@@ -331,30 +543,61 @@ __device__ __forceinline__ void block_max_store(float c, int* __restrict__ out) 
     __syncthreads();
     if (threadIdx.x == 0) {
         int b = wmax[0];
-#pragma unroll
-        for (int w = 1; w < kBlock / 64; ++w) b = wmax[w] > b ? wmax[w] : b;
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = wmax[w] > b ? wmax[w] : b;
         out[blockIdx.x] = b;
     }
 }
 
-// block_max_store plus the block sum of max(0, c) (fixed reduction tree), as
-// float bits at out[gridDim.x + blockIdx.x]
-__device__ __forceinline__ void block_max_sum_store(float c, int* __restrict__ out) {
+// block_max_store of cmax plus the block sum of the threads' csum (fixed
+// reduction tree), as float bits at out[gridDim.x + blockIdx.x]
+__device__ __forceinline__ void block_max_sum_store(float cmax, float csum, int* __restrict__ out) {
     __shared__ float wsum[kBlock / 64];
-    float s = c > 0.0f ? c : 0.0f;
+    float s = csum;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
-    block_max_store(c, out);  // contains the barrier that publishes wsum
+    block_max_store(cmax, out);  // contains the barrier that publishes wsum
     if (threadIdx.x == 0) {
         float b = 0.0f;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) b += wsum[w];
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b += wsum[w];
         out[gridDim.x + blockIdx.x] = __float_as_int(b);
     }
 }
 
+// Each neighbour pass is a per-particle body run by two kernels: one thread
+// per slot (flat) or one tile per block with LDS staging (tiled, suffix _t).
+// `visit(f)` calls f(j, cur_j, aux_j) over N(i) in ascending slot order.
+
 // lambda pass: density constraint and its gradient norm (PBF eq. 1, 8, 11)
+template <bool kFast, class Visit>
+__device__ __forceinline__ float lambda_body(int i, const float4* __restrict__ xa, float4* __restrict__ xl,
+                                             const PbfK& k, Visit&& visit) {
+    const float4 pi = xa[i];
+    float rho = 0.0f, a = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    visit([&](int, const float4 pj, const float4) {
+        const float dx = pi.x - pj.x;
+        const float dy = pi.y - pj.y;
+        const float dz = pi.z - pj.z;
+        const float r2 = dist2<kFast>(dx, dy, dz);
+        if (r2 <= k.h2) {
+            const float t = poly_t<kFast>(r2, k);
+            rho = acc<kFast>(rho, t * t, t);
+            if (r2 > 0.0f) {
+                const float s = spiky_s<kFast>(r2, k);
+                a = acc<kFast>(a, s * s, r2);
+                gx = acc<kFast>(gx, s, dx);
+                gy = acc<kFast>(gy, s, dy);
+                gz = acc<kFast>(gz, s, dz);
+            }
+        }
+    });
+    const float Ci = rho * k.c_rho - 1.0f;
+    const float gg = gx * gx + gy * gy + gz * gz;
+    const float den = k.mr0sq * (a + gg) + k.eps;
+    xl[i] = make_float4(pi.x, pi.y, pi.z, -Ci / den);
+    return Ci;
+}
+
 template <bool kBuild, bool kFast>
 __global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict__ xa,
                                                         int* __restrict__ key2,
@@ -367,38 +610,64 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict_
     if (gate && *gate == 0) return;  // uniform: before any barrier
     const int i = xcd_index();
     float Ci = -1e30f;
-    if (i < n) {
-        const float4 pi = xa[i];
-        float rho = 0.0f, a = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-        for_each_neighbor<kBuild>(i, n, key2, cs, masks, xp, xa, nullptr, k,
-                                  [&](int, const float4 pj, const float4) {
-            const float dx = pi.x - pj.x;
-            const float dy = pi.y - pj.y;
-            const float dz = pi.z - pj.z;
-            const float r2 = dist2<kFast>(dx, dy, dz);
-            if (r2 <= k.h2) {
-                const float t = poly_t<kFast>(r2, k);
-                rho = acc<kFast>(rho, t * t, t);
-                if (r2 > 0.0f) {
-                    const float s = spiky_s<kFast>(r2, k);
-                    a = acc<kFast>(a, s * s, r2);
-                    gx = acc<kFast>(gx, s, dx);
-                    gy = acc<kFast>(gy, s, dy);
-                    gz = acc<kFast>(gz, s, dz);
-                }
-            }
+    if (i < n)
+        Ci = lambda_body<kFast>(i, xa, xl, k, [&](auto&& f) {
+            for_each_neighbor<kBuild>(i, n, key2, cs, masks, xp, xa, nullptr, k, f);
         });
-        Ci = rho * k.c_rho - 1.0f;
-        const float gg = gx * gx + gy * gy + gz * gz;
-        const float den = k.mr0sq * (a + gg) + k.eps;
-        xl[i] = make_float4(pi.x, pi.y, pi.z, -Ci / den);
-    }
     block_max_store(Ci, maxc);
+}
+
+template <bool kBuild, bool kFast>
+__global__ __launch_bounds__(kTileT) void k_pbf_lambda_t(const float4* __restrict__ xa,
+                                                          int* __restrict__ key2,
+                                                          const int* __restrict__ cs, int n, PbfK k,
+                                                          float4* __restrict__ xl,
+                                                          int* __restrict__ maxc,
+                                                          unsigned long long* __restrict__ masks,
+                                                          const float4* __restrict__ xp,
+                                                          const int* __restrict__ gate,
+                                                          const int4* __restrict__ tiles,
+                                                          const int* __restrict__ ntiles, int lcap) {
+    if (gate && *gate == 0) return;
+    extern __shared__ float4 lds[];
+    __shared__ TileShared S;
+    float cmax = -1e30f;
+    const int nt = *ntiles;
+    for (int t = tile_first(); t < nt; t += gridDim.x) {
+        tile_stage(S, tiles[t], cs, k, xa, nullptr, lds, nullptr, lcap);
+        const int i = S.i0 + (int)threadIdx.x;
+        if (i < S.i1) {
+            const float c = lambda_body<kFast>(i, xa, xl, k, [&](auto&& f) {
+                for_each_neighbor_tiled<kBuild>(S, i, n, key2, masks, xp, xa, nullptr, lds, nullptr, k, f);
+            });
+            cmax = c > cmax ? c : cmax;
+        }
+        __syncthreads();  // the next tile overwrites the staged data
+    }
+    block_max_store(cmax, maxc);
 }
 
 hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
                              int* maxc, unsigned long long* masks, const float4* xp, int build,
-                             const int* gate, hipStream_t s) {
+                             const int* gate, const TileArgs& ta, hipStream_t s) {
+    // a building pass tests candidates against the staged input: it must be x*
+    if (ta.tiles && (!build || xa == xp)) {
+        const dim3 g(ta.grid), b(kTileT);
+        const size_t sh = (size_t)ta.lcap * sizeof(float4);
+        if (build && k.exact)
+            hipLaunchKernelGGL((k_pbf_lambda_t<true, false>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
+                               gate, ta.tiles, ta.ntiles, ta.lcap);
+        else if (build)
+            hipLaunchKernelGGL((k_pbf_lambda_t<true, true>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
+                               gate, ta.tiles, ta.ntiles, ta.lcap);
+        else if (k.exact)
+            hipLaunchKernelGGL((k_pbf_lambda_t<false, false>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
+                               gate, ta.tiles, ta.ntiles, ta.lcap);
+        else
+            hipLaunchKernelGGL((k_pbf_lambda_t<false, true>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
+                               gate, ta.tiles, ta.ntiles, ta.lcap);
+        return hipGetLastError();
+    }
     const dim3 g(grid_for(n)), b(kBlock);
     if (build && k.exact)
         hipLaunchKernelGGL((k_pbf_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
@@ -413,24 +682,14 @@ hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, 
 
 // delta-p pass with s_corr (PBF eq. 12-14), Jacobi update, box clamp.
 // kNS: s_corr exponent fixed at compile time (4), or 0 for k.n_scorr.
-template <bool kFast, int kNS>
-__global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict__ xl,
-                                                        int* __restrict__ key2,
-                                                        const int* __restrict__ cs, int n, PbfK k,
-                                                        float4* __restrict__ xa,
-                                                        const float4* __restrict__ xold,
-                                                        float4* __restrict__ vout, float dt,
-                                                        unsigned long long* __restrict__ masks,
-                                                        const float4* __restrict__ xp,
-                                                        const int* __restrict__ gate) {
-    if (gate && *gate == 0) return;
-    const int i = xcd_index();
-    if (i >= n) return;
+template <bool kFast, int kNS, class Visit>
+__device__ __forceinline__ void deltap_body(int i, const float4* __restrict__ xl, const PbfK& k,
+                                            float4* __restrict__ xa, const float4* __restrict__ xold,
+                                            float4* __restrict__ vout, float dt, Visit&& visit) {
     const float4 pi = xl[i];
     const float li = pi.w;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k,
-                             [&](int, const float4 pj, const float4) {
+    visit([&](int, const float4 pj, const float4) {
         const float dx = pi.x - pj.x;
         const float dy = pi.y - pj.y;
         const float dz = pi.z - pj.z;
@@ -469,11 +728,73 @@ __global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict_
     }
 }
 
+template <bool kFast, int kNS>
+__global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict__ xl,
+                                                        int* __restrict__ key2,
+                                                        const int* __restrict__ cs, int n, PbfK k,
+                                                        float4* __restrict__ xa,
+                                                        const float4* __restrict__ xold,
+                                                        float4* __restrict__ vout, float dt,
+                                                        unsigned long long* __restrict__ masks,
+                                                        const float4* __restrict__ xp,
+                                                        const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int i = xcd_index();
+    if (i >= n) return;
+    deltap_body<kFast, kNS>(i, xl, k, xa, xold, vout, dt, [&](auto&& f) {
+        for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k, f);
+    });
+}
+
+template <bool kFast, int kNS>
+__global__ __launch_bounds__(kTileT) void k_pbf_deltap_t(const float4* __restrict__ xl,
+                                                          int* __restrict__ key2,
+                                                          const int* __restrict__ cs, int n, PbfK k,
+                                                          float4* __restrict__ xa,
+                                                          const float4* __restrict__ xold,
+                                                          float4* __restrict__ vout, float dt,
+                                                          unsigned long long* __restrict__ masks,
+                                                          const float4* __restrict__ xp,
+                                                          const int* __restrict__ gate,
+                                                          const int4* __restrict__ tiles,
+                                                          const int* __restrict__ ntiles, int lcap) {
+    if (gate && *gate == 0) return;
+    extern __shared__ float4 lds[];
+    __shared__ TileShared S;
+    const int nt = *ntiles;
+    for (int t = tile_first(); t < nt; t += gridDim.x) {
+        tile_stage(S, tiles[t], cs, k, xl, nullptr, lds, nullptr, lcap);
+        const int i = S.i0 + (int)threadIdx.x;
+        if (i < S.i1)
+            deltap_body<kFast, kNS>(i, xl, k, xa, xold, vout, dt, [&](auto&& f) {
+                for_each_neighbor_tiled<false>(S, i, n, key2, masks, xp, xl, nullptr, lds, nullptr, k, f);
+            });
+        __syncthreads();
+    }
+}
+
 hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
                              const float4* xold, float4* vout, float dt, unsigned long long* masks,
-                             const float4* xp, const int* gate, hipStream_t s) {
-    const dim3 g(grid_for(n)), b(kBlock);
+                             const float4* xp, const int* gate, const TileArgs& ta, hipStream_t s) {
     const bool n4 = k.n_scorr == 4;
+    if (ta.tiles) {
+        const dim3 g(ta.grid), b(kTileT);
+        const size_t sh = (size_t)ta.lcap * sizeof(float4);
+        if (k.exact && n4)
+            hipLaunchKernelGGL((k_pbf_deltap_t<false, 4>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
+                               xp, gate, ta.tiles, ta.ntiles, ta.lcap);
+        else if (k.exact)
+            hipLaunchKernelGGL((k_pbf_deltap_t<false, 0>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
+                               xp, gate, ta.tiles, ta.ntiles, ta.lcap);
+        else if (n4)
+            hipLaunchKernelGGL((k_pbf_deltap_t<true, 4>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
+                               xp, gate, ta.tiles, ta.ntiles, ta.lcap);
+        else
+            hipLaunchKernelGGL((k_pbf_deltap_t<true, 0>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
+                               xp, gate, ta.tiles, ta.ntiles, ta.lcap);
+        return hipGetLastError();
+    }
+    const dim3 g(grid_for(n)), b(kBlock);
     if (k.exact && n4)
         hipLaunchKernelGGL((k_pbf_deltap<false, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
     else if (k.exact)
@@ -505,6 +826,36 @@ hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, floa
 // kernel sums give the density at the final positions; its constraint's block
 // maximum goes to maxd (the learned step's fallback test and the density
 // error statistics).
+template <bool kFast, bool kCheck, class Visit>
+__device__ __forceinline__ float xsph_body(int i, const float4* __restrict__ xa, const float4* __restrict__ vt,
+                                           const PbfK& k, float4* __restrict__ vout, float4* __restrict__ xout,
+                                           Visit&& visit) {
+    const float4 pi = xa[i];
+    const float4 vi = vt[i];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, rho = 0.0f;
+    visit([&](int, const float4 pj, const float4 vj) {
+        const float dx = pi.x - pj.x;
+        const float dy = pi.y - pj.y;
+        const float dz = pi.z - pj.z;
+        const float r2 = dist2<kFast>(dx, dy, dz);
+        if (r2 <= k.h2) {
+            const float t = poly_t<kFast>(r2, k);
+            const float w = (t * t) * t;
+            sx = acc<kFast>(sx, vj.x - vi.x, w);
+            sy = acc<kFast>(sy, vj.y - vi.y, w);
+            sz = acc<kFast>(sz, vj.z - vi.z, w);
+            if (kCheck) rho += w;
+        }
+    });
+    vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
+    xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
+    const float Ci = rho * k.c_rho - 1.0f;
+    // a non-finite position (no pair passes the distance test) or a NaN
+    // must fail the density test
+    const bool finite = __builtin_isfinite(pi.x) && __builtin_isfinite(pi.y) && __builtin_isfinite(pi.z);
+    return (finite && Ci == Ci) ? Ci : __builtin_inff();
+}
+
 template <bool kFast, bool kCheck>
 __global__ __launch_bounds__(kBlock) void k_pbf_xsph(const float4* __restrict__ xa,
                                                       const float4* __restrict__ vt,
@@ -519,39 +870,66 @@ __global__ __launch_bounds__(kBlock) void k_pbf_xsph(const float4* __restrict__ 
     if (gate && *gate == 0) return;  // uniform: before any barrier
     const int i = xcd_index();
     float Ci = -1e30f;
-    if (i < n) {
-        const float4 pi = xa[i];
-        const float4 vi = vt[i];
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f, rho = 0.0f;
-        for_each_neighbor<false>(i, n, key2, cs, masks, xp, xa, vt, k,
-                                 [&](int, const float4 pj, const float4 vj) {
-            const float dx = pi.x - pj.x;
-            const float dy = pi.y - pj.y;
-            const float dz = pi.z - pj.z;
-            const float r2 = dist2<kFast>(dx, dy, dz);
-            if (r2 <= k.h2) {
-                const float t = poly_t<kFast>(r2, k);
-                const float w = (t * t) * t;
-                sx = acc<kFast>(sx, vj.x - vi.x, w);
-                sy = acc<kFast>(sy, vj.y - vi.y, w);
-                sz = acc<kFast>(sz, vj.z - vi.z, w);
-                if (kCheck) rho += w;
-            }
+    if (i < n)
+        Ci = xsph_body<kFast, kCheck>(i, xa, vt, k, vout, xout, [&](auto&& f) {
+            for_each_neighbor<false>(i, n, key2, cs, masks, xp, xa, vt, k, f);
         });
-        vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
-        xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
-        Ci = rho * k.c_rho - 1.0f;
-        // a non-finite position (no pair passes the distance test) or a NaN
-        // must fail the density test
-        const bool finite = __builtin_isfinite(pi.x) && __builtin_isfinite(pi.y) && __builtin_isfinite(pi.z);
-        Ci = (finite && Ci == Ci) ? Ci : __builtin_inff();
+    if (kCheck) block_max_sum_store(Ci, Ci > 0.0f ? Ci : 0.0f, maxd);
+}
+
+template <bool kFast, bool kCheck>
+__global__ __launch_bounds__(kTileT) void k_pbf_xsph_t(const float4* __restrict__ xa,
+                                                        const float4* __restrict__ vt,
+                                                        int* __restrict__ key2,
+                                                        const int* __restrict__ cs, int n, PbfK k,
+                                                        float4* __restrict__ vout,
+                                                        float4* __restrict__ xout,
+                                                        unsigned long long* __restrict__ masks,
+                                                        const float4* __restrict__ xp,
+                                                        int* __restrict__ maxd,
+                                                        const int* __restrict__ gate,
+                                                        const int4* __restrict__ tiles,
+                                                        const int* __restrict__ ntiles, int lcap) {
+    if (gate && *gate == 0) return;
+    extern __shared__ float4 lds[];  // positions [lcap], then velocities [lcap]
+    __shared__ TileShared S;
+    float cmax = -1e30f, csum = 0.0f;
+    const int nt = *ntiles;
+    for (int t = tile_first(); t < nt; t += gridDim.x) {
+        tile_stage(S, tiles[t], cs, k, xa, vt, lds, lds + lcap, lcap);
+        const int i = S.i0 + (int)threadIdx.x;
+        if (i < S.i1) {
+            const float c = xsph_body<kFast, kCheck>(i, xa, vt, k, vout, xout, [&](auto&& f) {
+                for_each_neighbor_tiled<false>(S, i, n, key2, masks, xp, xa, vt, lds, lds + lcap, k, f);
+            });
+            cmax = c > cmax ? c : cmax;
+            csum += c > 0.0f ? c : 0.0f;
+        }
+        __syncthreads();
     }
-    if (kCheck) block_max_sum_store(Ci, maxd);
+    if (kCheck) block_max_sum_store(cmax, csum, maxd);
 }
 
 hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
                            PbfK k, float4* vout, float4* xout, unsigned long long* masks,
-                           const float4* xp, int* maxd, const int* gate, hipStream_t s) {
+                           const float4* xp, int* maxd, const int* gate, const TileArgs& ta, hipStream_t s) {
+    if (ta.tiles) {
+        const dim3 g(ta.grid), b(kTileT);
+        const size_t sh = 2 * (size_t)ta.lcap * sizeof(float4);
+        if (k.exact && maxd)
+            hipLaunchKernelGGL((k_pbf_xsph_t<false, true>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
+                               xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
+        else if (k.exact)
+            hipLaunchKernelGGL((k_pbf_xsph_t<false, false>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
+                               xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
+        else if (maxd)
+            hipLaunchKernelGGL((k_pbf_xsph_t<true, true>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
+                               xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
+        else
+            hipLaunchKernelGGL((k_pbf_xsph_t<true, false>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
+                               xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
+        return hipGetLastError();
+    }
     const dim3 g(grid_for(n)), b(kBlock);
     if (k.exact && maxd)
         hipLaunchKernelGGL((k_pbf_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
@@ -578,21 +956,18 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 //   f[29]       mean |d| / h over N(i) \ i
 //   f[30]       mean |d|^2 / h^2 over N(i) \ i
 //   f[31]       min(1, distance to the nearest wall of the clamp box / h)
-template <bool kFast, bool kRecord>
-__global__ __launch_bounds__(kBlock) void k_pbf_features(
-    int* __restrict__ key2, const int* __restrict__ cs, int n, PbfK k,
-    unsigned long long* __restrict__ masks, const float4* __restrict__ xp,
-    const float4* __restrict__ xold, const float* __restrict__ mu, const float* __restrict__ inv_sigma,
-    bf16x8_t* __restrict__ fbf, float* __restrict__ frec, int rec_stride, int rec_off, float inv_h) {
-    const int i = xcd_index();
-    if (i >= n) return;
+template <bool kFast, bool kRecord, class Visit>
+__device__ __forceinline__ void features_body(int i, const float4* __restrict__ xp, const float4* __restrict__ xold,
+                                              const PbfK& k, const float* __restrict__ mu,
+                                              const float* __restrict__ inv_sigma, bf16x8_t* __restrict__ fbf,
+                                              float* __restrict__ frec, int rec_stride, int rec_off, float inv_h,
+                                              Visit&& visit) {
     const float4 pi = xp[i];
     float rho = 0.0f, cnt = 0.0f, sr = 0.0f, sr2 = 0.0f;
     float bx[8], by[8], bz[8], bn[8];
 #pragma unroll
     for (int b = 0; b < 8; ++b) bx[b] = by[b] = bz[b] = bn[b] = 0.0f;
-    for_each_neighbor<true>(i, n, key2, cs, masks, xp, xp, nullptr, k,
-                            [&](int j, const float4 pj, const float4) {
+    visit([&](int j, const float4 pj, const float4) {
         const float dx = pj.x - pi.x;
         const float dy = pj.y - pi.y;
         const float dz = pj.z - pi.z;
@@ -657,12 +1032,57 @@ __global__ __launch_bounds__(kBlock) void k_pbf_features(
     }
 }
 
+template <bool kFast, bool kRecord>
+__global__ __launch_bounds__(kBlock) void k_pbf_features(
+    int* __restrict__ key2, const int* __restrict__ cs, int n, PbfK k,
+    unsigned long long* __restrict__ masks, const float4* __restrict__ xp,
+    const float4* __restrict__ xold, const float* __restrict__ mu, const float* __restrict__ inv_sigma,
+    bf16x8_t* __restrict__ fbf, float* __restrict__ frec, int rec_stride, int rec_off, float inv_h) {
+    const int i = xcd_index();
+    if (i >= n) return;
+    features_body<kFast, kRecord>(i, xp, xold, k, mu, inv_sigma, fbf, frec, rec_stride, rec_off, inv_h,
+                                  [&](auto&& f) {
+        for_each_neighbor<true>(i, n, key2, cs, masks, xp, xp, nullptr, k, f);
+    });
+}
+
+template <bool kFast>
+__global__ __launch_bounds__(kTileT) void k_pbf_features_t(
+    int* __restrict__ key2, const int* __restrict__ cs, int n, PbfK k,
+    unsigned long long* __restrict__ masks, const float4* __restrict__ xp,
+    const float4* __restrict__ xold, const float* __restrict__ mu, const float* __restrict__ inv_sigma,
+    bf16x8_t* __restrict__ fbf, float inv_h, const int4* __restrict__ tiles, const int* __restrict__ ntiles,
+    int lcap) {
+    extern __shared__ float4 lds[];
+    __shared__ TileShared S;
+    const int nt = *ntiles;
+    for (int t = tile_first(); t < nt; t += gridDim.x) {
+        tile_stage(S, tiles[t], cs, k, xp, nullptr, lds, nullptr, lcap);
+        const int i = S.i0 + (int)threadIdx.x;
+        if (i < S.i1)
+            features_body<kFast, false>(i, xp, xold, k, mu, inv_sigma, fbf, nullptr, 1, 0, inv_h, [&](auto&& f) {
+                for_each_neighbor_tiled<true>(S, i, n, key2, masks, xp, xp, nullptr, lds, nullptr, k, f);
+            });
+        __syncthreads();
+    }
+}
+
 hipError_t launch_pbf_features(int* key2, const int* cs, int n, PbfK k, unsigned long long* masks,
                                const float4* xp, const float4* xold, const float* mu,
                                const float* inv_sigma, void* fbf, float* frec, int rec_stride,
-                               int rec_off, hipStream_t s) {
+                               int rec_off, const TileArgs& ta, hipStream_t s) {
     const dim3 g(grid_for(n)), b(kBlock);
     const float inv_h = 1.0f / sqrtf(k.h2);
+    if (!frec && ta.tiles) {
+        const size_t sh = (size_t)ta.lcap * sizeof(float4);
+        if (k.exact)
+            hipLaunchKernelGGL((k_pbf_features_t<false>), dim3(ta.grid), dim3(kTileT), sh, s, key2, cs, n, k, masks,
+                               xp, xold, mu, inv_sigma, (bf16x8_t*)fbf, inv_h, ta.tiles, ta.ntiles, ta.lcap);
+        else
+            hipLaunchKernelGGL((k_pbf_features_t<true>), dim3(ta.grid), dim3(kTileT), sh, s, key2, cs, n, k, masks,
+                               xp, xold, mu, inv_sigma, (bf16x8_t*)fbf, inv_h, ta.tiles, ta.ntiles, ta.lcap);
+        return hipGetLastError();
+    }
     if (frec) {
         if (rec_stride < 1) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_pbf_features<false, true>), g, b, 0, s, key2, cs, n, k, masks, xp, xold, mu,

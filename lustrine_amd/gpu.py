@@ -14,9 +14,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblustrine_gpu.so")
 
-NPHASES = 11
+NPHASES = 12
 PHASES = ("predict_key", "cell_scan", "scatter", "stable_permute", "lambda", "deltap", "xsph",
-          "lambda_build", "features", "mlp", "decide")
+          "lambda_build", "features", "mlp", "decide", "tiles")
 
 # learned warm start: network file layout (include/lustrine_gpu.h)
 FEATURES = 32
@@ -84,6 +84,7 @@ _SIGS = {
     "lus_pbf_batch_init": (C.c_int, [C.c_void_p, _P(LusPbfParams), C.c_int, _i, _f, _f, _f]),
     "lus_pbf_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
     "lus_pbf_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "lus_pbf_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "lus_pbf_step_latency": (C.c_int, [C.c_void_p, C.c_float, C.c_int, _f]),
     "lus_pbf_download": (C.c_int, [C.c_void_p, C.c_int, _f, _f, _i]),
     "lus_pbf_last_max_constraint": (C.c_int, [C.c_void_p, _f]),
@@ -317,6 +318,9 @@ class Gpu:
 
     def pbf_set_graph(self, on=True):
         _check(self._lib.lus_pbf_set_graph(self._h, int(bool(on))), "lus_pbf_set_graph")
+
+    def pbf_set_tiling(self, tiled=True, zt=16):
+        _check(self._lib.lus_pbf_set_tiling(self._h, int(bool(tiled)), zt), "lus_pbf_set_tiling")
 
     def pbf_step_latency(self, dt, steps):
         ms = np.zeros(steps, np.float32)
