@@ -10,8 +10,12 @@ exactly --steps steps after --warmup untimed steps.
 Per-kernel times come from a second run over the same simulation window (the
 scene is re-initialised, same warmup and steps) with an event after every
 kernel; the roofline entry uses the dominant kernel's average launch time from
-that run.  The CPU baseline is the FP32 oracle (one thread) timed on a bounded
-sample of the same scene on this host.  Prints one JSON line.
+that run.  `learned_vs_plain` compares the learned warm start (network
+lustrine_amd/data/warm_start.npz, one plain iteration) with plain K = 4..16 on
+the same window and the same density-error measure.  The CPU baseline is the
+FP32 oracle (one thread) timed on a bounded sample of the same scene on this
+host.  Prints one JSON line.  --workload cfg3 runs the full learned sweep,
+cfg4 the 64-scene batch.
 """
 from __future__ import annotations
 
@@ -182,6 +186,24 @@ def equal_error_speedup(plain, learned_row, key="err_mean"):
     return learned_row["particle_steps_per_s"] / best["particle_steps_per_s"], best["iterations"]
 
 
+def learned_vs_plain(G, cfg, x0, dt, warmup, steps, net, sweep=(4, 8, 12, 16)):
+    """The compact comparison of the default run: plain K over `sweep` and the
+    learned step with one plain iteration, same window, same error measure."""
+    with G.Gpu(0) as g:
+        g.pbf_set_network(net)
+        plain = [measure_mode(g, G, cfg, x0, dt, warmup, steps, k) for k in sweep]
+        lrn = measure_mode(g, G, cfg, x0, dt, warmup, steps, cfg.iterations, learned_iters=1)
+    out = {"learned": lrn, "plain_sweep": plain}
+    for key in ("err_mean", "err_max", "avg_compression_mean"):
+        sp, k = equal_error_speedup(plain, lrn, key)
+        out[f"speedup_at_equal_{key}"] = sp
+        out[f"plain_iterations_at_equal_{key}"] = k
+    out["note"] = ("error e_t = max_i max(0, rho_i/rho0 - 1) at the final positions per step; speedup None: "
+                   f"no plain K in {list(sweep)} reaches the learned error (then the learned throughput over "
+                   "the largest K's bounds it from below)")
+    return out
+
+
 def bench_learned(args, G, scenes) -> int:
     """Learned warm start vs the plain solver on the 1M dam break: an
     error-vs-throughput curve (plain K over PLAIN_SWEEP, learned with one plain
@@ -239,18 +261,78 @@ def bench_learned(args, G, scenes) -> int:
     return 0
 
 
+def bench_mixed(args, G, scenes) -> int:
+    """Mixed sand + fluid over the voxel heightfield: moving particle-steps/s
+    (solids are not counted), 3 iterations, hipGraph replay."""
+    m = scenes.cfg1()
+    pos, ph = m.arrays()
+    params = G.pbf_params(m.fluid, exact=args.exact)
+    mp = G.mix_params(mu_s=m.mu_s, mu_k=m.mu_k, collision=m.collision, friction=m.friction)
+    dt = np.float32(m.fluid.dt)
+    with G.Gpu(0) as g:
+        g.mix_init(params, mp, pos, ph)
+        g.pbf_set_graph(not args.no_graph)
+        ms, _, _ = g.pbf_bench(dt, args.warmup, args.steps, phases=False)
+        phase_ms = launches = None
+        if not args.no_phases:
+            g.mix_init(params, mp, pos, ph)
+            _, phase_ms, launches = g.pbf_bench(dt, args.warmup, args.steps, phases=True)
+    value = m.n_moving * args.steps / (ms / 1e3)
+    out = {
+        "metric": "mixed sand+fluid moving particle-steps/s",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": m.name, "moving": m.n_moving, "solids": int((ph == 2).sum()),
+                   "sand": int((ph == 1).sum()), "fluid": int((ph == 0).sum()), "box": list(m.fluid.box),
+                   "iterations": m.fluid.iterations, "mu_s": m.mu_s, "mu_k": m.mu_k,
+                   "arithmetic": "exact" if args.exact else "fused",
+                   "launch": "plain" if args.no_graph else "hipgraph", "parallelism": "single-gpu"},
+    }
+    if phase_ms is not None:
+        out["phases"] = {k: {"total_ms": round(t, 4), "launches": launches[k],
+                             "avg_launch_us": round(t / launches[k] * 1e3, 2)}
+                         for k, t in phase_ms.items() if launches[k]}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        O.build()
+        p = O.pbf_params(m.fluid)
+        omp = O.mix_params(mu_s=m.mu_s, mu_k=m.mu_k, collision=m.collision, friction=m.friction)
+        x, v = pos.copy(), np.zeros_like(pos)
+        ids = np.arange(pos.shape[0], dtype=np.int32)
+        phs = ph.copy()
+        O.mixed_step(p, omp, x, v, ids, phs, dt)
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            O.mixed_step(p, omp, x, v, ids, phs, dt)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= args.cpu_seconds or steps >= 50:
+                break
+        cb = {"value": m.n_moving * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+              "sample": f"steps 2-{steps + 1} of the mixed scene ({el:.1f} s), FP32 oracle, 1 thread of {cpu_model()}"}
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg2", "cfg2_tail", "cfg3", "cfg4"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg1", "cfg2", "cfg2_tail", "cfg3", "cfg4"])
     ap.add_argument("--network", default=None, help="warm-start network file (cfg3)")
     ap.add_argument("--iterations", type=int, default=None)
     ap.add_argument("--exact", action="store_true",
                     help="oracle operation order (bit-identical results) instead of the fused path")
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--no-phases", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-vs-plain comparison of the default run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args(argv)
@@ -270,6 +352,8 @@ def main(argv=None):
         return bench_batch(args, G, scenes)
     if args.workload == "cfg3":
         return bench_learned(args, G, scenes)
+    if args.workload == "cfg1":
+        return bench_mixed(args, G, scenes)
     cfg = {"cfg0": scenes.cfg0, "cfg2": scenes.cfg2, "cfg2_tail": scenes.cfg2_tail}[args.workload]()
     if args.iterations is not None:
         cfg.iterations = args.iterations
@@ -343,6 +427,11 @@ def main(argv=None):
                            "alg_bytes_per_launch": d["alg_bytes_per_launch"],
                            "measured_copy_GBps": round(copy_gbps, 1)}
         out["phases"] = phases
+    if not args.no_learned and args.workload == "cfg2" and cfg.iterations >= 1:
+        from lustrine_amd import learned as L
+        if os.path.exists(L.DEFAULT_NETWORK):
+            net, _ = L.load(L.DEFAULT_NETWORK)
+            out["learned_vs_plain"] = learned_vs_plain(G, cfg, x0, dt, args.warmup, args.steps, net)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(cfg, args.cpu_seconds)
         out["cpu_baseline"] = cb

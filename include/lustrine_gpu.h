@@ -132,6 +132,22 @@ int lus_pbf_init(LusGpu* ctx, const LusPbfParams* p, int n, const float* pos, co
  * concatenation; lus_pbf_download with order = 1 returns that order. */
 int lus_pbf_batch_init(LusGpu* ctx, const LusPbfParams* p, int nscenes, const int* counts,
                        const float* boxes, const float* pos, const float* vel);
+/* Mixed sand + fluid over static solid particles: one particle array with
+ * phase[i] = 0 fluid, 1 sand, 2 solid (solids never move).  Fluid follows the
+ * PBF step with sand and solids counted in its density (their lambda is 0);
+ * sand follows the contact + friction projection of simulate_sand
+ * (src/Simulate.cpp:226-311) against sand (half weight, friction), fluid
+ * (half weight, contact only) and solid (full weight, friction) neighbours
+ * within `diameter`; p->iterations Jacobi iterations for both; XSPH over
+ * fluid pairs.  Steps with lus_pbf_step; lus_pbf_download returns every
+ * phase (the ids give the input order).  No reference counterpart for the
+ * coupling (the reference runs sand or fluid, src/LustrineWrapper.cpp:666-677). */
+typedef struct LusMixParams {
+    float diameter;                          /* contact distance d = 2r */
+    float collision, friction, mu_s, mu_k;   /* roles of src/Simulate.cpp:159-163 */
+} LusMixParams;
+int lus_mix_init(LusGpu* ctx, const LusPbfParams* p, const LusMixParams* mp, int n, const float* pos,
+                 const float* vel, const int* phase);
 /* Enqueues `steps` substeps of length dt on the context stream. */
 int lus_pbf_step(LusGpu* ctx, float dt, int steps);
 /* on = 1: lus_pbf_step replays a hipGraph holding two captured substeps

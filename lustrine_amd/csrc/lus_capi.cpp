@@ -135,6 +135,10 @@ struct PbfState {
     bool learned = false, check = false;
     int learned_iters = 1;
     float bound = 0.01f;
+    // mixed sand + fluid scene: phases ping-pong like id
+    bool mixed = false;
+    int *ph = nullptr, *ph2 = nullptr;
+    int n_fluid = 0;
     // tiled neighbour passes (LDS staging of each tile's 9 stencil rows);
     // off by default: measured slower than one thread per slot on gfx950
     bool tiled = false;
@@ -283,6 +287,10 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         k.sid = P.sid;
         k.sid_out = P.sid2;
     }
+    if (P.mixed) {
+        k.ph = P.ph;
+        k.ph_out = P.ph2;
+    }
     if (ev) LUS_HIP(mark(-1));
     LUS_HIP(lus::launch_pbf_key(P.x, P.v, P.n, k, dt, P.key, P.slot, P.counts, s));
     LUS_HIP(mark(0));
@@ -295,7 +303,7 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
     LUS_HIP(mark(3));
     // tiles of the step (cell columns cut into chunks of P.zt cells), or flat launches
     lus::TileArgs ta, tx;
-    if (P.tiled) {
+    if (P.tiled && !P.mixed) {
         LUS_HIP(lus::launch_tiles(P.cs, P.key2, k, P.zt, P.tcount, P.toff, P.tscan, P.tl, s));
         LUS_HIP(mark(11));
         ta.tiles = tx.tiles = P.tl;
@@ -304,9 +312,39 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         ta.lcap = P.lcap;
         tx.lcap = P.lcap_x;
     }
-    const int nred = P.tiled ? P.tgrid : P.nblk;  // block-reduction entries written
+    const int nred = ta.tiles ? P.tgrid : P.nblk;  // block-reduction entries written
     const int K = P.p.iterations;
     const bool learned = P.learned && !rec;
+    if (P.mixed) {
+        // mixed sand + fluid: lambda (fluid), correction (fluid / sand), XSPH (fluid)
+        for (int it = 0; it < K; ++it) {
+            LUS_HIP(lus::launch_mix_lambda(it == 0 ? P.xp : P.xa, P.key2, P.cs, P.n, k, P.xl, P.masks, P.xp,
+                                           it == 0 ? 1 : 0, s));
+            LUS_HIP(mark(it == 0 ? 7 : 4));
+            LUS_HIP(lus::launch_mix_correct(P.xl, P.key2, P.cs, P.n, k, P.xa, P.x2, it == K - 1 ? P.v2 : nullptr,
+                                            dt, P.masks, P.xp, s));
+            LUS_HIP(mark(5));
+        }
+        if (K == 0) {
+            LUS_HIP(lus::launch_mix_lambda(P.xp, P.key2, P.cs, P.n, k, P.xl, P.masks, P.xp, 1, s));
+            LUS_HIP(lus::launch_pbf_velocity(P.xp, P.x2, P.n, dt, P.v2, s));
+        }
+        LUS_HIP(lus::launch_mix_xsph(K > 0 ? P.xa : P.xp, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp,
+                                     P.check ? P.maxd : nullptr, s));
+        LUS_HIP(mark(6));
+        if (P.check) {
+            LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, std::max(P.n_fluid, 1), P.bound, 0, P.flag, P.counters,
+                                           P.hist, LUS_DENSITY_HISTORY, s));
+            LUS_HIP(mark(10));
+        }
+        P.nred = P.nblk;
+        std::swap(P.x, P.x2);
+        std::swap(P.id, P.id2);
+        std::swap(P.ph, P.ph2);
+        P.parity ^= 1;
+        P.stepped = true;
+        return LUS_OK;
+    }
     // The plain iterations: xp (predicted) -> lambda -> xl -> delta-p -> xa ->
     // lambda -> xl ...; the last delta-p pass also writes v2 = (x* - x) / dt.
     // built: the neighbour masks exist already (features pass).
@@ -406,8 +444,8 @@ void pbf_drop_graph(PbfState& P) {
 // (two steps restore the ping-pong buffer assignment the capture baked in)
 int pbf_run(LusGpu* c, float dt, int steps) {
     PbfState& P = c->pbf;
-    if (P.learned && (!c->net.ready || P.p.iterations < 1))
-        return set_error(LUS_ESTATE, "learned PBF step: needs a network and iterations >= 1");
+    if (P.learned && (!c->net.ready || P.p.iterations < 1 || P.mixed))
+        return set_error(LUS_ESTATE, "learned PBF step: needs a network, iterations >= 1 and a fluid scene");
     if (!P.use_graph) {
         for (int s = 0; s < steps; ++s) {
             int r = pbf_enqueue(c, dt, nullptr, nullptr);
@@ -765,6 +803,35 @@ int lus_pbf_batch_init(LusGpu* c, const LusPbfParams* p, int nscenes, const int*
     return pbf_setup(c, &q, (int)total, pos, vel, &tab, &sid);
 }
 
+int lus_mix_init(LusGpu* c, const LusPbfParams* p, const LusMixParams* mp, int n, const float* pos,
+                 const float* vel, const int* phase) {
+    if (!c || !p || !mp || n <= 0 || !pos || !phase) return set_error(LUS_EINVAL, "lus_mix_init: bad arguments");
+    if (!(mp->diameter > 0.0f) || !(mp->collision >= 0.0f) || !(mp->friction >= 0.0f) || !(mp->mu_s >= 0.0f) ||
+        !(mp->mu_k >= 0.0f))
+        return set_error(LUS_EINVAL, "lus_mix_init: bad sand constants");
+    int nf = 0;
+    for (int i = 0; i < n; ++i) {
+        if (phase[i] < 0 || phase[i] > 2) return set_error(LUS_EINVAL, "lus_mix_init: phase %d at %d", phase[i], i);
+        nf += phase[i] == 0;
+    }
+    if (c->pbf.learned) return set_error(LUS_EINVAL, "lus_mix_init: learned mode is on");
+    int r = lus_pbf_init(c, p, n, pos, vel);
+    if (r) return r;
+    PbfState& P = c->pbf;
+    LUS_HIP(P.mem.alloc(&P.ph, n));
+    LUS_HIP(P.mem.alloc(&P.ph2, n));
+    LUS_HIP(hipMemcpy(P.ph, phase, sizeof(int) * n, hipMemcpyHostToDevice));
+    LUS_HIP(hipDeviceSynchronize());
+    P.k.mix_d = mp->diameter;
+    P.k.mix_collision = mp->collision;
+    P.k.mix_friction = mp->friction;
+    P.k.mix_mu_s = mp->mu_s;
+    P.k.mix_mu_k = mp->mu_k;
+    P.mixed = true;
+    P.n_fluid = nf;
+    return LUS_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -780,6 +847,9 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
     P.parity = 0;
     P.scenes = nullptr;
     P.sid = P.sid2 = nullptr;
+    P.mixed = false;
+    P.ph = P.ph2 = nullptr;
+    P.n_fluid = n;
     P.nscenes = scenes ? (int)scenes->size() : 0;
     P.p = *p;
     P.k = pbf_constants(*p);
@@ -1175,6 +1245,8 @@ int lus_pbf_set_mode(LusGpu* c, int learned, int learned_iterations, float bound
     PbfState& P = c->pbf;
     if (learned && P.ready && P.p.iterations < 1)
         return set_error(LUS_EINVAL, "lus_pbf_set_mode: the plain rerun needs iterations >= 1");
+    if (learned && P.ready && P.mixed)
+        return set_error(LUS_EINVAL, "lus_pbf_set_mode: no learned step for mixed sand + fluid scenes");
     LUS_HIP(hipSetDevice(c->device));
     LUS_HIP(hipStreamSynchronize(c->stream));
     pbf_drop_graph(P);

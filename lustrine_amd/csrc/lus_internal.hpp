@@ -33,6 +33,10 @@ struct PbfK {
     const SceneBox* scenes;     // batch table, nullptr for a single scene
     const int* sid;             // scene of each particle (input order), batch only
     int* sid_out;               // scene of each particle in cell order, batch only
+    // mixed sand + fluid scene (nullptr ph: every particle is fluid)
+    const int* ph;              // phase (0 fluid, 1 sand, 2 solid), input order
+    int* ph_out;                // phase in cell order
+    float mix_d, mix_collision, mix_friction, mix_mu_s, mix_mu_k;
 };
 
 // simulate_sand constants (src/Simulate.cpp:159-163 and the Simulation fields).
@@ -117,6 +121,16 @@ hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, floa
 hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
                            PbfK k, float4* vout, float4* xout, unsigned long long* masks,
                            const float4* xp, int* maxd, const int* gate, const TileArgs& ta, hipStream_t s);
+
+// ---- mixed sand + fluid passes (k.ph, k.ph_out set) ----
+hipError_t launch_mix_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
+                             unsigned long long* masks, const float4* xp, int build, hipStream_t s);
+hipError_t launch_mix_correct(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
+                              const float4* xold, float4* vout, float dt, unsigned long long* masks,
+                              const float4* xp, hipStream_t s);
+hipError_t launch_mix_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n, PbfK k,
+                           float4* vout, float4* xout, unsigned long long* masks, const float4* xp, int* maxd,
+                           hipStream_t s);
 
 // ---- learned warm start ----
 // Per-particle features of the predicted positions (build pass: also writes

@@ -429,7 +429,8 @@ __global__ __launch_bounds__(kBlock) void k_pbf_key(const float4* __restrict__ x
     int c = 0;
     if (active) {
         const Bounds b = bounds_of(k, k.scenes ? k.sid[i] : 0);
-        const float4 xs = pbf_predict(x[i], v[i], k, b, dt);
+        // solids of a mixed scene keep their position
+        const float4 xs = (k.ph && k.ph[i] == 2) ? x[i] : pbf_predict(x[i], v[i], k, b, dt);
         c = pbf_cell(xs, k, b.yoff);
         key[i] = c;
     }
@@ -458,10 +459,12 @@ __global__ __launch_bounds__(kBlock) void k_pbf_permute(
     const float4 xi = x[i];
     const int s = k.scenes ? k.sid[i] : 0;
     x2[o] = xi;
-    xp[o] = pbf_predict(xi, v[i], k, bounds_of(k, s), dt);
+    const int ph = k.ph ? k.ph[i] : 0;
+    xp[o] = ph == 2 ? xi : pbf_predict(xi, v[i], k, bounds_of(k, s), dt);
     id2[o] = id[i];
     key2[o] = c;
     if (k.scenes) k.sid_out[o] = s;
+    if (k.ph) k.ph_out[o] = ph;
 }
 
 hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_start,
@@ -939,6 +942,257 @@ hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const 
         hipLaunchKernelGGL((k_pbf_xsph<true, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
     else
         hipLaunchKernelGGL((k_pbf_xsph<true, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Mixed sand + fluid over static solids.  One cell-ordered array holds every
+// phase (k.ph_out[slot]: 0 fluid, 1 sand, 2 solid) and shares the neighbour
+// sets.  Fluid: density over every neighbour, the other-particle gradient term
+// over fluid neighbours only, lambda = 0 for sand and solids, delta-p as the
+// PBF pass.  Sand: the contact + friction projection of simulate_sand
+// (src/Simulate.cpp:240-300) against sand (half weight, friction), fluid
+// (half weight, contact only) and solid (full weight, friction) neighbours
+// within d, friction against the step's old positions.  Solids do not move.
+// The sand arithmetic is IEEE in both arithmetic modes.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mix_sand_pair(const float4 pi, const float4 xoi, const float4 pj, const float4 xoj,
+                                              int jph, const PbfK& k, float& dpx, float& dpy, float& dpz) {
+    float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
+    if (sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz)) == 0.0f) {
+        ijx = 0.0f;
+        ijy = 0.00001f;
+        ijz = 0.0f;
+    }
+    const float len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
+    if (len > k.mix_d) return;
+    float sc;
+    if (jph == 2) {
+        sc = k.mix_collision * (len - k.mix_d);
+    } else {
+        sc = k.mix_collision * 0.5f;
+        sc = sc * (len - k.mix_d);
+    }
+    const float tx = (sc * ijx) / len, ty = (sc * ijy) / len, tz = (sc * ijz) / len;
+    dpx = dpx - tx;
+    dpy = dpy - ty;
+    dpz = dpz - tz;
+    if (jph == 0) return;  // fluid neighbour: contact only
+    const float dd = sqrtf(dot3(tx, ty, tz, tx, ty, tz));
+    const float xidx = (pi.x - tx) - xoi.x, xidy = (pi.y - ty) - xoi.y, xidz = (pi.z - tz) - xoi.z;
+    float xjdx, xjdy, xjdz, nx, ny, nz;
+    if (jph == 1) {
+        xjdx = (pj.x + tx) - xoj.x;
+        xjdy = (pj.y + ty) - xoj.y;
+        xjdz = (pj.z + tz) - xoj.z;
+        nx = (pi.x - tx) - (pj.x + tx);
+        ny = (pi.y - ty) - (pj.y + ty);
+        nz = (pi.z - tz) - (pj.z + tz);
+    } else {
+        xjdx = 0.0f;
+        xjdy = 0.0f;
+        xjdz = 0.0f;
+        nx = (pi.x - tx) - pj.x;
+        ny = (pi.y - ty) - pj.y;
+        nz = (pi.z - tz) - pj.z;
+    }
+    const float inv = 1.0f / sqrtf(dot3(nx, ny, nz, nx, ny, nz));
+    nx = nx * inv;
+    ny = ny * inv;
+    nz = nz * inv;
+    const float rx = xidx - xjdx, ry = xidy - xjdy, rz = xidz - xjdz;
+    const float rd = dot3(rx, ry, rz, nx, ny, nz);
+    const float xtx = rx - rd * nx, xty = ry - rd * ny, xtz = rz - rd * nz;
+    const float lt = sqrtf(dot3(xtx, xty, xtz, xtx, xty, xtz));
+    if ((dd * k.mix_mu_s) > (lt + 1e-9f)) {
+        dpx = dpx - k.mix_friction * xtx;
+        dpy = dpy - k.mix_friction * xty;
+        dpz = dpz - k.mix_friction * xtz;
+    } else {
+        const float q = (k.mix_mu_k * dd) / (lt + 1e-9f);
+        const float ratio = q < 1.0f ? q : 1.0f;
+        dpx = dpx - (k.mix_friction * xtx) * ratio;
+        dpy = dpy - (k.mix_friction * xty) * ratio;
+        dpz = dpz - (k.mix_friction * xtz) * ratio;
+    }
+}
+
+template <bool kBuild, bool kFast>
+__global__ __launch_bounds__(kBlock) void k_mix_lambda(const float4* __restrict__ xa, int* __restrict__ key2,
+                                                       const int* __restrict__ cs, int n, PbfK k,
+                                                       float4* __restrict__ xl,
+                                                       unsigned long long* __restrict__ masks,
+                                                       const float4* __restrict__ xp) {
+    const int i = xcd_index();
+    if (i >= n) return;
+    const float4 pi = xa[i];
+    const int ph = k.ph_out[i];
+    if (ph != 0) {
+        // no constraint; a building pass still writes the masks
+        if (kBuild)
+            for_each_neighbor<true>(i, n, key2, cs, masks, xp, xa, nullptr, k, [](int, const float4, const float4) {});
+        xl[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
+        return;
+    }
+    float rho = 0.0f, a = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    for_each_neighbor<kBuild>(i, n, key2, cs, masks, xp, xa, nullptr, k, [&](int j, const float4 pj, const float4) {
+        const float dx = pi.x - pj.x;
+        const float dy = pi.y - pj.y;
+        const float dz = pi.z - pj.z;
+        const float r2 = dist2<kFast>(dx, dy, dz);
+        if (r2 <= k.h2) {
+            const float t = poly_t<kFast>(r2, k);
+            rho = acc<kFast>(rho, t * t, t);
+            if (r2 > 0.0f) {
+                const float s = spiky_s<kFast>(r2, k);
+                if (k.ph_out[j] == 0) a = acc<kFast>(a, s * s, r2);
+                gx = acc<kFast>(gx, s, dx);
+                gy = acc<kFast>(gy, s, dy);
+                gz = acc<kFast>(gz, s, dz);
+            }
+        }
+    });
+    const float Ci = rho * k.c_rho - 1.0f;
+    const float gg = gx * gx + gy * gy + gz * gz;
+    const float den = k.mr0sq * (a + gg) + k.eps;
+    xl[i] = make_float4(pi.x, pi.y, pi.z, -Ci / den);
+}
+
+hipError_t launch_mix_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
+                             unsigned long long* masks, const float4* xp, int build, hipStream_t s) {
+    const dim3 g(grid_for(n)), b(kBlock);
+    if (build && k.exact)
+        hipLaunchKernelGGL((k_mix_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
+    else if (build)
+        hipLaunchKernelGGL((k_mix_lambda<true, true>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
+    else if (k.exact)
+        hipLaunchKernelGGL((k_mix_lambda<false, false>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
+    else
+        hipLaunchKernelGGL((k_mix_lambda<false, true>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
+    return hipGetLastError();
+}
+
+template <bool kFast>
+__global__ __launch_bounds__(kBlock) void k_mix_correct(const float4* __restrict__ xl, int* __restrict__ key2,
+                                                        const int* __restrict__ cs, int n, PbfK k,
+                                                        float4* __restrict__ xa, const float4* __restrict__ xold,
+                                                        float4* __restrict__ vout, float dt,
+                                                        unsigned long long* __restrict__ masks,
+                                                        const float4* __restrict__ xp) {
+    const int i = xcd_index();
+    if (i >= n) return;
+    const int ph = k.ph_out[i];
+    const float4 pi = xl[i];
+    float4 r = make_float4(pi.x, pi.y, pi.z, 0.0f);
+    if (ph == 1) {
+        const float4 xoi = xold[i];
+        float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
+        for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k, [&](int j, const float4 pj, const float4) {
+            if (j == i) return;
+            mix_sand_pair(pi, xoi, pj, xold[j], k.ph_out[j], k, dpx, dpy, dpz);
+        });
+        r.x = rclamp(pi.x + dpx, k.lo[0], k.hi[0]);
+        r.y = rclamp(pi.y + dpy, k.lo[1], k.hi[1]);
+        r.z = rclamp(pi.z + dpz, k.lo[2], k.hi[2]);
+    } else if (ph == 0) {
+        const float li = pi.w;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k, [&](int, const float4 pj, const float4) {
+            const float dx = pi.x - pj.x;
+            const float dy = pi.y - pj.y;
+            const float dz = pi.z - pj.z;
+            const float r2 = dist2<kFast>(dx, dy, dz);
+            if (r2 <= k.h2 && r2 > 0.0f) {
+                const float s = spiky_s<kFast>(r2, k);
+                const float t = poly_t<kFast>(r2, k);
+                const float ratio = ((t * t) * t) * k.inv_tq3;
+                float pw = ratio;
+                for (int e = 1; e < k.n_scorr; ++e) pw = pw * ratio;
+                const float sc = -k.scorr_k * pw;
+                const float coef = (li + pj.w) + sc;
+                const float c = coef * s;
+                ax = acc<kFast>(ax, c, dx);
+                ay = acc<kFast>(ay, c, dy);
+                az = acc<kFast>(az, c, dz);
+            }
+        });
+        r.x = rclamp(pi.x + ax * k.neg_inv_r0, k.lo[0], k.hi[0]);
+        r.y = rclamp(pi.y + ay * k.neg_inv_r0, k.lo[1], k.hi[1]);
+        r.z = rclamp(pi.z + az * k.neg_inv_r0, k.lo[2], k.hi[2]);
+    }
+    xa[i] = r;
+    if (vout) {
+        const float4 xo = xold[i];
+        vout[i] = make_float4((r.x - xo.x) / dt, (r.y - xo.y) / dt, (r.z - xo.z) / dt, 0.0f);
+    }
+}
+
+hipError_t launch_mix_correct(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
+                              const float4* xold, float4* vout, float dt, unsigned long long* masks,
+                              const float4* xp, hipStream_t s) {
+    const dim3 g(grid_for(n)), b(kBlock);
+    if (k.exact)
+        hipLaunchKernelGGL((k_mix_correct<false>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+    else
+        hipLaunchKernelGGL((k_mix_correct<true>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+    return hipGetLastError();
+}
+
+template <bool kFast, bool kCheck>
+__global__ __launch_bounds__(kBlock) void k_mix_xsph(const float4* __restrict__ xa, const float4* __restrict__ vt,
+                                                     int* __restrict__ key2, const int* __restrict__ cs, int n,
+                                                     PbfK k, float4* __restrict__ vout, float4* __restrict__ xout,
+                                                     unsigned long long* __restrict__ masks,
+                                                     const float4* __restrict__ xp, int* __restrict__ maxd) {
+    const int i = xcd_index();
+    float Ci = -1e30f;
+    if (i < n) {
+        const float4 pi = xa[i];
+        const float4 vi = vt[i];
+        if (k.ph_out[i] != 0) {
+            vout[i] = vi;
+            xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
+        } else {
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f, rho = 0.0f;
+            for_each_neighbor<false>(i, n, key2, cs, masks, xp, xa, vt, k,
+                                     [&](int j, const float4 pj, const float4 vj) {
+                const float dx = pi.x - pj.x;
+                const float dy = pi.y - pj.y;
+                const float dz = pi.z - pj.z;
+                const float r2 = dist2<kFast>(dx, dy, dz);
+                if (r2 <= k.h2) {
+                    const float t = poly_t<kFast>(r2, k);
+                    const float w = (t * t) * t;
+                    rho += w;
+                    if (k.ph_out[j] == 0) {
+                        sx = acc<kFast>(sx, vj.x - vi.x, w);
+                        sy = acc<kFast>(sy, vj.y - vi.y, w);
+                        sz = acc<kFast>(sz, vj.z - vi.z, w);
+                    }
+                }
+            });
+            vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
+            xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
+            const float c = rho * k.c_rho - 1.0f;
+            const bool finite = __builtin_isfinite(pi.x) && __builtin_isfinite(pi.y) && __builtin_isfinite(pi.z);
+            Ci = (finite && c == c) ? c : __builtin_inff();
+        }
+    }
+    if (kCheck) block_max_sum_store(Ci, Ci > 0.0f ? Ci : 0.0f, maxd);
+}
+
+hipError_t launch_mix_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n, PbfK k,
+                           float4* vout, float4* xout, unsigned long long* masks, const float4* xp, int* maxd,
+                           hipStream_t s) {
+    const dim3 g(grid_for(n)), b(kBlock);
+    if (k.exact && maxd)
+        hipLaunchKernelGGL((k_mix_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
+    else if (k.exact)
+        hipLaunchKernelGGL((k_mix_xsph<false, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
+    else if (maxd)
+        hipLaunchKernelGGL((k_mix_xsph<true, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
+    else
+        hipLaunchKernelGGL((k_mix_xsph<true, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
     return hipGetLastError();
 }
 

@@ -64,6 +64,11 @@ _f = _P(C.c_float)
 _i = _P(C.c_int)
 _l = _P(C.c_long)
 
+class LusMixParams(C.Structure):
+    _fields_ = [("diameter", C.c_float), ("collision", C.c_float), ("friction", C.c_float),
+                ("mu_s", C.c_float), ("mu_k", C.c_float)]
+
+
 class LusMlpFitParams(C.Structure):
     _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float), ("lr_final", C.c_float),
                 ("val_fraction", C.c_float), ("seed", C.c_ulonglong)]
@@ -82,6 +87,7 @@ _SIGS = {
     "lus_gpu_sync": (C.c_int, [C.c_void_p]),
     "lus_pbf_init": (C.c_int, [C.c_void_p, _P(LusPbfParams), C.c_int, _f, _f]),
     "lus_pbf_batch_init": (C.c_int, [C.c_void_p, _P(LusPbfParams), C.c_int, _i, _f, _f, _f]),
+    "lus_mix_init": (C.c_int, [C.c_void_p, _P(LusPbfParams), _P(LusMixParams), C.c_int, _f, _f, _i]),
     "lus_pbf_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
     "lus_pbf_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
@@ -232,6 +238,12 @@ def pbf_params(cfg, iterations=None, exact=True) -> LusPbfParams:
     return p
 
 
+def mix_params(diameter=1.0, collision=0.8, friction=0.7, mu_s=0.6, mu_k=0.4) -> LusMixParams:
+    """Sand constants of the mixed scene: contact distance 2r, the collision
+    and friction coefficients of src/Simulate.cpp:159-161, friction limits."""
+    return LusMixParams(diameter, collision, friction, mu_s, mu_k)
+
+
 def sand_params(X, Y, Z, radius=0.5, diameter=1.0, kernel_radius_scale=3.1, substeps=4,
                 gravity=(0.0, -10.0, 0.0), mass=5.0, collision=0.8, friction=0.7, mu_s=0.95,
                 mu_k=0.9) -> LusSandParams:
@@ -311,6 +323,15 @@ class Gpu:
         assert pos.shape[0] == int(counts.sum()) and boxes.shape[0] == counts.shape[0]
         _check(self._lib.lus_pbf_batch_init(self._h, C.byref(params), counts.shape[0], _ptr_i(counts),
                                             _ptr_f(boxes), _ptr_f(pos), _ptr_f(vel)), "lus_pbf_batch_init")
+        self.n = pos.shape[0]
+
+    def mix_init(self, params: LusPbfParams, mix: LusMixParams, pos, phase, vel=None):
+        pos = _f32(pos, (-1, 3))
+        phase = np.ascontiguousarray(phase, np.int32)
+        vel = None if vel is None else _f32(vel, (-1, 3))
+        assert phase.shape[0] == pos.shape[0]
+        _check(self._lib.lus_mix_init(self._h, C.byref(params), C.byref(mix), pos.shape[0], _ptr_f(pos),
+                                      _ptr_f(vel), _ptr_i(phase)), "lus_mix_init")
         self.n = pos.shape[0]
 
     def pbf_step(self, dt, steps=1):

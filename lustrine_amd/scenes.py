@@ -190,6 +190,79 @@ def dam_break_aspect(n_target: int, aspect: float, seed: int, **kw) -> PbfConfig
 
 
 @dataclass
+class MixedConfig:
+    """Sand + fluid block over a voxel heightfield of static solid particles.
+
+    The moving block is `fluid`'s jittered lattice; each moving particle is
+    sand with probability 1/2 (Philox, seed `phase_seed`, counter (i, 0)).
+    Column (x, z) of the field has height hmin + floor((hmax - hmin + 1) u)
+    (Philox, seed `field_seed`, counter (x * nz + z, 0)) and one solid particle
+    at each voxel centre (x + 0.5, y + 0.5, z + 0.5), y < height."""
+    name: str
+    fluid: PbfConfig
+    field: tuple = (128, 128)
+    hmin: int = 1
+    hmax: int = 16
+    field_seed: int = 11
+    phase_seed: int = 12
+    mu_s: float = 0.6
+    mu_k: float = 0.4
+    collision: float = 0.8          # src/Simulate.cpp:159-160
+    friction: float = 0.7           # src/Simulate.cpp:161
+
+    def heights(self) -> np.ndarray:
+        fx, fz = self.field
+        u = uniform01(fx * fz, 1, self.field_seed)[:, 0]
+        return (self.hmin + np.floor((self.hmax - self.hmin + 1) * u)).astype(np.int32).reshape(fx, fz)
+
+    def solids(self) -> np.ndarray:
+        h = self.heights()
+        fx, fz = self.field
+        out = []
+        for x in range(fx):
+            for z in range(fz):
+                y = np.arange(h[x, z], dtype=np.float32)
+                col = np.empty((y.shape[0], 3), np.float32)
+                col[:, 0] = np.float32(x) + np.float32(0.5)
+                col[:, 1] = y + np.float32(0.5)
+                col[:, 2] = np.float32(z) + np.float32(0.5)
+                out.append(col)
+        return np.concatenate(out).astype(np.float32)
+
+    def moving_phase(self) -> np.ndarray:
+        u = uniform01(self.fluid.n, 1, self.phase_seed)[:, 0]
+        return np.where(u < 0.5, 1, 0).astype(np.int32)
+
+    def arrays(self):
+        """(positions, phases): the moving block followed by the solids."""
+        sol = self.solids()
+        pos = np.concatenate([self.fluid.positions(), sol]).astype(np.float32)
+        ph = np.concatenate([self.moving_phase(), np.full(sol.shape[0], 2, np.int32)])
+        return pos, ph
+
+    @property
+    def n_moving(self) -> int:
+        return self.fluid.n
+
+
+def cfg1(**kw) -> MixedConfig:
+    """Mixed sand + fluid: 262,144 moving particles (64^3 lattice, block
+    origin (32, 17, 32)) over a 128 x 128 heightfield with heights 1..16
+    (about 139 k solids) in a 128^3 box, 3 iterations, 200 steps."""
+    fl = PbfConfig(name="mixed_sand_fluid_262k", lattice=(64, 64, 64), box=(128, 128, 128),
+                   origin=(32.0, 17.0, 32.0), iterations=3, steps=200, seed=1)
+    return MixedConfig(name="mixed_sand_fluid_262k", fluid=fl, **kw)
+
+
+def mixed_small(nx=10, ny=8, nz=10, box=(24, 24, 24), field=(24, 24), hmax=4, seed=2, **kw) -> MixedConfig:
+    fl = PbfConfig(name=f"mixed_small_{nx}x{ny}x{nz}", lattice=(nx, ny, nz), box=box,
+                   origin=(float((box[0] - nx) // 2), float(hmax + 1), float((box[2] - nz) // 2)),
+                   iterations=3, seed=seed)
+    return MixedConfig(name=fl.name, fluid=fl, field=field, hmax=hmax, field_seed=seed + 100,
+                       phase_seed=seed + 200, **kw)
+
+
+@dataclass
 class BatchConfig:
     """Independent dam-break scenes stepped together."""
     name: str

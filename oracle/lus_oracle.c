@@ -610,6 +610,7 @@ int orc_simulate_fluid(OrcSim* s, float* p, float* ps, float* v, float* lambdas,
 #define SFX(name) name##_f32
 #define RSQRT sqrtf
 #include "pbf_core.inc"
+#include "mixed_core.inc"
 #undef REAL
 #undef SFX
 #undef RSQRT
@@ -618,6 +619,7 @@ int orc_simulate_fluid(OrcSim* s, float* p, float* ps, float* v, float* lambdas,
 #define SFX(name) name##_f64
 #define RSQRT sqrt
 #include "pbf_core.inc"
+#include "mixed_core.inc"
 #undef REAL
 #undef SFX
 #undef RSQRT

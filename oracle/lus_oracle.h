@@ -137,6 +137,21 @@ int orc_pbf_step_f64(const OrcPbfParams* p, int n, double* x, double* v, int* id
  * from x itself (no predict).  Used to judge the learned warm start. */
 double orc_pbf_density_error_f64(const OrcPbfParams* p, int n, const double* x);
 
+/* ---- mixed sand + fluid substep over static solid particles ---- */
+typedef struct OrcMixParams {
+    float diameter;                        /* contact distance d = 2r */
+    float collision, friction, mu_s, mu_k; /* src/Simulate.cpp:159-163 roles */
+} OrcMixParams;
+
+/* One substep; phase[i]: 0 fluid, 1 sand, 2 solid (static).  x, v, id, phase
+ * in/out, returned in cell order; p->iterations Jacobi iterations.
+ * final_err (optional, 2 values): max and mean over fluid particles of
+ * max(0, rho_i/rho0 - 1) at the final positions.  See mixed_core.inc. */
+int orc_mixed_step_f32(const OrcPbfParams* p, const OrcMixParams* mp, int n, float* x, float* v,
+                       int* id, int* phase, float dt, float* final_err);
+int orc_mixed_step_f64(const OrcPbfParams* p, const OrcMixParams* mp, int n, double* x, double* v,
+                       int* id, int* phase, double dt, double* final_err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,11 @@ class OrcPbfParams(C.Structure):
     ]
 
 
+class OrcMixParams(C.Structure):
+    _fields_ = [("diameter", C.c_float), ("collision", C.c_float), ("friction", C.c_float),
+                ("mu_s", C.c_float), ("mu_k", C.c_float)]
+
+
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
@@ -119,6 +124,12 @@ def lib():
         L.orc_pbf_step_f64.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_double,
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_mixed_step_f32.argtypes = [C.POINTER(OrcPbfParams), C.POINTER(OrcMixParams), C.c_int,
+                                         C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.c_float, C.POINTER(C.c_float)]
+        L.orc_mixed_step_f64.argtypes = [C.POINTER(OrcPbfParams), C.POINTER(OrcMixParams), C.c_int,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.c_double, C.POINTER(C.c_double)]
         L.orc_pbf_density_error_f64.restype = C.c_double
         L.orc_pbf_density_error_f64.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_double)]
     return _lib
@@ -288,3 +299,23 @@ def pbf_step(params: OrcPbfParams, x, v, ids, dt, fp64=False, final_err=False):
 def density_error(params, x):
     x = np.ascontiguousarray(x, np.float64)
     return lib().orc_pbf_density_error_f64(C.byref(params), x.shape[0], _dp(x))
+
+
+def mix_params(diameter=1.0, collision=0.8, friction=0.7, mu_s=0.6, mu_k=0.4) -> OrcMixParams:
+    return OrcMixParams(diameter, collision, friction, mu_s, mu_k)
+
+
+def mixed_step(params: OrcPbfParams, mp: OrcMixParams, x, v, ids, phase, dt, fp64=False):
+    """One mixed sand + fluid substep in place (cell order on return); returns
+    (max, mean) over fluid of max(0, rho/rho0 - 1) at the final positions."""
+    n = x.shape[0]
+    assert phase.dtype == np.int32 and ids.dtype == np.int32
+    if fp64:
+        assert x.dtype == np.float64 and v.dtype == np.float64
+        fe = (C.c_double * 2)()
+        lib().orc_mixed_step_f64(C.byref(params), C.byref(mp), n, _dp(x), _dp(v), _ip(ids), _ip(phase), dt, fe)
+    else:
+        assert x.dtype == np.float32 and v.dtype == np.float32
+        fe = (C.c_float * 2)()
+        lib().orc_mixed_step_f32(C.byref(params), C.byref(mp), n, _fp(x), _fp(v), _ip(ids), _ip(phase), dt, fe)
+    return fe[0], fe[1]
