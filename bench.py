@@ -101,13 +101,21 @@ def bench_batch(args, G, scenes) -> int:
     if args.iterations is not None:
         for s in b.scenes:
             s.iterations = args.iterations
+    from lustrine_amd import autotune as T
     params = G.pbf_params(base, exact=args.exact)
     dt = np.float32(base.dt)
+    counts, boxes, pos = b.counts(), b.boxes(), b.positions()
     with G.Gpu(0) as g:
-        g.pbf_batch_init(params, b.counts(), b.boxes(), b.positions())
+        def init(launch):
+            g.pbf_batch_init(T.cell_params(params, None, launch.cell_scale), counts, boxes, pos)
+        tuner = T.Tuner()
+        best, table = (T.Launch(), []) if args.no_autotune else tuner.tune(g, init, b.n, dt)
+        init(best)
+        T.apply(g, best)
         g.pbf_set_graph(not args.no_graph)
         ms, _, _ = g.pbf_bench(dt, args.warmup, args.steps, phases=False)
-        g.pbf_batch_init(params, b.counts(), b.boxes(), b.positions())
+        init(best)
+        T.apply(g, best)
         g.pbf_set_graph(not args.no_graph)
         g.pbf_step(dt, args.warmup)
         lat = g.pbf_step_latency(dt, args.steps)
@@ -131,6 +139,8 @@ def bench_batch(args, G, scenes) -> int:
                    "launch": "plain" if args.no_graph else "hipgraph", "parallelism": "single-gpu"},
         "scene_step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                                   "max": float(lat.max())},
+        "autotune": {"bucket": T.bucket(b.n), "choice": T.asdict(best),
+                     "candidates_ms_per_step": [dict(**T.asdict(c), ms=round(t, 4)) for c, t in table]},
         "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -331,6 +341,7 @@ def main(argv=None):
                     help="oracle operation order (bit-identical results) instead of the fused path")
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--no-phases", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="cfg4: default launch instead of the autotuner")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-plain comparison of the default run")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -160,6 +160,9 @@ int lus_pbf_set_graph(LusGpu* ctx, int on);
  * loads.  Same results bit for bit.  No reference counterpart (launch
  * configuration). */
 int lus_pbf_set_tiling(LusGpu* ctx, int tiled, int zt);
+/* Threads per block (64, 128 or 256; default 256) of the per-slot neighbour
+ * passes.  Same results.  No reference counterpart (launch configuration). */
+int lus_pbf_set_launch(LusGpu* ctx, int block);
 /* Runs `steps` substeps with a hipEvent between steps (between step pairs in
  * graph mode, each step of a pair getting half); ms[k] = step k's time. */
 int lus_pbf_step_latency(LusGpu* ctx, float dt, int steps, float* ms);

@@ -148,6 +148,7 @@ struct PbfState {
     int tgrid = 0;         // blocks of a tiled launch
     int lcap = 0, lcap_x = 0;  // staged float4 per array (XSPH stages two arrays)
     int nred = 0;          // block-reduction entries the last step wrote
+    int fblock = lus::kBlock;  // threads per block of the flat neighbour passes
 };
 
 // the network of the learned step (device copies), kept across scene inits
@@ -312,7 +313,9 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         ta.lcap = P.lcap;
         tx.lcap = P.lcap_x;
     }
-    const int nred = ta.tiles ? P.tgrid : P.nblk;  // block-reduction entries written
+    ta.block = tx.block = P.fblock;
+    // block-reduction entries the passes write
+    const int nred = ta.tiles ? P.tgrid : lus::grid_for(P.n, P.fblock);
     const int K = P.p.iterations;
     const bool learned = P.learned && !rec;
     if (P.mixed) {
@@ -333,11 +336,11 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
                                      P.check ? P.maxd : nullptr, s));
         LUS_HIP(mark(6));
         if (P.check) {
-            LUS_HIP(lus::launch_pbf_decide(P.maxd, P.nblk, std::max(P.n_fluid, 1), P.bound, 0, P.flag, P.counters,
+            LUS_HIP(lus::launch_pbf_decide(P.maxd, lus::grid_for(P.n), std::max(P.n_fluid, 1), P.bound, 0, P.flag, P.counters,
                                            P.hist, LUS_DENSITY_HISTORY, s));
             LUS_HIP(mark(10));
         }
-        P.nred = P.nblk;
+        P.nred = lus::grid_for(P.n);
         std::swap(P.x, P.x2);
         std::swap(P.id, P.id2);
         std::swap(P.ph, P.ph2);
@@ -882,7 +885,7 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
     LUS_HIP(m.alloc(&P.counts, P.C + 1));
     LUS_HIP(m.alloc(&P.cs, P.C + 1));
     LUS_HIP(m.alloc(&P.tiles, lus::scan_tile_count(P.C + 1) + 1));
-    P.nblk = lus::grid_for(n);
+    P.nblk = lus::grid_for(n, 64);  // block-reduction entries of the finest flat launch
     LUS_HIP(m.alloc(&P.maxc, (size_t)P.nmaxc * P.nblk));
     LUS_HIP(m.alloc(&P.maxd, 2 * (size_t)P.nblk));
     LUS_HIP(m.alloc(&P.flag, 1));
@@ -909,10 +912,10 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
         const int avail = maxsh - 2048;  // static shared memory of the tiled kernels
         P.lcap = std::min(2048, avail / 16);
         P.lcap_x = std::min(1536, avail / 32);
-        P.tgrid = std::min(2048, P.nblk);
+        P.tgrid = std::min(2048, lus::grid_for(n));
         if (P.lcap < 256 || P.lcap_x < 256) P.tiled = false;  // too little LDS to stage a tile
     }
-    P.nred = P.nblk;
+    P.nred = lus::grid_for(n, P.fblock);
     std::vector<float4> hx = to_f4(pos, n), hv = to_f4(vel, n);
     std::vector<int> hid((size_t)n);
     for (int i = 0; i < n; ++i) hid[i] = i;
@@ -976,6 +979,17 @@ int lus_pbf_set_graph(LusGpu* c, int on) {
         LUS_HIP(hipStreamSynchronize(c->stream));
         pbf_drop_graph(c->pbf);
     }
+    return LUS_OK;
+}
+
+int lus_pbf_set_launch(LusGpu* c, int block) {
+    if (!c) return set_error(LUS_EINVAL, "lus_pbf_set_launch: NULL context");
+    if (block != 64 && block != 128 && block != 256)
+        return set_error(LUS_EINVAL, "lus_pbf_set_launch: block must be 64, 128 or 256");
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    pbf_drop_graph(c->pbf);
+    c->pbf.fblock = block;
     return LUS_OK;
 }
 
@@ -1096,6 +1110,7 @@ int lus_pbf_time_kernels(LusGpu* c, int reps, float* us) {
     LUS_HIP(hipEventCreate(&e1));
     // the last step's tiles still describe the current cell order
     lus::TileArgs ta, tx;
+    ta.block = tx.block = P.fblock;
     if (P.tiled) {
         ta.tiles = tx.tiles = P.tl;
         ta.ntiles = tx.ntiles = P.toff + lus::tile_chunks(P.k, P.zt);

@@ -8,17 +8,22 @@ namespace lus {
 constexpr int kBlock = 256;
 constexpr int kXcd = 8;
 
-static inline int grid_for(int n) {
-    int b = (n + kBlock - 1) / kBlock;
+// blocks for n threads at `block` threads per block, padded to a multiple of 8
+static inline int grid_for(int n, int block = kBlock) {
+    int b = (n + block - 1) / block;
     if (b < 1) b = 1;
     return ((b + kXcd - 1) / kXcd) * kXcd;
 }
 
+// Global index with an XCD-aware block remap: the hardware deals blocks
+// round-robin to the 8 XCDs, so block b works on tile (b % 8) * (grid / 8) +
+// b / 8 and each XCD walks one contiguous eighth of the array (grid padded to
+// a multiple of 8).
 __device__ __forceinline__ int xcd_index() {
     const int nb = gridDim.x;
     const int b = blockIdx.x;
     const int tile = (b % kXcd) * (nb / kXcd) + b / kXcd;
-    return tile * kBlock + threadIdx.x;
+    return tile * (int)blockDim.x + (int)threadIdx.x;
 }
 
 __device__ __forceinline__ float rclamp(float x, float lo, float hi) {

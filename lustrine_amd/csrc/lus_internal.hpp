@@ -86,6 +86,7 @@ struct TileArgs {
     const int* ntiles = nullptr;  // device tile count
     int grid = 0;                 // blocks of kTileT threads, a multiple of 8
     int lcap = 0;                 // staged float4 per staged array
+    int block = kBlock;           // threads per block of a flat launch (64, 128 or 256)
 };
 int tile_chunks(const PbfK& k, int zt);
 // tcount: tile_chunks + 1 ints (last one 0); toff: the same; scan_tiles:

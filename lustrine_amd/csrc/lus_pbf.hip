@@ -671,7 +671,7 @@ hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, 
                                gate, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
-    const dim3 g(grid_for(n)), b(kBlock);
+    const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (build && k.exact)
         hipLaunchKernelGGL((k_pbf_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
     else if (build)
@@ -797,7 +797,7 @@ hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, 
                                xp, gate, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
-    const dim3 g(grid_for(n)), b(kBlock);
+    const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (k.exact && n4)
         hipLaunchKernelGGL((k_pbf_deltap<false, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
     else if (k.exact)
@@ -933,7 +933,7 @@ hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const 
                                xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
-    const dim3 g(grid_for(n)), b(kBlock);
+    const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (k.exact && maxd)
         hipLaunchKernelGGL((k_pbf_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
     else if (k.exact)
@@ -1325,7 +1325,7 @@ hipError_t launch_pbf_features(int* key2, const int* cs, int n, PbfK k, unsigned
                                const float4* xp, const float4* xold, const float* mu,
                                const float* inv_sigma, void* fbf, float* frec, int rec_stride,
                                int rec_off, const TileArgs& ta, hipStream_t s) {
-    const dim3 g(grid_for(n)), b(kBlock);
+    const dim3 g(grid_for(n, ta.block)), b(ta.block);
     const float inv_h = 1.0f / sqrtf(k.h2);
     if (!frec && ta.tiles) {
         const size_t sh = (size_t)ta.lcap * sizeof(float4);

@@ -91,6 +91,7 @@ _SIGS = {
     "lus_pbf_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
     "lus_pbf_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "lus_pbf_set_launch": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_step_latency": (C.c_int, [C.c_void_p, C.c_float, C.c_int, _f]),
     "lus_pbf_download": (C.c_int, [C.c_void_p, C.c_int, _f, _f, _i]),
     "lus_pbf_last_max_constraint": (C.c_int, [C.c_void_p, _f]),
@@ -339,6 +340,9 @@ class Gpu:
 
     def pbf_set_graph(self, on=True):
         _check(self._lib.lus_pbf_set_graph(self._h, int(bool(on))), "lus_pbf_set_graph")
+
+    def pbf_set_launch(self, block=256):
+        _check(self._lib.lus_pbf_set_launch(self._h, block), "lus_pbf_set_launch")
 
     def pbf_set_tiling(self, tiled=True, zt=16):
         _check(self._lib.lus_pbf_set_tiling(self._h, int(bool(tiled)), zt), "lus_pbf_set_tiling")

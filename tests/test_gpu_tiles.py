@@ -93,3 +93,48 @@ def test_set_tiling_rejects_bad_chunk(gpu_mod):
             g.pbf_set_tiling(True, 0)
         with pytest.raises(gpu_mod.LusError):
             g.pbf_set_tiling(True, 33)
+
+
+@pytest.mark.parametrize("block", [64, 128])
+@pytest.mark.parametrize("exact", [True, False])
+def test_block_sizes_equal_default(gpu_mod, block, exact):
+    cfg = scenes.cfg0()
+    x0 = cfg.positions()
+    ref = _run(gpu_mod, cfg, x0, 4, exact, False)
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_init(gpu_mod.pbf_params(cfg, exact=exact), x0)
+        g.pbf_set_launch(block)
+        g.pbf_set_graph(True)
+        g.pbf_step(np.float32(cfg.dt), 4)
+        x, v, ids = g.pbf_download(order=0)
+        mc = g.pbf_last_max_constraint()
+    assert np.array_equal(ids, ref[2]) and np.array_equal(x, ref[0]) and np.array_equal(v, ref[1])
+    assert mc == ref[3]
+    with gpu_mod.Gpu(0) as g:
+        with pytest.raises(gpu_mod.LusError):
+            g.pbf_set_launch(96)
+
+
+def test_batch_with_larger_cells_matches_oracle(gpu_mod, oracle):
+    from lustrine_amd import autotune as T
+    b = scenes.BatchConfig(name="t", scenes=[scenes.dam_break_of_size(s, 60 + k)
+                                             for k, s in enumerate([700, 2600, 1500])])
+    base = gpu_mod.pbf_params(b.scenes[0], exact=True)
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_batch_init(T.cell_params(base, None, 1.125), b.counts(), b.boxes(), b.positions())
+        g.pbf_set_launch(128)
+        g.pbf_step(np.float32(b.scenes[0].dt), 3)
+        gx, gv, _ = g.pbf_download(order=1)
+    off = 0
+    for s in b.scenes:
+        p = oracle.pbf_params(s)
+        p.cell_size = 2.25
+        p.gridX, p.gridY, p.gridZ = (int(np.float32(d) / np.float32(2.25)) + 1 for d in s.box)
+        x, v = s.positions(), np.zeros((s.n, 3), np.float32)
+        ids = np.arange(s.n, dtype=np.int32)
+        for _ in range(3):
+            oracle.pbf_step(p, x, v, ids, np.float32(s.dt))
+        ox, ov = np.zeros_like(x), np.zeros_like(v)
+        ox[ids], ov[ids] = x, v
+        assert np.array_equal(gx[off:off + s.n], ox) and np.array_equal(gv[off:off + s.n], ov), s.name
+        off += s.n
