@@ -112,7 +112,7 @@ struct PbfState {
            *xp = nullptr;
     int *id = nullptr, *id2 = nullptr, *key = nullptr, *slot = nullptr, *tmp = nullptr,
         *key2 = nullptr, *counts = nullptr, *cs = nullptr, *tiles = nullptr, *maxc = nullptr;
-    unsigned long long* masks = nullptr;  // 9 neighbour-row masks per particle
+    unsigned long long* masks = nullptr;  // neighbour sets: lists (flat) or row masks (tiled)
     int nmaxc = 0;  // iterations with a block-max table
     int nblk = 0;   // blocks of one particle-parallel launch
     bool stepped = false;
@@ -840,6 +840,9 @@ int lus_mix_init(LusGpu* c, const LusPbfParams* p, const LusMixParams* mp, int n
 namespace {
 int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const float* vel,
               const std::vector<lus::SceneBox>* scenes, const std::vector<int>* sid) {
+    // the neighbour passes address particle data with 32-bit byte offsets
+    if (n > lus::kMaxParticles)
+        return set_error(LUS_EINVAL, "PBF scene: %d particles, at most %d", n, lus::kMaxParticles);
     LUS_HIP(hipSetDevice(c->device));
     PbfState& P = c->pbf;
     LUS_HIP(hipStreamSynchronize(c->stream));
@@ -875,7 +878,7 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
     LUS_HIP(m.alloc(&P.xa, n));
     LUS_HIP(m.alloc(&P.xl, n));
     LUS_HIP(m.alloc(&P.xp, n));
-    LUS_HIP(m.alloc(&P.masks, (size_t)9 * n));
+    LUS_HIP(m.alloc(&P.masks, lus::nbr_words(n)));
     LUS_HIP(m.alloc(&P.id, n));
     LUS_HIP(m.alloc(&P.id2, n));
     LUS_HIP(m.alloc(&P.key, n));

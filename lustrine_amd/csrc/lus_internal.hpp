@@ -76,6 +76,22 @@ hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, cons
                             LusGrid g, float h2, const int* inv_perm, const long* off, int* cnt,
                             int* idx, int fill, hipStream_t s);
 
+// Neighbour sets of a step, one buffer of nbr_words(n) 64-bit words.  Flat
+// passes: neighbour lists, cnt[i] (n ints, padded to 16 bytes), then per run
+// of 64 slots (w = i / 64, l = i % 64) kNbrCap / 4 groups of 64 int4, entries
+// 4g .. 4g+3 of slot i in int4 number (w * kNbrCap / 4 + g) * 64 + l; entries
+// in ascending slot order.  A particle with more than kNbrCap neighbours
+// re-tests its candidates in every pass.  Tiled passes: 9 row masks per
+// particle, masks[r * n + i].  A step uses one of the two forms.
+constexpr int kNbrCap = 64;
+constexpr int kNbrGroups = kNbrCap / 4;
+// particles of one PBF context: float4 data of any slot lies within 4 GiB
+constexpr int kMaxParticles = (1 << 28) - 1;
+inline size_t nbr_words(size_t n) {
+    const size_t int4s = (n + 3) / 4 + (n + 63) / 64 * (size_t)kNbrGroups * 64;
+    return 2 * int4s > 9 * n ? 2 * int4s : 9 * n;
+}
+
 // Tiled neighbour passes (lus_pbf.hip): tiles of at most kTileT slots of one
 // (y, x) cell column, built per step from cells in chunks of zt <= kTileZMax
 // along z; each block stages its tile's 9 stencil rows in LDS.

@@ -11,14 +11,18 @@
 //
 // Neighbour sets.  N(i) = { j : |x*_i - x*_j|^2 <= h^2 } from the predicted
 // positions, once per step.  The 27-cell stencil of a particle is 9 rows of
-// contiguous slots (y-major cells, z contiguous), so N(i) is stored as one
-// 64-bit mask per row, masks[r * n + i], bit t = slot j0_r + t.  The first
-// lambda pass tests every candidate once and writes the masks; the other
-// passes of the step load 9 masks and visit only set bits, in ascending slot
-// order (the order of the CPU oracle's lists).  A particle whose stencil has a
-// row longer than 64 slots is flagged in its key (kOvf) and re-tests its
-// candidates against the stored predicted positions in every pass, which
-// gives the same set in the same order.
+// contiguous slots (y-major cells, z contiguous).  The first lambda pass tests
+// every candidate once, row by row through a 64-bit hit mask per row, visits
+// the hits and appends them to i's neighbour list, in ascending slot order
+// (the order of the CPU oracle's lists).  The other passes of the step walk
+// the lists, one entry per iteration in every lane, so a wave runs
+// max_i |N(i)| iterations instead of the sum over the 9 rows of the longest
+// row (about half as many on a dam break).  A particle with more than
+// kNbrCap neighbours re-tests its candidates against the stored predicted
+// positions in every pass, which gives the same set in the same order.  The
+// tiled passes keep the row masks instead (masks[r * n + i], bit t = slot
+// j0_r + t) and flag a particle whose stencil has a row longer than 64 slots
+// in its key (kOvf).
 //
 // Arithmetic modes (template kFast, chosen by LusPbfParams::exact):
 //   exact  every expression in the oracle's order, IEEE '/' and sqrtf, no
@@ -131,45 +135,51 @@ __device__ __forceinline__ void visit_row(M mm, int base, int delta, const float
     f(j, pa, aa);
 }
 
-// Visit N(i) in ascending slot order, calling f(j, cur[j], aux[j]).  build:
-// test the candidates (positions xp) and store the masks; otherwise load them.
-// A row whose mask has no bit above 31 in any lane of the wave is walked with
-// 32-bit mask arithmetic.
-template <bool kBuild, class F>
-__device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict__ key2,
-                                                  const int* __restrict__ cs,
-                                                  unsigned long long* __restrict__ masks,
-                                                  const float4* __restrict__ xp,
-                                                  const float4* __restrict__ cur,
-                                                  const float4* __restrict__ aux, const PbfK& k,
-                                                  F&& f) {
-    const int kw = key2[i];
-    const int key = kw & ~kOvf;
+// float4 of slot j of a particle array, addressed with a 32-bit byte offset
+// from the array base (a context holds at most kMaxParticles < 2^28 slots)
+__device__ __forceinline__ float4 at4(const float4* __restrict__ a, int j) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(a) + (unsigned)j * 16u);
+}
+
+// Neighbour-list buffer (layout in lus_internal.hpp): counts, then int4 groups.
+__device__ __forceinline__ int* nbr_cnt(unsigned long long* nb) { return reinterpret_cast<int*>(nb); }
+// group 0 of slot i's list; group g is 64 int4 further
+__device__ __forceinline__ int4* nbr_list(unsigned long long* nb, int n, int i) {
+    int4* lists = reinterpret_cast<int4*>(nb) + ((size_t)n + 3) / 4;
+    return lists + (size_t)(i >> 6) * (kNbrGroups * 64) + (i & 63);
+}
+
+// Test the 27-cell candidates of slot i (positions xp) row by row, in
+// ascending slot order -- a row of at most 64 slots through a 64-bit hit mask
+// (32-bit mask arithmetic when no lane of the wave has a hit above bit 31), a
+// longer row slot by slot -- and call f(j, cur[j], aux[j]) on every hit while
+// appending it to i's list four at a time (one 16-byte store per group, so
+// the lanes of a wave write whole segments); entries past kNbrCap are
+// counted, not stored.  Writes cnt[i] = |N(i)|.
+template <class F>
+__device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, const int* __restrict__ cs,
+                                           const float4* __restrict__ xp, const PbfK& k, int* __restrict__ cnt,
+                                           int4* __restrict__ lst, const float4* __restrict__ cur,
+                                           const float4* __restrict__ aux, F& f) {
+    const int key = key2[i] & ~kOvf;
     const int cy = key / k.gXZ;
     const int rm = key - cy * k.gXZ;
     const int cx = rm / k.gZ;
     const int cz = rm - cx * k.gZ;
     const int z0 = cz > 0 ? cz - 1 : 0;
     const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
-    bool ovf;
-    if (kBuild) {
-        int maxlen = 0;
-#pragma unroll
-        for (int r = 0; r < 9; ++r) {
-            const int y = cy + r / 3 - 1;
-            const int x = cx + r % 3 - 1;
-            if (y >= 0 && y < k.gY && x >= 0 && x < k.gX) {
-                const int row = y * k.gXZ + x * k.gZ;
-                const int l = cs[row + z1 + 1] - cs[row + z0];
-                maxlen = l > maxlen ? l : maxlen;
-            }
-        }
-        ovf = maxlen > 64;
-        if (ovf) key2[i] = kw | kOvf;
-    } else {
-        ovf = (kw & kOvf) != 0;
-    }
-    const float4 pi = (kBuild || ovf) ? xp[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 pi = xp[i];
+    int c = 0;
+    int4 q = make_int4(i, i, i, i);
+    auto emit = [&](int j, const float4 pj, const float4 aj) {
+        q.x = q.y;
+        q.y = q.z;
+        q.z = q.w;
+        q.w = j;
+        ++c;
+        if ((c & 3) == 0 && c <= kNbrCap) lst[(size_t)((c >> 2) - 1) * 64] = q;
+        f(j, pj, aj);
+    };
 #pragma unroll 1
     for (int r = 0; r < 9; ++r) {
         const int y = cy + r / 3 - 1;
@@ -178,33 +188,122 @@ __device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict_
         const int row = y * k.gXZ + x * k.gZ;
         const int j0 = cs[row + z0];
         const int j1 = cs[row + z1 + 1];
-        if (ovf) {
+        const int L = j1 - j0;
+        if (L > 64) {
             for (int j = j0; j < j1; ++j)
-                if (within_h(pi, xp[j], k)) f(j, cur[j], aux ? aux[j] : make_float4(0.f, 0.f, 0.f, 0.f));
+                if (within_h(pi, xp[j], k)) emit(j, at4(cur, j), aux_at(aux, j));
             continue;
         }
-        unsigned long long mm;
-        if (kBuild) {
-            mm = 0ull;
-            const float4* __restrict__ base = xp + j0;
-            const int L = j1 - j0;
-            int t = 0;
-            for (; t + 4 <= L; t += 4) {
-                const float4 q0 = base[t], q1 = base[t + 1], q2 = base[t + 2], q3 = base[t + 3];
-                mm |= within_h(pi, q0, k) ? (1ull << t) : 0ull;
-                mm |= within_h(pi, q1, k) ? (2ull << t) : 0ull;
-                mm |= within_h(pi, q2, k) ? (4ull << t) : 0ull;
-                mm |= within_h(pi, q3, k) ? (8ull << t) : 0ull;
-            }
-            for (; t < L; ++t) mm |= within_h(pi, base[t], k) ? (1ull << t) : 0ull;
-            masks[(size_t)r * n + i] = mm;
-        } else {
-            mm = masks[(size_t)r * n + i];
+        unsigned long long mm = 0ull;
+        const float4* __restrict__ base = xp + j0;
+        int t = 0;
+        for (; t + 4 <= L; t += 4) {
+            const float4 q0 = base[t], q1 = base[t + 1], q2 = base[t + 2], q3 = base[t + 3];
+            mm |= within_h(pi, q0, k) ? (1ull << t) : 0ull;
+            mm |= within_h(pi, q1, k) ? (2ull << t) : 0ull;
+            mm |= within_h(pi, q2, k) ? (4ull << t) : 0ull;
+            mm |= within_h(pi, q3, k) ? (8ull << t) : 0ull;
         }
+        for (; t < L; ++t) mm |= within_h(pi, base[t], k) ? (1ull << t) : 0ull;
         if (__any((int)((mm >> 32) != 0ull)))
-            visit_row<unsigned long long>(mm, j0, 0, cur, aux, f);
+            visit_row<unsigned long long>(mm, j0, 0, cur, aux, emit);
         else
-            visit_row<unsigned int>((unsigned int)mm, j0, 0, cur, aux, f);
+            visit_row<unsigned int>((unsigned int)mm, j0, 0, cur, aux, emit);
+    }
+    // the last, partial group: entries to the front, slot i in the unused places
+    const int rem = c & 3;
+    if (rem != 0 && c < kNbrCap) {
+        for (int s = rem; s < 4; ++s) {
+            q.x = q.y;
+            q.y = q.z;
+            q.z = q.w;
+            q.w = i;
+        }
+        lst[(size_t)(c >> 2) * 64] = q;
+    }
+    cnt[i] = c;
+}
+
+// Walk a list of c <= kNbrCap entries, calling f(j, cur[j], aux[j]) in list
+// order, four entries (one group) at a time: the group's neighbour data are
+// requested, then the next group's entries (so waiting for the data never
+// waits for the list), then f runs on the four.  Every lane of a wave is in
+// the same group, so a group load is one whole 1 KB segment.  Places of the
+// last group past c hold slot i (in-bounds data, not visited); the next
+// group's entries are used only when the walk goes on.  (Masking the loads
+// of places past c, or requesting the next group's data before f runs on
+// the current one, measured slower.)
+template <class F>
+__device__ __forceinline__ void visit_list(int c, const int4* __restrict__ lst, const float4* __restrict__ cur,
+                                           const float4* __restrict__ aux, F& f) {
+    if (c <= 0) return;
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    int4 e = lst[0];
+    for (int t = 0; t < c; t += 4) {
+        const float4 p0 = at4(cur, e.x), p1 = at4(cur, e.y), p2 = at4(cur, e.z), p3 = at4(cur, e.w);
+        float4 q0 = z, q1 = z, q2 = z, q3 = z;
+        if (aux) {
+            q0 = at4(aux, e.x);
+            q1 = at4(aux, e.y);
+            q2 = at4(aux, e.z);
+            q3 = at4(aux, e.w);
+        }
+        const int g = (t >> 2) + 1 < kNbrGroups ? (t >> 2) + 1 : kNbrGroups - 1;
+        const int4 en = lst[(size_t)g * 64];
+        // keep every component of the first neighbour in the group's loads
+        // (a component used only under f's branch would be loaded there,
+        // after the next group's entries, and wait for them)
+        asm volatile("" ::"v"(p0.x), "v"(p0.y), "v"(p0.z), "v"(p0.w));
+        f(e.x, p0, q0);
+        if (t + 1 < c) f(e.y, p1, q1);
+        if (t + 2 < c) f(e.z, p2, q2);
+        if (t + 3 < c) f(e.w, p3, q3);
+        e = en;
+    }
+}
+
+// Visit N(i) in ascending slot order, calling f(j, cur[j], aux[j]).  build:
+// test the candidates, visiting and listing the hits (build_list); otherwise
+// walk the stored list.  A particle whose list overflowed re-tests its candidates
+// against the stored predicted positions xp (the same set in the same order).
+// nb: the neighbour-list buffer.
+template <bool kBuild, class F>
+__device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict__ key2,
+                                                  const int* __restrict__ cs,
+                                                  unsigned long long* __restrict__ nb,
+                                                  const float4* __restrict__ xp,
+                                                  const float4* __restrict__ cur,
+                                                  const float4* __restrict__ aux, const PbfK& k,
+                                                  F&& f) {
+    int* __restrict__ cnt = nbr_cnt(nb);
+    int4* __restrict__ lst = nbr_list(nb, n, i);
+    if (kBuild) {
+        build_list(i, key2, cs, xp, k, cnt, lst, cur, aux, f);
+        return;
+    }
+    const int c = cnt[i];
+    if (c <= kNbrCap) {
+        visit_list(c, lst, cur, aux, f);
+        return;
+    }
+    const int key = key2[i] & ~kOvf;
+    const int cy = key / k.gXZ;
+    const int rm = key - cy * k.gXZ;
+    const int cx = rm / k.gZ;
+    const int cz = rm - cx * k.gZ;
+    const int z0 = cz > 0 ? cz - 1 : 0;
+    const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
+    const float4 pi = xp[i];
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        const int y = cy + r / 3 - 1;
+        const int x = cx + r % 3 - 1;
+        if (y < 0 || y >= k.gY || x < 0 || x >= k.gX) continue;
+        const int row = y * k.gXZ + x * k.gZ;
+        const int j0 = cs[row + z0];
+        const int j1 = cs[row + z1 + 1];
+        for (int j = j0; j < j1; ++j)
+            if (within_h(pi, xp[j], k)) f(j, cur[j], aux_at(aux, j));
     }
 }
 

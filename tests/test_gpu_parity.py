@@ -224,6 +224,29 @@ def test_pbf_dense_cluster_rows_over_64(gpu_mod, oracle):
     print(f"dense ulp(x)={max_ulp(gx, x)} ulp(v)={max_ulp(gv, v)}")
 
 
+def test_pbf_neighbour_list_overflow_mixed(gpu_mod, oracle):
+    # a block at rest spacing (about 30 neighbours, stored lists) around a core
+    # at spacing 0.6 (about 150 neighbours, more than the 64-entry list: the
+    # re-test path), so waves hold both kinds; exact mode, bit for bit
+    outer = scenes.jittered_lattice(16, 12, 16, cell=1.0, origin=(4.0, 4.0, 4.0), amplitude=0.05, seed=11)
+    c = np.array([12.0, 10.0, 12.0], np.float32)
+    keep = np.abs(outer - c).max(axis=1) > 2.6
+    core = scenes.jittered_lattice(8, 8, 8, cell=0.6, origin=tuple(c - 2.1), amplitude=0.02, seed=12)
+    x0 = np.concatenate([outer[keep], core]).astype(np.float32)
+    cfg = scenes.small(8, 8, 8, box=(28, 24, 28))
+    p = oracle.pbf_params(cfg)
+    x, v = x0.copy(), np.zeros_like(x0)
+    ids = np.arange(x0.shape[0], dtype=np.int32)
+    for _ in range(2):
+        oracle.pbf_step(p, x, v, ids, np.float32(cfg.dt))
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_init(gpu_mod.pbf_params(cfg), x0, None)
+        g.pbf_step(np.float32(cfg.dt), 2)
+        gx, gv, gid = g.pbf_download(order=0)
+    assert np.array_equal(gid, ids)
+    assert np.array_equal(gx, x) and np.array_equal(gv, v)
+
+
 def test_pbf_original_order_download(gpu_mod):
     cfg = scenes.small(10, 10, 10, box=(20, 20, 20))
     x0 = cfg.positions()
