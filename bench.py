@@ -196,13 +196,13 @@ def equal_error_speedup(plain, learned_row, key="err_mean"):
     return learned_row["particle_steps_per_s"] / best["particle_steps_per_s"], best["iterations"]
 
 
-def learned_vs_plain(G, cfg, x0, dt, warmup, steps, net, sweep=(4, 8, 12, 16)):
+def learned_vs_plain(G, cfg, x0, dt, warmup, steps, net, sweep=(4, 8, 12, 16), graph=True):
     """The compact comparison of the default run: plain K over `sweep` and the
     learned step with one plain iteration, same window, same error measure."""
     with G.Gpu(0) as g:
         g.pbf_set_network(net)
-        plain = [measure_mode(g, G, cfg, x0, dt, warmup, steps, k) for k in sweep]
-        lrn = measure_mode(g, G, cfg, x0, dt, warmup, steps, cfg.iterations, learned_iters=1)
+        plain = [measure_mode(g, G, cfg, x0, dt, warmup, steps, k, graph=graph) for k in sweep]
+        lrn = measure_mode(g, G, cfg, x0, dt, warmup, steps, cfg.iterations, learned_iters=1, graph=graph)
     out = {"learned": lrn, "plain_sweep": plain}
     for key in ("err_mean", "err_max", "avg_compression_mean"):
         sp, k = equal_error_speedup(plain, lrn, key)
@@ -442,7 +442,8 @@ def main(argv=None):
         from lustrine_amd import learned as L
         if os.path.exists(L.DEFAULT_NETWORK):
             net, _ = L.load(L.DEFAULT_NETWORK)
-            out["learned_vs_plain"] = learned_vs_plain(G, cfg, x0, dt, args.warmup, args.steps, net)
+            out["learned_vs_plain"] = learned_vs_plain(G, cfg, x0, dt, args.warmup, args.steps, net,
+                                                   graph=not args.no_graph)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(cfg, args.cpu_seconds)
         out["cpu_baseline"] = cb
