@@ -83,7 +83,7 @@ hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, cons
 // in ascending slot order.  A particle with more than kNbrCap neighbours
 // re-tests its candidates in every pass.  Tiled passes: 9 row masks per
 // particle, masks[r * n + i].  A step uses one of the two forms.
-constexpr int kNbrCap = 64;
+constexpr int kNbrCap = 128;  // about 4x the rest-state count (a K = 1 solve compresses 2.5x)
 constexpr int kNbrGroups = kNbrCap / 4;
 // particles of one PBF context: float4 data of any slot lies within 4 GiB
 constexpr int kMaxParticles = (1 << 28) - 1;
@@ -123,7 +123,8 @@ hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_st
 // x2 only (old positions in cell order), as launch_pbf_permute writes it
 hipError_t launch_pbf_restore_old(const int* tmp, const int* key, const int* cell_start,
                                   const float4* x, int n, float4* x2, const int* gate, hipStream_t s);
-// masks: 9 * n neighbour-row masks; build = 1 on the first lambda pass of a step
+// masks: the neighbour-set buffer (nbr_words(n) words); build = 1 on the first
+// lambda pass of a step
 hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
                              int* maxc, unsigned long long* masks, const float4* xp, int build,
                              const int* gate, const TileArgs& ta, hipStream_t s);

@@ -226,7 +226,7 @@ def test_pbf_dense_cluster_rows_over_64(gpu_mod, oracle):
 
 def test_pbf_neighbour_list_overflow_mixed(gpu_mod, oracle):
     # a block at rest spacing (about 30 neighbours, stored lists) around a core
-    # at spacing 0.6 (about 150 neighbours, more than the 64-entry list: the
+    # at spacing 0.6 (about 150 neighbours, more than the 128-entry list: the
     # re-test path), so waves hold both kinds; exact mode, bit for bit
     outer = scenes.jittered_lattice(16, 12, 16, cell=1.0, origin=(4.0, 4.0, 4.0), amplitude=0.05, seed=11)
     c = np.array([12.0, 10.0, 12.0], np.float32)
