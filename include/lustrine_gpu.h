@@ -295,6 +295,32 @@ int lus_sand_download(LusGpu* ctx, float* pos, float* pos_star, float* vel, int*
  * src/neighbors/Neighbors.cpp:262-364), reference slot indices. */
 int lus_sand_neighbors(LusGpu* ctx, long* off, int* idx, long cap, long* total);
 
+/* --- consumers of the grid state, served from the device cell tables --- */
+/* query_cell_num_particles (src/Lustrine.cpp:864-914): the number of ids in
+ * the cell lists of the last step's grid (uniform_gird_cells: each cell's
+ * static solids, then its sand slots) over the cells from
+ * get_cell_indices(min_pos) to get_cell_indices(max_pos) inclusive
+ * (src/neighbors/Utils.hpp:13-21: clamp to [cs/2, domain - cs/2], divide,
+ * truncate); include_solid == 0 counts only ids below the current sand count
+ * (Simulation::ptr_sand_end).  The lists are the last step's, as in the
+ * reference (a later sink eviction does not rebuild them); 0 before the first
+ * step.  Only the count crosses to the host. */
+int lus_sand_query_cells(LusGpu* ctx, const float* min_pos, const float* max_pos, int include_solid,
+                         int* count);
+/* Sink eviction (src/Lustrine.cpp:806-836): `cells` is the concatenation of
+ * sink_cells[i] of every sink that fires this frame (state set and timer at
+ * or above its frequency), in sink order.  Every sand slot listed by those
+ * cells in the last step's grid is removed in list order as the reference
+ * does it -- positions, positions_star, velocities and attracted of the last
+ * sand slot move into the removed slot and the sand count shrinks by one --
+ * so repeated cells and slots behave as there.  *n_sand (nullable) receives
+ * the new sand count (the reference's ptr_sand_end); the caller keeps its
+ * counters (num_sand_particles, num_remaining_sand_particles) and the sink
+ * timers.  Only the listed cells' two cell starts cross to the host.  A cell
+ * id outside the grid is LUS_EINVAL; before the first step nothing is
+ * removed. */
+int lus_sand_sink_evict(LusGpu* ctx, const int* cells, int ncells, int* n_sand);
+
 #ifdef __cplusplus
 }
 #endif

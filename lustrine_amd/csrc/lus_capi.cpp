@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <unordered_map>
 #include <vector>
 
 #include "lus_device.hpp"
@@ -185,6 +186,7 @@ struct SandState {
     long nbr_cap = 0;
     Arena nbr_mem;
     float4* final_iter = nullptr;
+    int* qbuf = nullptr;  // one int: cell-box count
     bool stepped = false;
 };
 
@@ -1658,6 +1660,7 @@ int lus_sand_init(LusGpu* c, const LusSandParams* p, int n_sand, const float* po
     LUS_HIP(m.alloc(&S.nbr_cnt, n + 1));
     LUS_HIP(m.alloc(&S.nbr_cnt1, n));
     LUS_HIP(m.alloc(&S.ntiles, lus::scan_tile_count(n + 1) + 1));
+    LUS_HIP(m.alloc(&S.qbuf, 1));
     std::vector<float4> hx = to_f4(pos, n), hv = to_f4(vel, n);
     std::vector<int> ha((size_t)n, 0);
     if (attracted) memcpy(ha.data(), attracted, sizeof(int) * n);
@@ -1767,6 +1770,106 @@ int lus_sand_download(LusGpu* c, float* pos, float* pos_star, float* vel, int* a
         if (!S.stepped) return set_error(LUS_ESTATE, "lus_sand_download: no step yet");
         LUS_HIP(hipMemcpy(perm, S.perm, sizeof(int) * n, hipMemcpyDeviceToHost));
     }
+    return LUS_OK;
+}
+
+// get_cell_indices (src/neighbors/Utils.hpp:13-21): clamp to
+// [cs/2, domain - cs/2], divide by the cell size, truncate
+static void sand_cell_indices(const LusSandParams& p, const float* pos, int* out) {
+    const float cs = p.grid.cell_size;
+    const float half = cs * 0.5f;
+    for (int a = 0; a < 3; ++a) {
+        const float hi = p.domain[a] - half;
+        float q = pos[a] < half ? half : pos[a];
+        q = q > hi ? hi : q;
+        volatile float d = q / cs;
+        out[a] = (int)d;
+    }
+}
+
+int lus_sand_query_cells(LusGpu* c, const float* min_pos, const float* max_pos, int include_solid, int* count) {
+    if (!c || !c->sand.ready) return set_error(LUS_ESTATE, "lus_sand_query_cells: no sand scene");
+    if (!min_pos || !max_pos || !count) return set_error(LUS_EINVAL, "lus_sand_query_cells: bad arguments");
+    SandState& S = c->sand;
+    *count = 0;
+    // before the first step the reference's cell lists are empty
+    if (!S.stepped) return LUS_OK;
+    int lo[3], hi[3];
+    sand_cell_indices(S.p, min_pos, lo);
+    sand_cell_indices(S.p, max_pos, hi);
+    if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) return LUS_OK;
+    LUS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    LUS_HIP(hipMemsetAsync(S.qbuf, 0, sizeof(int), s));
+    LUS_HIP(lus::launch_cell_box_count(S.cs, S.ocs, S.solid_ref, lo, hi, S.k.gX, S.k.gZ, S.n,
+                                       include_solid ? 1 : 0, S.qbuf, s));
+    LUS_HIP(hipMemcpyAsync(count, S.qbuf, sizeof(int), hipMemcpyDeviceToHost, s));
+    LUS_HIP(hipStreamSynchronize(s));
+    return LUS_OK;
+}
+
+int lus_sand_sink_evict(LusGpu* c, const int* cells, int ncells, int* n_sand) {
+    if (!c || !c->sand.ready) return set_error(LUS_ESTATE, "lus_sand_sink_evict: no sand scene");
+    if (ncells < 0 || (ncells > 0 && !cells)) return set_error(LUS_EINVAL, "lus_sand_sink_evict: bad arguments");
+    SandState& S = c->sand;
+    for (int k = 0; k < ncells; ++k)
+        if (cells[k] < 0 || cells[k] >= S.C)
+            return set_error(LUS_EINVAL, "lus_sand_sink_evict: cell %d (entry %d) outside the grid", cells[k], k);
+    if (S.stepped && ncells > 0) {
+        LUS_HIP(hipSetDevice(c->device));
+        hipStream_t s = c->stream;
+        Arena tmp;
+        int* d_cells;
+        int2* d_rng;
+        LUS_HIP(tmp.alloc(&d_cells, ncells));
+        LUS_HIP(tmp.alloc(&d_rng, ncells));
+        LUS_HIP(hipMemcpyAsync(d_cells, cells, sizeof(int) * ncells, hipMemcpyHostToDevice, s));
+        LUS_HIP(lus::launch_cell_ranges(S.cs, d_cells, ncells, d_rng, s));
+        std::vector<int2> rng((size_t)ncells);
+        LUS_HIP(hipMemcpyAsync(rng.data(), d_rng, sizeof(int2) * ncells, hipMemcpyDeviceToHost, s));
+        LUS_HIP(hipStreamSynchronize(s));
+        // temp_removal: the sand slots of the listed cells in list order (each
+        // cell's static solids come first in the reference's lists; their
+        // slots lie above the sand range there, so none is taken)
+        std::vector<int> rem;
+        for (int k = 0; k < ncells; ++k)
+            for (int q = rng[k].x; q < rng[k].y; ++q)
+                if (q < S.n) rem.push_back(q);
+        // the reference's removal, one after another: slot e takes the data of
+        // the last sand slot and the range shrinks by one.  Replayed on slot
+        // indices; only the final content of each touched slot is moved.
+        std::unordered_map<int, int> src;
+        auto origin = [&](int q) {
+            auto it = src.find(q);
+            return it == src.end() ? q : it->second;
+        };
+        int end = S.n;
+        for (int e : rem) {
+            if (end == 0) break;
+            const int from = origin(end - 1);
+            src[e] = from;
+            --end;
+        }
+        std::vector<int2> mv;
+        for (const auto& kv : src)
+            if (kv.first < end && kv.second != kv.first) mv.push_back(make_int2(kv.first, kv.second));
+        const int m = (int)mv.size();
+        if (m > 0) {
+            int2* d_mv;
+            float4 *t0, *t1, *t2;
+            int* t3;
+            LUS_HIP(tmp.alloc(&d_mv, m));
+            LUS_HIP(tmp.alloc(&t0, m));
+            LUS_HIP(tmp.alloc(&t1, m));
+            LUS_HIP(tmp.alloc(&t2, m));
+            LUS_HIP(tmp.alloc(&t3, m));
+            LUS_HIP(hipMemcpyAsync(d_mv, mv.data(), sizeof(int2) * m, hipMemcpyHostToDevice, s));
+            LUS_HIP(lus::launch_slot_moves(d_mv, m, S.x, S.final_iter, S.v, S.att, t0, t1, t2, t3, s));
+            LUS_HIP(hipStreamSynchronize(s));
+        }
+        S.n = end;
+    }
+    if (n_sand) *n_sand = S.n;
     return LUS_OK;
 }
 

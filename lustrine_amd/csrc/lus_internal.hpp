@@ -223,4 +223,17 @@ hipError_t launch_sand_substep(const float4* A, const float4* xold, const float4
 hipError_t launch_sand_vupdate(const float4* A, float4* xold_to_new, float4* vout, int n, float dt,
                                hipStream_t s);
 
+// grid-state consumers (lus_kernels.hip).  Cell box lo..hi (x, y, z; inclusive)
+// of a grid with cell starts cs (sand slots) and ocs (static solids, reference
+// slots solid_ref); *out += the listed ids (all_ids) or those below sand_end.
+// *out must be zeroed first; the grid is gX x . x gZ, y-major.
+hipError_t launch_cell_box_count(const int* cs, const int* ocs, const int* solid_ref, const int* lo,
+                                 const int* hi, int gX, int gZ, int sand_end, int all_ids, int* out,
+                                 hipStream_t s);
+// out[k] = (cs[cells[k]], cs[cells[k] + 1]); cells must be valid cell ids
+hipError_t launch_cell_ranges(const int* cs, const int* cells, int n, int2* out, hipStream_t s);
+// a[mv[k].x] = a_old[mv[k].y] for the four arrays; t*: m staging entries each
+hipError_t launch_slot_moves(const int2* mv, int m, float4* a0, float4* a1, float4* a2, int* a3,
+                             float4* t0, float4* t1, float4* t2, int* t3, hipStream_t s);
+
 }  // namespace lus

@@ -620,4 +620,108 @@ hipError_t launch_sand_vupdate(const float4* A, float4* xold_to_new, float4* vou
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ grid-state consumers
+// Particles listed by the cells of a cell box (query_cell_num_particles,
+// src/Lustrine.cpp:864-914).  A cell's list is its static solids, then its
+// sand slots; with y-major cells the z run of one (y, x) row is contiguous in
+// both tables, so one thread counts a whole row from four cell starts.
+// all_ids: every listed id; otherwise only ids below sand_end (sand slots
+// below it, solids whose reference slot is below it).
+__global__ __launch_bounds__(kBlock) void k_cell_box_count(const int* __restrict__ cs,
+                                                           const int* __restrict__ ocs,
+                                                           const int* __restrict__ solid_ref, int4 lo,
+                                                           int4 hi, int gX, int gZ, int sand_end,
+                                                           int all_ids, int* __restrict__ out) {
+    const int nx = hi.x - lo.x + 1;
+    const int rows = (hi.y - lo.y + 1) * nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = 0;
+    if (t < rows) {
+        const int y = lo.y + t / nx, x = lo.x + t % nx;
+        const int row = (y * gX + x) * gZ;
+        const int a = cs[row + lo.z], b = cs[row + hi.z + 1];
+        const int sa = ocs[row + lo.z], sb = ocs[row + hi.z + 1];
+        if (all_ids) {
+            c = (b - a) + (sb - sa);
+        } else {
+            const int e = b < sand_end ? b : sand_end;
+            c = e > a ? e - a : 0;
+            for (int q = sa; q < sb; ++q) c += solid_ref[q] < sand_end ? 1 : 0;
+        }
+    }
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
+hipError_t launch_cell_box_count(const int* cs, const int* ocs, const int* solid_ref, const int* lo,
+                                 const int* hi, int gX, int gZ, int sand_end, int all_ids, int* out,
+                                 hipStream_t s) {
+    const int rows = (hi[1] - lo[1] + 1) * (hi[0] - lo[0] + 1);
+    const int4 l = make_int4(lo[0], lo[1], lo[2], 0), h = make_int4(hi[0], hi[1], hi[2], 0);
+    hipLaunchKernelGGL(k_cell_box_count, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cs, ocs,
+                       solid_ref, l, h, gX, gZ, sand_end, all_ids, out);
+    return hipGetLastError();
+}
+
+// out[2k], out[2k+1] = first and end slot of cell cells[k]
+__global__ __launch_bounds__(kBlock) void k_cell_ranges(const int* __restrict__ cs,
+                                                        const int* __restrict__ cells, int n,
+                                                        int2* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = cells[k];
+    out[k] = make_int2(cs[c], cs[c + 1]);
+}
+
+hipError_t launch_cell_ranges(const int* cs, const int* cells, int n, int2* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cell_ranges, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cs, cells, n,
+                       out);
+    return hipGetLastError();
+}
+
+// Slot moves a[dst[k]] = a_old[src[k]] for three float4 arrays and one int
+// array: every source is read into the staging arrays before any slot is
+// written (two launches), so moves may chain and overlap.
+__global__ __launch_bounds__(kBlock) void k_moves_gather(const int2* __restrict__ mv, int m,
+                                                         const float4* __restrict__ a0,
+                                                         const float4* __restrict__ a1,
+                                                         const float4* __restrict__ a2,
+                                                         const int* __restrict__ a3,
+                                                         float4* __restrict__ t0, float4* __restrict__ t1,
+                                                         float4* __restrict__ t2, int* __restrict__ t3) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int src = mv[k].y;
+    t0[k] = a0[src];
+    t1[k] = a1[src];
+    t2[k] = a2[src];
+    t3[k] = a3[src];
+}
+
+__global__ __launch_bounds__(kBlock) void k_moves_scatter(const int2* __restrict__ mv, int m,
+                                                          float4* __restrict__ a0, float4* __restrict__ a1,
+                                                          float4* __restrict__ a2, int* __restrict__ a3,
+                                                          const float4* __restrict__ t0,
+                                                          const float4* __restrict__ t1,
+                                                          const float4* __restrict__ t2,
+                                                          const int* __restrict__ t3) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int dst = mv[k].x;
+    a0[dst] = t0[k];
+    a1[dst] = t1[k];
+    a2[dst] = t2[k];
+    a3[dst] = t3[k];
+}
+
+hipError_t launch_slot_moves(const int2* mv, int m, float4* a0, float4* a1, float4* a2, int* a3,
+                             float4* t0, float4* t1, float4* t2, int* t3, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    const dim3 g((m + kBlock - 1) / kBlock), b(kBlock);
+    hipLaunchKernelGGL(k_moves_gather, g, b, 0, s, mv, m, a0, a1, a2, a3, t0, t1, t2, t3);
+    hipLaunchKernelGGL(k_moves_scatter, g, b, 0, s, mv, m, a0, a1, a2, a3, t0, t1, t2, t3);
+    return hipGetLastError();
+}
+
 }  // namespace lus

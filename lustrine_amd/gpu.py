@@ -114,6 +114,8 @@ _SIGS = {
     "lus_sand_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, _f]),
     "lus_sand_download": (C.c_int, [C.c_void_p, _f, _f, _f, _i, _i]),
     "lus_sand_neighbors": (C.c_int, [C.c_void_p, _l, _i, C.c_long, _l]),
+    "lus_sand_query_cells": (C.c_int, [C.c_void_p, _f, _f, C.c_int, _i]),
+    "lus_sand_sink_evict": (C.c_int, [C.c_void_p, _i, C.c_int, _i]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -462,6 +464,23 @@ class Gpu:
         _check(self._lib.lus_sand_neighbors(self._h, _ptr_l(off), _ptr_i(idx), idx.shape[0], C.byref(total)),
                "lus_sand_neighbors")
         return off, idx[: total.value]
+
+    def sand_query_cells(self, min_pos, max_pos, include_solid):
+        """query_cell_num_particles over the last step's grid."""
+        lo, hi = _f32(min_pos, (3,)), _f32(max_pos, (3,))
+        out = C.c_int(0)
+        _check(self._lib.lus_sand_query_cells(self._h, _ptr_f(lo), _ptr_f(hi), int(bool(include_solid)),
+                                              C.byref(out)), "lus_sand_query_cells")
+        return out.value
+
+    def sand_sink_evict(self, cells):
+        """Sink eviction of the listed cells; returns the new sand count."""
+        c = np.ascontiguousarray(cells, np.int32)
+        out = C.c_int(0)
+        _check(self._lib.lus_sand_sink_evict(self._h, _ptr_i(c) if c.size else None, int(c.size),
+                                             C.byref(out)), "lus_sand_sink_evict")
+        self.n_sand = out.value
+        return out.value
 
 
 class Trainer:

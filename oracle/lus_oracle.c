@@ -324,7 +324,85 @@ static void sand_neighbors_v1(const OrcSim* s, OrcSandState* st, IList* nbrs) {
                 }
     }
     free(keys); free(sorted); free(counts); free(tp); free(tps); free(tv); free(ta);
-    free(sand_off); free(cell_of);
+    free(cell_of);
+    /* the cell lists stay as they are until the next search: the sinks and
+     * the cell queries of Lustrine.cpp read them after the step */
+    free(st->cell_sand_off);
+    st->cell_sand_off = sand_off;
+}
+
+/* get_cell_indices (src/neighbors/Utils.hpp:13-21): clamp to
+ * [cs/2, domain - cs/2] per axis, divide by the cell size, truncate */
+static void cell_indices(const OrcSim* s, const float* pos, int* out) {
+    const float half = s->cell_size * 0.5f;
+    const float hi[3] = {s->domainX - half, s->domainY - half, s->domainZ - half};
+    for (int a = 0; a < 3; a++) {
+        float q = pos[a] < half ? half : pos[a];
+        q = q > hi[a] ? hi[a] : q;
+        q = q / s->cell_size;
+        out[a] = (int)q;
+    }
+}
+
+int orc_query_cell_num_particles(const OrcSim* s, const OrcSandState* st, const float* min_pos,
+                                 const float* max_pos, int include_solid) {
+    if (!st->cell_sand_off) return 0;
+    int lo[3], hi[3];
+    cell_indices(s, min_pos, lo);
+    cell_indices(s, max_pos, hi);
+    int counter = 0;
+    for (int y = lo[1]; y <= hi[1]; y++)
+        for (int x = lo[0]; x <= hi[0]; x++)
+            for (int z = lo[2]; z <= hi[2]; z++) {
+                const int c = y * s->gridX * s->gridZ + x * s->gridZ + z;
+                const long s0 = st->static_off[c], s1 = st->static_off[c + 1];
+                const long d0 = st->cell_sand_off[c], d1 = st->cell_sand_off[c + 1];
+                if (include_solid) {
+                    counter += (int)((s1 - s0) + (d1 - d0));
+                } else {
+                    for (long t = s0; t < s1; t++) counter += st->static_idx[t] < st->ptr_sand_end;
+                    for (long t = d0; t < d1; t++) counter += t < st->ptr_sand_end;
+                }
+            }
+    return counter;
+}
+
+int orc_sink_evict(const OrcSim* s, OrcSandState* st, const int* cells, int ncells) {
+    if (!st->cell_sand_off) return 0;
+    /* temp_removal: the ids below ptr_sand_end of the listed cells, in list
+     * order (static solids first, then sand slots) */
+    long m = 0, cap = 16;
+    int* rem = (int*)malloc(sizeof(int) * (size_t)cap);
+    for (int k = 0; k < ncells; k++) {
+        const int c = cells[k];
+        const long s0 = st->static_off[c], s1 = st->static_off[c + 1];
+        const long d0 = st->cell_sand_off[c], d1 = st->cell_sand_off[c + 1];
+        for (long t = 0; t < (s1 - s0) + (d1 - d0); t++) {
+            const int id = t < s1 - s0 ? st->static_idx[s0 + t] : (int)(d0 + (t - (s1 - s0)));
+            if (id >= st->ptr_sand_end) continue;
+            if (m == cap) {
+                cap *= 2;
+                rem = (int*)realloc(rem, sizeof(int) * (size_t)cap);
+            }
+            rem[m++] = id;
+        }
+    }
+    /* swap-remove in turn with the last sand slot; the reference would read
+     * slot -1 once the sand range is empty, so the removal stops there */
+    int removed = 0;
+    for (long t = 0; t < m && st->ptr_sand_end > 0; t++) {
+        const int e = rem[t], last = st->ptr_sand_end - 1;
+        memmove(st->positions + 3 * (size_t)e, st->positions + 3 * (size_t)last, sizeof(float) * 3);
+        memmove(st->positions_star + 3 * (size_t)e, st->positions_star + 3 * (size_t)last, sizeof(float) * 3);
+        memmove(st->velocities + 3 * (size_t)e, st->velocities + 3 * (size_t)last, sizeof(float) * 3);
+        st->attracted[e] = st->attracted[last];
+        st->ptr_sand_end--;
+        st->num_sand_particles--;
+        removed++;
+    }
+    free(rem);
+    (void)s;
+    return removed;
 }
 
 int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
@@ -506,9 +584,9 @@ int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
 
 void orc_sand_state_free(OrcSandState* st) {
     free(st->static_off); free(st->static_idx); free(st->sorted_indices);
-    free(st->nbr_off); free(st->nbr_idx);
+    free(st->nbr_off); free(st->nbr_idx); free(st->cell_sand_off);
     st->static_off = NULL; st->static_idx = NULL; st->sorted_indices = NULL;
-    st->nbr_off = NULL; st->nbr_idx = NULL; st->nbr_cap = 0;
+    st->nbr_off = NULL; st->nbr_idx = NULL; st->nbr_cap = 0; st->cell_sand_off = NULL;
 }
 
 /* ------------------------------------------------------------------ */

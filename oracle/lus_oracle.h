@@ -95,7 +95,20 @@ typedef struct OrcSandState {
     long* nbr_off;
     int*  nbr_idx;
     long  nbr_cap;
+    /* sand part of every cell's list in the last step's grid
+     * (uniform_gird_cells after the static solids): cell c holds the sand
+     * slots [cell_sand_off[c], cell_sand_off[c+1]); owned, NULL before the
+     * first step */
+    long* cell_sand_off;
 } OrcSandState;
+
+/* query_cell_num_particles (src/Lustrine.cpp:864-914) over the last step's
+ * cell lists; 0 before the first step (the lists are empty then). */
+int orc_query_cell_num_particles(const OrcSim* s, const OrcSandState* st, const float* min_pos,
+                                 const float* max_pos, int include_solid);
+/* Sink eviction (src/Lustrine.cpp:806-836) for the concatenated cell lists
+ * `cells` of the firing sinks.  Returns the number of particles removed. */
+int orc_sink_evict(const OrcSim* s, OrcSandState* st, const int* cells, int ncells);
 
 /* One call of simulate_sand.  Returns 0 on success. */
 int  orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt);

@@ -54,6 +54,7 @@ class OrcSandState(C.Structure):
         ("sorted_indices", C.POINTER(C.c_int)),
         ("nbr_off", C.POINTER(C.c_long)), ("nbr_idx", C.POINTER(C.c_int)),
         ("nbr_cap", C.c_long),
+        ("cell_sand_off", C.POINTER(C.c_long)),
     ]
 
 
@@ -116,6 +117,11 @@ def lib():
                                           C.c_long]
         L.orc_simulate_sand.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState), C.c_float]
         L.orc_sand_state_free.argtypes = [C.POINTER(OrcSandState)]
+        L.orc_query_cell_num_particles.restype = C.c_int
+        L.orc_query_cell_num_particles.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState),
+                                                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]
+        L.orc_sink_evict.restype = C.c_int
+        L.orc_sink_evict.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState), C.POINTER(C.c_int), C.c_int]
         L.orc_simulate_fluid.argtypes = [C.POINTER(OrcSim)] + [C.POINTER(C.c_float)] * 4 + \
             [C.c_int] * 4 + [C.c_float]
         L.orc_pbf_step_f32.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_float),
@@ -238,6 +244,24 @@ class SandStream:
 
     def perm(self):
         return np.ctypeslib.as_array(self.st.sorted_indices, shape=(self.n,)).copy()
+
+    def query_cells(self, min_pos, max_pos, include_solid):
+        """query_cell_num_particles (src/Lustrine.cpp:864-914) on the last step's grid."""
+        lo = np.ascontiguousarray(min_pos, np.float32)
+        hi = np.ascontiguousarray(max_pos, np.float32)
+        return int(lib().orc_query_cell_num_particles(C.byref(self.s), C.byref(self.st), _fp(lo), _fp(hi),
+                                                      int(bool(include_solid))))
+
+    def sink_evict(self, cells):
+        """Sink eviction (src/Lustrine.cpp:806-836) of the listed cells; returns the count."""
+        c = np.ascontiguousarray(cells, np.int32)
+        k = int(lib().orc_sink_evict(C.byref(self.s), C.byref(self.st), _ip(c), int(c.shape[0])))
+        self.n = int(self.st.ptr_sand_end)
+        return k
+
+    def cell_sand_off(self):
+        """The sand part of every cell's list in the last step's grid (CSR)."""
+        return np.ctypeslib.as_array(self.st.cell_sand_off, shape=(self.s.num_grid_cells + 1,)).copy()
 
     def neighbors(self):
         off = np.ctypeslib.as_array(self.st.nbr_off, shape=(self.n + 1,)).copy()

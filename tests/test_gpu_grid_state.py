@@ -1,0 +1,110 @@
+"""Consumers of the sand grid state served from the device cell tables:
+query_cell_num_particles (src/Lustrine.cpp:864-914) and the sink eviction
+(src/Lustrine.cpp:806-836), against the oracle's restatement on the same
+stream.  Counts and moved slots are integers and slot contents: bit for bit."""
+import numpy as np
+import pytest
+
+from lustrine_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+BOX = (20, 20, 20)
+DT = np.float32(0.016)
+
+
+def _streams(gpu_mod, oracle, n_side=12, seed=7):
+    pos = scenes.jittered_lattice(n_side, n_side, n_side, origin=(2.0, 3.0, 2.0), amplitude=0.25, seed=seed)
+    sol = scenes.lattice(BOX[0], 2, BOX[2], origin=(0.0, 0.0, 0.0))
+    s = oracle.init_constants(*BOX, 0.5, 1.0, 3.1)
+    st = oracle.SandStream(s, pos, sol)
+    ns, total = sol.shape[0], st.total
+    solid_index = np.arange(total - 1, total - 1 - ns, -1, dtype=np.int32)  # written downward
+    sp = gpu_mod.sand_params(*BOX, kernel_radius_scale=3.1)
+    sp.ptr_solid_start = total - ns
+    g = gpu_mod.Gpu(0)
+    g.sand_init(sp, pos, None, None, sol, solid_index)
+    return s, st, g
+
+
+def _state(st, n):
+    return st.p[:n].copy(), st.ps[:n].copy(), st.v[:n].copy(), st.att[:n].copy()
+
+
+def _boxes(rng, k):
+    lo = rng.uniform(-3.0, 23.0, (k, 3)).astype(np.float32)
+    hi = lo + rng.uniform(-2.0, 12.0, (k, 3)).astype(np.float32)  # some inverted
+    return lo, hi
+
+
+def test_query_cells_matches_oracle(gpu_mod, oracle):
+    s, st, g = _streams(gpu_mod, oracle)
+    with g:
+        assert g.sand_query_cells([0, 0, 0], BOX, True) == 0  # no step yet: empty lists
+        for _ in range(3):
+            st.step(DT)
+            g.sand_step(DT)
+        rng = np.random.default_rng(3)
+        lo, hi = _boxes(rng, 150)
+        for a, b in zip(lo, hi):
+            for inc in (True, False):
+                assert g.sand_query_cells(a, b, inc) == st.query_cells(a, b, inc), (a, b, inc)
+        n, ns = st.n, st.st.ptr_solid_end - st.st.ptr_solid_start
+        assert g.sand_query_cells([-9, -9, -9], [99, 99, 99], False) == n
+        assert g.sand_query_cells([-9, -9, -9], [99, 99, 99], True) == n + ns
+
+
+def test_sink_evict_matches_oracle(gpu_mod, oracle):
+    s, st, g = _streams(gpu_mod, oracle, seed=11)
+    with g:
+        for _ in range(2):
+            st.step(DT)
+            g.sand_step(DT)
+        # the cells of a box in the block's upper part, one of them listed twice,
+        # and cells with solids only (nothing below the sand end)
+        off = st.cell_sand_off()
+        occupied = np.nonzero(np.diff(off))[0]
+        rng = np.random.default_rng(5)
+        cells = [int(c) for c in rng.choice(occupied, 40, replace=False)]
+        cells = cells + [cells[3], 5]  # a repeated cell; cell (0, 0, 5) of the solid floor
+        n0 = st.n
+        k = st.sink_evict(cells)
+        n1 = g.sand_sink_evict(cells)
+        assert n1 == st.n == n0 - k and k > 40
+        gx, gps, gv, gatt, _ = g.sand_download()
+        ox, ops, ov, oatt = _state(st, n1)
+        assert np.array_equal(gatt, oatt)
+        assert np.array_equal(gx, ox) and np.array_equal(gps, ops) and np.array_equal(gv, ov)
+        # queries read the step's lists with the new sand end
+        lo, hi = _boxes(rng, 40)
+        for a, b in zip(lo, hi):
+            assert g.sand_query_cells(a, b, False) == st.query_cells(a, b, False)
+            assert g.sand_query_cells(a, b, True) == st.query_cells(a, b, True)
+        # the next steps run on the shortened sand range
+        for _ in range(2):
+            st.step(DT)
+            g.sand_step(DT)
+        gx, gps, gv, gatt, gperm = g.sand_download()
+        ox, ops, ov, oatt = _state(st, n1)
+        assert np.array_equal(gperm, st.perm()) and np.array_equal(gatt, oatt)
+        assert np.array_equal(gx, ox) and np.array_equal(gv, ov)
+
+
+def test_sink_evict_edges(gpu_mod, oracle):
+    s, st, g = _streams(gpu_mod, oracle, n_side=6, seed=2)
+    with g:
+        n = st.n
+        assert g.sand_sink_evict([0, 1, 2]) == n  # before the first step: nothing removed
+        st.step(DT)
+        g.sand_step(DT)
+        assert g.sand_sink_evict([]) == n
+        with pytest.raises(gpu_mod.LusError):
+            g.sand_sink_evict([s.num_grid_cells])
+        with pytest.raises(gpu_mod.LusError):
+            g.sand_sink_evict([-1])
+        # every cell twice: the sand range empties, further removals stop
+        allc = np.arange(s.num_grid_cells, dtype=np.int32)
+        assert st.sink_evict(np.concatenate([allc, allc])) == n
+        assert g.sand_sink_evict(np.concatenate([allc, allc])) == 0
+        assert g.sand_query_cells([0, 0, 0], BOX, False) == 0
+        assert g.sand_query_cells([0, 0, 0], BOX, True) == st.query_cells([0, 0, 0], BOX, True)

@@ -349,3 +349,105 @@ def test_philox_known_answers():
     pi = scenes.philox4x32_10(np.array([[0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344]], np.uint32),
                               (0xA4093822, 0x299F31D0))[0]
     assert [int(t) for t in pi] == [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+# ------------------------------------------------------------------ grid-state consumers
+def _placed_scene(oracle, seed=0):
+    """Sand particles placed in cells with even coordinates (one empty cell
+    between occupied ones, up to 4 per cell at offsets +-0.5 along x and z):
+    with no gravity and no velocity nothing moves, so the cell of every
+    particle at sort time is the cell it was placed in."""
+    s = oracle.init_constants(16, 16, 16, 0.5, 1.0, 4.0)  # cell 2, grid 9^3
+    s.gravity[:] = [0.0, 0.0, 0.0]
+    rng = np.random.default_rng(seed)
+    offs = np.array([[-0.5, 0, -0.5], [0.5, 0, -0.5], [-0.5, 0, 0.5], [0.5, 0, 0.5]], np.float32)
+    pos, per_cell = [], {}
+    for cx in (0, 2, 4, 6):
+        for cy in (0, 2, 4, 6):
+            for cz in (0, 2, 4, 6):
+                k = int(rng.integers(0, 5))
+                if k == 0:
+                    continue
+                ctr = np.array([2 * cx + 1, 2 * cy + 1, 2 * cz + 1], np.float32)
+                pos += [ctr + o for o in offs[:k]]
+                per_cell[(cx, cy, cz)] = k
+    return s, np.array(pos, np.float32), per_cell
+
+
+def test_query_cell_num_particles_closed_form(oracle):
+    s, pos, per_cell = _placed_scene(oracle, seed=1)
+    # two solids in occupied-free cells with odd coordinates
+    sol = np.array([[3.0, 3.0, 3.0], [3.2, 3.0, 3.6]], np.float32)
+    st = oracle.SandStream(s, pos, sol)
+    assert st.query_cells([0, 0, 0], [16, 16, 16], True) == 0  # lists empty before a step
+    st.step(np.float32(0.01))
+    n, ns = pos.shape[0], sol.shape[0]
+    assert np.array_equal(st.p[:n][np.argsort(st.perm())], pos)  # nothing moved
+    cs = 2.0
+    for (cx, cy, cz), k in list(per_cell.items())[:40]:
+        ctr = [2 * cx + 1, 2 * cy + 1, 2 * cz + 1]
+        assert st.query_cells(ctr, ctr, True) == k
+        assert st.query_cells(ctr, ctr, False) == k
+    # the solids' cell (1, 1, 1): counted only with include_solid
+    assert st.query_cells([3, 3, 3], [3, 3, 3], True) == ns
+    assert st.query_cells([3, 3, 3], [3, 3, 3], False) == 0
+    # whole domain, and a box beyond it (clamped to [cs/2, 16 - cs/2])
+    assert st.query_cells([0, 0, 0], [16, 16, 16], True) == n + ns
+    assert st.query_cells([-50, -50, -50], [99, 99, 99], False) == n
+    # an index box: cells 0..4 along x and z, 2..6 along y (clamped ends)
+    want = sum(k for (cx, cy, cz), k in per_cell.items() if cx <= 4 and 2 <= cy <= 6 and cz <= 4)
+    assert st.query_cells([-1, 2 * cs - 0.01 + 0.02, 0], [9.9, 13.9, 9.0], False) == want
+    # min above max on one axis: empty loops
+    assert st.query_cells([10, 0, 0], [2, 16, 16], True) == 0
+    # additivity over a split of y at a cell face
+    a = st.query_cells([0, 0, 0], [16, 7.9, 16], True)
+    b = st.query_cells([0, 8.0, 0], [16, 16, 16], True)
+    assert a + b == n + ns
+
+
+def _one_per_cell(oracle, m=12):
+    """m sand particles, one per cell along x (cells (k, 0, 0), 2 apart, out
+    of contact): their slots after the sort are their ranks along x."""
+    s = oracle.init_constants(40, 16, 16, 0.5, 1.0, 4.0)  # cell 2, gridX 21
+    s.gravity[:] = [0.0, 0.0, 0.0]
+    pos = np.array([[2.0 * k + 1.0, 1.0, 1.0] for k in range(m)], np.float32)
+    st = oracle.SandStream(s, pos[::-1].copy(), None)  # reversed: the sort reorders them
+    st.step(np.float32(0.01))
+    assert np.array_equal(st.p[:m], pos)
+    cell = [k * s.gridZ for k in range(m)]  # cell of slot k: x = k, y = z = 0
+    # distinct per-slot velocities and flags to follow the moves
+    st.v[:m] = np.arange(3 * m, dtype=np.float32).reshape(m, 3)
+    st.att[:m] = np.arange(m) + 100
+    return s, st, pos, cell
+
+
+def test_sink_evict_swap_remove_order(oracle):
+    # removal list [5, 9] with 12 sand slots: slot 5 takes slot 11, slot 9 takes slot 10
+    _, st, pos, cell = _one_per_cell(oracle)
+    v0, a0 = st.v[:12].copy(), st.att[:12].copy()
+    assert st.sink_evict([cell[5], cell[9]]) == 2
+    assert st.n == 10
+    order = [0, 1, 2, 3, 4, 11, 6, 7, 8, 10]
+    assert np.array_equal(st.p[:10], pos[order])
+    assert np.array_equal(st.v[:10], v0[order]) and np.array_equal(st.att[:10], a0[order])
+
+
+def test_sink_evict_reference_quirks(oracle):
+    # [5, 11]: slot 5 takes slot 11, then "slot 11" (already past the end)
+    # takes slot 10 -- the particle of slot 10 is the one lost, 11 survives in 5
+    _, st, pos, cell = _one_per_cell(oracle)
+    assert st.sink_evict([cell[5], cell[11]]) == 2
+    assert np.array_equal(st.p[:10], pos[[0, 1, 2, 3, 4, 11, 6, 7, 8, 9]])
+    # a cell listed twice: slot 3 is removed twice (it takes 11, then 10)
+    _, st, pos, cell = _one_per_cell(oracle)
+    assert st.sink_evict([cell[3], cell[3]]) == 2
+    assert np.array_equal(st.p[:10], pos[[0, 1, 2, 10, 4, 5, 6, 7, 8, 9]])
+    # every cell: the sand range empties; further removals stop there
+    _, st, pos, cell = _one_per_cell(oracle)
+    assert st.sink_evict(cell + cell) == 12 and st.n == 0
+    # the lists are the step's: the count below the new end shrinks, the
+    # solid-inclusive count does not
+    _, st, pos, cell = _one_per_cell(oracle)
+    st.sink_evict([cell[0]])
+    assert st.query_cells([0, 0, 0], [40, 16, 16], False) == 11
+    assert st.query_cells([0, 0, 0], [40, 16, 16], True) == 12
