@@ -18,9 +18,14 @@
  * simulate_pbf() is the position-based-fluids substep (no reference
  * counterpart with iterations and XSPH); particle order on the host is kept.
  *
+ * query_cell_num_particles() and evict_sinks() serve the two host readers of
+ * uniform_gird_cells (src/Lustrine.cpp:864-914 and :806-836) from the device
+ * cell table, which the host lists are not filled from on this path.
+ *
  * Device state persists per Simulation between calls.  It is uploaded again
- * when the sand range changes (sources and sinks move ptr_sand_end,
- * src/Lustrine.cpp:780, 831) or after mark_dirty().  Errors: the functions
+ * when the sand range changes behind the adapter's back (sources move
+ * ptr_sand_end, src/Lustrine.cpp:780; sinks run through evict_sinks() keep
+ * it in step) or after mark_dirty().  Errors: the functions
  * return the LUS_E* code (0 = ok) and leave the host state untouched on
  * failure; the message is in lus_gpu_last_error().  Not thread-safe, like
  * the reference (one host thread per Simulation).
@@ -183,6 +188,53 @@ int simulate_pbf(Sim* s, float dt, int iterations = 4, int exact = 0, int device
     if (rc) return rc;
     std::memcpy(fptr(s->positions_star), fptr(s->positions), sizeof(float) * 3 * (size_t)n);
     s->time_step = dt;
+    return LUS_OK;
+}
+
+/* query_cell_num_particles (src/Lustrine.cpp:864-914) for a Simulation
+ * stepped by simulate_sand() above, answered from the device cell table of
+ * the last step (the host uniform_gird_cells is not filled on this path).
+ * *count receives the reference's return value. */
+template <class Sim, class V>
+int query_cell_num_particles(Sim* s, const V& min_pos, const V& max_pos, bool include_solid, int* count) {
+    auto it = states().find(s);
+    if (it == states().end() || it->second.kind != 1) return LUS_ESTATE;
+    const float lo[3] = {min_pos.x, min_pos.y, min_pos.z};
+    const float hi[3] = {max_pos.x, max_pos.y, max_pos.z};
+    return lus_sand_query_cells(it->second.ctx, lo, hi, include_solid ? 1 : 0, count);
+}
+
+/* The sink loop of Lustrine::simulate (src/Lustrine.cpp:806-836) after
+ * simulate_sand(): the cells of every firing sink (state set, timer at or
+ * above its frequency; timers are advanced by the caller as there) are
+ * evicted on the device in the reference's order, then the sand range of the
+ * host arrays and the counters ptr_sand_end, num_sand_particles and
+ * num_remaining_sand_particles are brought up to date.  Slots past the new
+ * sand end are not written on the host. */
+template <class Sim>
+int evict_sinks(Sim* s) {
+    auto it = states().find(s);
+    if (it == states().end() || it->second.kind != 1) return LUS_ESTATE;
+    State& st = it->second;
+    std::vector<int> cells;
+    for (int i = 0; i < s->sink->num_sinks; i++)
+        if (s->sink->state[i] && s->sink->timers[i] >= s->sink->frequencies[i])
+            cells.insert(cells.end(), s->sink->sink_cells[i].begin(), s->sink->sink_cells[i].end());
+    if (cells.empty()) return LUS_OK;
+    int n = st.n_sand;
+    int rc = lus_sand_sink_evict(st.ctx, cells.data(), (int)cells.size(), &n);
+    if (rc) return rc;
+    const int removed = st.n_sand - n;
+    if (removed == 0) return LUS_OK;
+    if (n > 0) {
+        rc = lus_sand_download(st.ctx, fptr(s->positions), fptr(s->positions_star), fptr(s->velocities),
+                               s->attracted, nullptr);
+        if (rc) return rc;
+    }
+    s->ptr_sand_end -= removed;
+    s->num_sand_particles -= removed;
+    s->num_remaining_sand_particles += removed;
+    st.n_sand = n;
     return LUS_OK;
 }
 

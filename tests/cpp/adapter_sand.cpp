@@ -1,11 +1,17 @@
 // Drives lus_adapter::simulate_sand / simulate_pbf through a stand-in with the
 // member names and layout of Lustrine::Simulation (src/Simulation.hpp:135-251)
 // that the adapter reads.  Input/output: raw little-endian files.
-//   adapter_sand.exe sand <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
-//   adapter_sand.exe pbf  <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
+//   adapter_sand.exe sand     <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
+//   adapter_sand.exe pbf      <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
+//   adapter_sand.exe sandsink <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
 // in:  int32 n_sand, int32 n_solid, int32 total, float32[3*n_sand] sand,
-//      float32[3*n_solid] solids (written downward from slot total-1)
-// out: float32[3*n] positions, float32[3*n] velocities, int32[n] attracted
+//      float32[3*n_solid] solids (written downward from slot total-1);
+//      sandsink: then int32 ncells, int32[ncells] cells of one sink that fires
+//      after every step (the sink loop of Lustrine::simulate)
+// out: float32[3*n] positions, float32[3*n] velocities, int32[n] attracted;
+//      sandsink: first int32 n (the final ptr_sand_end), the whole-domain
+//      query_cell_num_particles without and with solids, and
+//      num_remaining_sand_particles
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -21,6 +27,15 @@ struct vec3 {
 };
 struct BulletSim {
     vec3 player_position{0.0f, 0.0f, 0.0f};
+};
+struct ParticleSink {  // src/Simulation.hpp:94-103
+    std::vector<bool> state;
+    std::vector<float> timers;
+    std::vector<float> frequencies;
+    std::vector<int> despawned;
+    std::vector<std::vector<int>> sink_cells;
+    int num_sinks = 0;
+    std::vector<int> temp_removal;
 };
 struct Simulation {  // members read by the adapter, with their reference types
     BulletSim bullet_physics_simulation;
@@ -38,6 +53,8 @@ struct Simulation {  // members read by the adapter, with their reference types
     vec3* positions_star = nullptr;
     int ptr_sand_start = -1, ptr_sand_end = -1, ptr_solid_start = -1, ptr_solid_end = -1;
     int num_sand_particles = 0;
+    int num_remaining_sand_particles = 0;
+    ParticleSink* sink = nullptr;
     bool first_iteration = true;
     bool attract_flag = false, blow_flag = false;
     int* attracted = nullptr;
@@ -60,6 +77,19 @@ int main(int argc, char** argv) {
     std::vector<int> att((size_t)total, 0);
     std::vector<float> buf(3 * (size_t)(n + ns));
     if (fread(buf.data(), sizeof(float), buf.size(), f) != buf.size()) return 2;
+    stand_in::ParticleSink sink;
+    if (mode == "sandsink") {
+        int nc = 0;
+        if (fread(&nc, sizeof(int), 1, f) != 1) return 2;
+        std::vector<int> cells((size_t)nc);
+        if (fread(cells.data(), sizeof(int), (size_t)nc, f) != (size_t)nc) return 2;
+        sink.num_sinks = 1;
+        sink.state = {true};
+        sink.timers = {0.0f};
+        sink.frequencies = {0.0f};  // fires on every frame
+        sink.despawned = {0};
+        sink.sink_cells = {cells};
+    }
     fclose(f);
     for (int i = 0; i < n; ++i) pos[i] = {buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]};
     for (int k = 0; k < ns; ++k) {
@@ -89,10 +119,12 @@ int main(int argc, char** argv) {
     s.num_sand_particles = n;
     s.ptr_solid_start = ns ? total - ns : total - 1;
     s.ptr_solid_end = ns ? total : total - 1;
+    s.sink = &sink;
     const int steps = atoi(argv[4]);
     for (int t = 0; t < steps; ++t) {
-        const int rc = mode == "pbf" ? lus_adapter::simulate_pbf(&s, 1.0f / 60.0f, 4, 1)
-                                     : lus_adapter::simulate_sand(&s, 0.016f);
+        int rc = mode == "pbf" ? lus_adapter::simulate_pbf(&s, 1.0f / 60.0f, 4, 1)
+                               : lus_adapter::simulate_sand(&s, 0.016f);
+        if (!rc && mode == "sandsink") rc = lus_adapter::evict_sinks(&s);
         if (rc) {
             char msg[512];
             lus_gpu_last_error(msg, sizeof msg);
@@ -100,12 +132,24 @@ int main(int argc, char** argv) {
             return 1;
         }
     }
+    int q[2] = {0, 0};
+    if (mode == "sandsink") {
+        const stand_in::vec3 lo{0.0f, 0.0f, 0.0f}, hi{s.domainX, s.domainY, s.domainZ};
+        if (lus_adapter::query_cell_num_particles(&s, lo, hi, false, &q[0]) ||
+            lus_adapter::query_cell_num_particles(&s, lo, hi, true, &q[1]))
+            return 1;
+    }
     lus_adapter::release(&s);
     FILE* o = fopen(argv[3], "wb");
     if (!o) return 2;
-    fwrite(pos.data(), sizeof(float), 3 * (size_t)n, o);
-    fwrite(vel.data(), sizeof(float), 3 * (size_t)n, o);
-    fwrite(att.data(), sizeof(int), (size_t)n, o);
+    if (mode == "sandsink") {
+        const int hdr_out[4] = {s.ptr_sand_end, q[0], q[1], s.num_remaining_sand_particles};
+        fwrite(hdr_out, sizeof(int), 4, o);
+    }
+    const int nw = mode == "sandsink" ? s.ptr_sand_end : n;
+    fwrite(pos.data(), sizeof(float), 3 * (size_t)nw, o);
+    fwrite(vel.data(), sizeof(float), 3 * (size_t)nw, o);
+    fwrite(att.data(), sizeof(int), (size_t)nw, o);
     fclose(o);
     return 0;
 }

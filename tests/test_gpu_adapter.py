@@ -68,3 +68,41 @@ def test_adapter_simulate_pbf_matches_oracle(gpu_mod, oracle, tmp_path):
     pos, vel, _ = _run("pbf", tmp_path, sand, np.zeros((0, 3), np.float32), total, box, 4.0, 2)
     assert np.array_equal(pos, want_x)
     assert np.array_equal(vel, want_v)
+
+
+def test_adapter_sinks_and_cell_query_match_reference(gpu_mod, oracle, tmp_path):
+    # one sink firing after every step over a slab of cells, then the cell
+    # query of the whole domain: the adapter serves both from the device
+    box = (16, 16, 16)
+    sand = scenes.jittered_lattice(8, 8, 8, origin=(3.0, 2.0, 3.0), amplitude=0.2, seed=12)
+    solids = scenes.lattice(16, 1, 16)
+    s = oracle.init_constants(*box, 0.5, 1.0, 3.1)
+    gx, gz = s.gridX, s.gridZ
+    cells = [(y * gx + x) * gz + z for y in (3, 4) for x in range(3, 7) for z in range(2, 9)]
+    st = oracle.SandStream(s, sand, solids)
+    removed = 0
+    for _ in range(3):
+        st.step(np.float32(0.016))
+        removed += st.sink_evict(cells)
+    n = st.n
+    assert 0 < removed and n == sand.shape[0] - removed
+    inp = tmp_path / "sink_in.bin"
+    out = tmp_path / "sink_out.bin"
+    with open(inp, "wb") as f:
+        np.array([sand.shape[0], solids.shape[0], st.total], np.int32).tofile(f)
+        np.ascontiguousarray(sand, np.float32).tofile(f)
+        np.ascontiguousarray(solids, np.float32).tofile(f)
+        np.array([len(cells)] + cells, np.int32).tofile(f)
+    r = subprocess.run([EXE, "sandsink", str(inp), str(out), "3", *map(str, box), "3.1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.uint8)
+    hdr = raw[:16].view(np.int32)
+    assert hdr[0] == n and hdr[3] == removed
+    assert hdr[1] == st.query_cells([0, 0, 0], box, False)
+    assert hdr[2] == st.query_cells([0, 0, 0], box, True)
+    body = raw[16:]
+    pos = body[: 12 * n].view(np.float32).reshape(n, 3)
+    vel = body[12 * n: 24 * n].view(np.float32).reshape(n, 3)
+    att = body[24 * n: 28 * n].view(np.int32)
+    assert np.array_equal(pos, st.p[:n]) and np.array_equal(vel, st.v[:n]) and np.array_equal(att, st.att[:n])
