@@ -320,6 +320,17 @@ int lus_sand_query_cells(LusGpu* ctx, const float* min_pos, const float* max_pos
  * id outside the grid is LUS_EINVAL; before the first step nothing is
  * removed. */
 int lus_sand_sink_evict(LusGpu* ctx, const int* cells, int ncells, int* n_sand);
+/* Bullet coupling without a full download (src/BulletPhysics.cpp:602-646,
+ * set_particles_box_colliders_positions over the sand range): the first
+ * max_out sand slots, in slot order, whose current position x satisfies
+ * |x - player| <= half_extent on every axis, where half_extent is what
+ * particle_collide_with_player computes, (player_box_scale + 4 *
+ * particle_radius) * 0.5 per axis.  slots[k] and pos[3k..3k+2] (each
+ * nullable) receive the k-th such particle, *count how many were written
+ * (at most max_out; the reference places num_particles_allocated = 100
+ * colliders).  Only those records cross to the host. */
+int lus_sand_player_box(LusGpu* ctx, const float* player, const float* half_extent, int max_out, int* slots,
+                        float* pos, int* count);
 
 #ifdef __cplusplus
 }

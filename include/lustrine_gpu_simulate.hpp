@@ -238,6 +238,28 @@ int evict_sinks(Sim* s) {
     return LUS_OK;
 }
 
+/* The particle side of set_particles_box_colliders_positions
+ * (src/BulletPhysics.cpp:615-646) for a Simulation stepped by
+ * simulate_sand(): the positions of the first num_particles_allocated sand
+ * particles, in slot order, inside the player's box
+ * (particle_collide_with_player, :602-606, around
+ * bullet_physics_simulation.player_position, which the caller sets first as
+ * the reference does) go to out[0..*count); the caller places that many
+ * collider bodies and disables the rest.  The positions are the device's
+ * current ones, i.e. the host arrays simulate_sand() left.  LUS_ESTATE before
+ * the first simulate_sand() of this Simulation (use the host loop then). */
+template <class Sim, class V>
+int player_box_particles(Sim* s, V* out, int* count) {
+    auto it = states().find(s);
+    if (it == states().end() || it->second.kind != 1) return LUS_ESTATE;
+    const auto& b = s->bullet_physics_simulation;
+    const float r4 = b.particle_radius * 4.0f;
+    const float half[3] = {(b.player_box_scale.x + r4) * 0.5f, (b.player_box_scale.y + r4) * 0.5f,
+                           (b.player_box_scale.z + r4) * 0.5f};
+    const float player[3] = {b.player_position.x, b.player_position.y, b.player_position.z};
+    return lus_sand_player_box(it->second.ctx, player, half, b.num_particles_allocated, nullptr, fptr(out), count);
+}
+
 }  // namespace lus_adapter
 
 #endif /* LUSTRINE_GPU_SIMULATE_HPP */

@@ -1873,6 +1873,44 @@ int lus_sand_sink_evict(LusGpu* c, const int* cells, int ncells, int* n_sand) {
     return LUS_OK;
 }
 
+int lus_sand_player_box(LusGpu* c, const float* player, const float* half_extent, int max_out, int* slots,
+                        float* pos, int* count) {
+    if (!c || !c->sand.ready) return set_error(LUS_ESTATE, "lus_sand_player_box: no sand scene");
+    if (!player || !half_extent || !count || max_out < 0)
+        return set_error(LUS_EINVAL, "lus_sand_player_box: bad arguments");
+    SandState& S = c->sand;
+    *count = 0;
+    const int n = S.n;
+    if (n == 0 || max_out == 0) return LUS_OK;
+    LUS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int nb = (n + lus::kBlock - 1) / lus::kBlock;
+    Arena tmp;
+    int *bc, *off, *tiles, *d_slots;
+    float4* d_pos;
+    LUS_HIP(tmp.alloc(&bc, nb + 1));
+    LUS_HIP(tmp.alloc(&off, nb + 1));
+    LUS_HIP(tmp.alloc(&tiles, lus::scan_tile_count(nb + 1) + 1));
+    LUS_HIP(tmp.alloc(&d_slots, max_out));
+    LUS_HIP(tmp.alloc(&d_pos, max_out));
+    LUS_HIP(hipMemsetAsync(bc + nb, 0, sizeof(int), s));
+    LUS_HIP(lus::launch_box_count(S.x, n, player, half_extent, bc, s));
+    LUS_HIP(lus::launch_exclusive_scan(bc, off, nb + 1, tiles, 0, s));
+    LUS_HIP(lus::launch_box_write(S.x, n, player, half_extent, off, max_out, d_slots, d_pos, s));
+    int total = 0;
+    LUS_HIP(hipMemcpyAsync(&total, off + nb, sizeof(int), hipMemcpyDeviceToHost, s));
+    LUS_HIP(hipStreamSynchronize(s));
+    const int k = std::min(total, max_out);
+    if (k > 0 && slots) LUS_HIP(hipMemcpy(slots, d_slots, sizeof(int) * k, hipMemcpyDeviceToHost));
+    if (k > 0 && pos) {
+        std::vector<float4> h((size_t)k);
+        LUS_HIP(hipMemcpy(h.data(), d_pos, sizeof(float4) * k, hipMemcpyDeviceToHost));
+        from_f4(h, pos);
+    }
+    *count = k;
+    return LUS_OK;
+}
+
 int lus_sand_neighbors(LusGpu* c, long* off, int* idx, long cap, long* total) {
     if (!c || !c->sand.ready || !c->sand.stepped || !off || !total)
         return set_error(LUS_ESTATE, "lus_sand_neighbors: no step yet");

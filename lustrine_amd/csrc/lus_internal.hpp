@@ -232,6 +232,14 @@ hipError_t launch_cell_box_count(const int* cs, const int* ocs, const int* solid
                                  hipStream_t s);
 // out[k] = (cs[cells[k]], cs[cells[k] + 1]); cells must be valid cell ids
 hipError_t launch_cell_ranges(const int* cs, const int* cells, int n, int2* out, hipStream_t s);
+// first k slots (plain slot order) with |x - c| <= h on every axis:
+// block_counts[b] = matches of slots [256 b, 256 b + 256), (n + 255) / 256
+// entries; after an exclusive scan into block_off, box_write stores the slots
+// and positions of the matches of rank < k
+hipError_t launch_box_count(const float4* x, int n, const float* c, const float* h, int* block_counts,
+                            hipStream_t s);
+hipError_t launch_box_write(const float4* x, int n, const float* c, const float* h, const int* block_off, int k,
+                            int* slots, float4* pos, hipStream_t s);
 // a[mv[k].x] = a_old[mv[k].y] for the four arrays; t*: m staging entries each
 hipError_t launch_slot_moves(const int2* mv, int m, float4* a0, float4* a1, float4* a2, int* a3,
                              float4* t0, float4* t1, float4* t2, int* t3, hipStream_t s);

@@ -715,6 +715,69 @@ __global__ __launch_bounds__(kBlock) void k_moves_scatter(const int2* __restrict
     a3[dst] = t3[k];
 }
 
+// Particles inside the player's collision box (Bullet coupling,
+// src/BulletPhysics.cpp:602-606): |x - player| <= half on every axis.
+__device__ __forceinline__ bool in_player_box(const float4 p, const float4 c, const float4 h) {
+    return fabsf(p.x - c.x) <= h.x && fabsf(p.y - c.y) <= h.y && fabsf(p.z - c.z) <= h.z;
+}
+
+// Ordered compaction of the first k slots inside the box, in two passes over
+// slots in plain order (block b holds slots [256 b, 256 b + 256)): the count
+// per block, then, after an exclusive scan of the counts, each match's rank
+// = its block's offset + the matches before it in the block.
+__global__ __launch_bounds__(kBlock) void k_box_count(const float4* __restrict__ x, int n, float4 c, float4 h,
+                                                      int* __restrict__ block_counts) {
+    __shared__ int wc[kBlock / 64];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool in = i < n && in_player_box(x[i], c, h);
+    const unsigned long long m = __ballot(in);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += wc[w];
+        block_counts[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_box_write(const float4* __restrict__ x, int n, float4 c, float4 h,
+                                                      const int* __restrict__ block_off, int k,
+                                                      int* __restrict__ slots, float4* __restrict__ pos) {
+    __shared__ int wc[kBlock / 64];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const float4 p = i < n ? x[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool in = i < n && in_player_box(p, c, h);
+    const unsigned long long m = __ballot(in);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wc[w] = __popcll(m);
+    __syncthreads();
+    int before = block_off[blockIdx.x];
+    for (int v = 0; v < w; ++v) before += wc[v];
+    const int rank = before + __popcll(m & ((1ull << lane) - 1ull));
+    if (in && rank < k) {
+        slots[rank] = i;
+        pos[rank] = p;
+    }
+}
+
+hipError_t launch_box_count(const float4* x, int n, const float* c, const float* h, int* block_counts,
+                            hipStream_t s) {
+    const int nb = (n + kBlock - 1) / kBlock;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_box_count, dim3(nb), dim3(kBlock), 0, s, x, n, make_float4(c[0], c[1], c[2], 0.0f),
+                       make_float4(h[0], h[1], h[2], 0.0f), block_counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_box_write(const float4* x, int n, const float* c, const float* h, const int* block_off, int k,
+                            int* slots, float4* pos, hipStream_t s) {
+    const int nb = (n + kBlock - 1) / kBlock;
+    if (nb == 0 || k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_box_write, dim3(nb), dim3(kBlock), 0, s, x, n, make_float4(c[0], c[1], c[2], 0.0f),
+                       make_float4(h[0], h[1], h[2], 0.0f), block_off, k, slots, pos);
+    return hipGetLastError();
+}
+
 hipError_t launch_slot_moves(const int2* mv, int m, float4* a0, float4* a1, float4* a2, int* a3,
                              float4* t0, float4* t1, float4* t2, int* t3, hipStream_t s) {
     if (m <= 0) return hipSuccess;

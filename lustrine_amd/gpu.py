@@ -116,6 +116,7 @@ _SIGS = {
     "lus_sand_neighbors": (C.c_int, [C.c_void_p, _l, _i, C.c_long, _l]),
     "lus_sand_query_cells": (C.c_int, [C.c_void_p, _f, _f, C.c_int, _i]),
     "lus_sand_sink_evict": (C.c_int, [C.c_void_p, _i, C.c_int, _i]),
+    "lus_sand_player_box": (C.c_int, [C.c_void_p, _f, _f, C.c_int, _i, _f, _i]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -481,6 +482,17 @@ class Gpu:
                                              C.byref(out)), "lus_sand_sink_evict")
         self.n_sand = out.value
         return out.value
+
+
+    def sand_player_box(self, player, half_extent, max_out=100):
+        """Slots and positions of the first max_out sand particles in the player's box."""
+        pl, he = _f32(player, (3,)), _f32(half_extent, (3,))
+        slots = np.zeros(max(max_out, 1), np.int32)
+        pos = np.zeros((max(max_out, 1), 3), np.float32)
+        cnt = C.c_int(0)
+        _check(self._lib.lus_sand_player_box(self._h, _ptr_f(pl), _ptr_f(he), int(max_out), _ptr_i(slots),
+                                             _ptr_f(pos), C.byref(cnt)), "lus_sand_player_box")
+        return slots[: cnt.value], pos[: cnt.value]
 
 
 class Trainer:

@@ -276,6 +276,19 @@ class SandStream:
             pass
 
 
+def player_box(positions, player, half_extent, max_out=100):
+    """set_particles_box_colliders_positions (src/BulletPhysics.cpp:615-646)
+    over the sand range: the first max_out slots, in slot order, with
+    glm::all(|x - player| <= half_extent) (particle_collide_with_player,
+    :602-606; half_extent = (player_box_scale + 4 particle_radius) * 0.5),
+    and their positions.  float32 arithmetic as there."""
+    p = np.asarray(positions, np.float32)
+    d = np.abs(p - np.asarray(player, np.float32))
+    inside = np.all(d <= np.asarray(half_extent, np.float32), axis=1)
+    slots = np.nonzero(inside)[0][:max_out].astype(np.int32)
+    return slots, p[slots]
+
+
 def simulate_fluid(s, p, ps, v, lambdas, sand, solid, dt):
     lib().orc_simulate_fluid(C.byref(s), _fp(p), _fp(ps), _fp(v), _fp(lambdas),
                              sand[0], sand[1], solid[0], solid[1], dt)

@@ -4,6 +4,7 @@
 //   adapter_sand.exe sand     <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
 //   adapter_sand.exe pbf      <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
 //   adapter_sand.exe sandsink <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
+//   adapter_sand.exe sandbox  <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
 // in:  int32 n_sand, int32 n_solid, int32 total, float32[3*n_sand] sand,
 //      float32[3*n_solid] solids (written downward from slot total-1);
 //      sandsink: then int32 ncells, int32[ncells] cells of one sink that fires
@@ -11,7 +12,9 @@
 // out: float32[3*n] positions, float32[3*n] velocities, int32[n] attracted;
 //      sandsink: first int32 n (the final ptr_sand_end), the whole-domain
 //      query_cell_num_particles without and with solids, and
-//      num_remaining_sand_particles
+//      num_remaining_sand_particles;
+//      sandbox: then float32 player_position[3], player_box_scale[3]; out
+//      first int32 count and float32[3*count] collider positions
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -25,8 +28,11 @@ namespace stand_in {
 struct vec3 {
     float x, y, z;
 };
-struct BulletSim {
+struct BulletSim {  // src/BulletPhysicsSimulation.hpp: the members the adapter reads
     vec3 player_position{0.0f, 0.0f, 0.0f};
+    int num_particles_allocated = 100;
+    vec3 player_box_scale{1.0f, 2.0f, 1.0f};
+    float particle_radius = 0.1f;
 };
 struct ParticleSink {  // src/Simulation.hpp:94-103
     std::vector<bool> state;
@@ -78,6 +84,8 @@ int main(int argc, char** argv) {
     std::vector<float> buf(3 * (size_t)(n + ns));
     if (fread(buf.data(), sizeof(float), buf.size(), f) != buf.size()) return 2;
     stand_in::ParticleSink sink;
+    float box_in[6] = {0, 0, 0, 0, 0, 0};
+    if (mode == "sandbox" && fread(box_in, sizeof(float), 6, f) != 6) return 2;
     if (mode == "sandsink") {
         int nc = 0;
         if (fread(&nc, sizeof(int), 1, f) != 1) return 2;
@@ -139,9 +147,20 @@ int main(int argc, char** argv) {
             lus_adapter::query_cell_num_particles(&s, lo, hi, true, &q[1]))
             return 1;
     }
+    std::vector<stand_in::vec3> near((size_t)s.bullet_physics_simulation.num_particles_allocated);
+    int near_count = 0;
+    if (mode == "sandbox") {
+        s.bullet_physics_simulation.player_position = {box_in[0], box_in[1], box_in[2]};
+        s.bullet_physics_simulation.player_box_scale = {box_in[3], box_in[4], box_in[5]};
+        if (lus_adapter::player_box_particles(&s, near.data(), &near_count)) return 1;
+    }
     lus_adapter::release(&s);
     FILE* o = fopen(argv[3], "wb");
     if (!o) return 2;
+    if (mode == "sandbox") {
+        fwrite(&near_count, sizeof(int), 1, o);
+        fwrite(near.data(), sizeof(float), 3 * (size_t)near_count, o);
+    }
     if (mode == "sandsink") {
         const int hdr_out[4] = {s.ptr_sand_end, q[0], q[1], s.num_remaining_sand_particles};
         fwrite(hdr_out, sizeof(int), 4, o);

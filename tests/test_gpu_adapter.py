@@ -106,3 +106,34 @@ def test_adapter_sinks_and_cell_query_match_reference(gpu_mod, oracle, tmp_path)
     vel = body[12 * n: 24 * n].view(np.float32).reshape(n, 3)
     att = body[24 * n: 28 * n].view(np.int32)
     assert np.array_equal(pos, st.p[:n]) and np.array_equal(vel, st.v[:n]) and np.array_equal(att, st.att[:n])
+
+
+def test_adapter_player_box_matches_reference(gpu_mod, oracle, tmp_path):
+    # the collider positions of set_particles_box_colliders_positions after
+    # 2 steps, from the device: the first 100 sand particles in the box
+    box = (16, 16, 16)
+    sand = scenes.jittered_lattice(10, 10, 10, origin=(3.0, 2.0, 3.0), amplitude=0.2, seed=13)
+    solids = scenes.lattice(16, 1, 16)
+    s = oracle.init_constants(*box, 0.5, 1.0, 3.1)
+    st = oracle.SandStream(s, sand, solids)
+    for _ in range(2):
+        st.step(np.float32(0.016))
+    player = np.array([7.5, 5.0, 8.0], np.float32)
+    scale = np.array([3.0, 4.0, 2.5], np.float32)
+    half = (scale + np.float32(0.1) * np.float32(4.0)) * np.float32(0.5)  # particle_radius 0.1
+    _, want = oracle.player_box(st.p[:st.n], player, half, 100)
+    inp = tmp_path / "box_in.bin"
+    out = tmp_path / "box_out.bin"
+    with open(inp, "wb") as f:
+        np.array([sand.shape[0], solids.shape[0], st.total], np.int32).tofile(f)
+        np.ascontiguousarray(sand, np.float32).tofile(f)
+        np.ascontiguousarray(solids, np.float32).tofile(f)
+        np.concatenate([player, scale]).astype(np.float32).tofile(f)
+    r = subprocess.run([EXE, "sandbox", str(inp), str(out), "2", *map(str, box), "3.1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.uint8)
+    k = int(raw[:4].view(np.int32)[0])
+    got = raw[4: 4 + 12 * k].view(np.float32).reshape(k, 3)
+    assert k == len(want) and k > 20
+    assert np.array_equal(got, want)

@@ -108,3 +108,27 @@ def test_sink_evict_edges(gpu_mod, oracle):
         assert g.sand_sink_evict(np.concatenate([allc, allc])) == 0
         assert g.sand_query_cells([0, 0, 0], BOX, False) == 0
         assert g.sand_query_cells([0, 0, 0], BOX, True) == st.query_cells([0, 0, 0], BOX, True)
+
+
+def test_player_box_matches_oracle(gpu_mod, oracle):
+    s, st, g = _streams(gpu_mod, oracle, n_side=16, seed=4)
+    with g:
+        n = st.n
+        pl, he = np.array([9.0, 9.0, 9.0], np.float32), np.array([1.5, 1.5, 1.5], np.float32)
+        slots, pos = g.sand_player_box(pl, he)  # before a step: the uploaded order
+        os_, op = oracle.player_box(st.p[:n], pl, he)
+        assert np.array_equal(slots, os_) and np.array_equal(pos, op)
+        for _ in range(2):
+            st.step(DT)
+            g.sand_step(DT)
+        rng = np.random.default_rng(8)
+        for t in range(30):
+            pl = rng.uniform(1.0, 19.0, 3).astype(np.float32)
+            he = rng.uniform(0.2, 6.0, 3).astype(np.float32)
+            k = [100, 7, 1, 4096][t % 4]
+            slots, pos = g.sand_player_box(pl, he, k)
+            os_, op = oracle.player_box(st.p[:n], pl, he, k)
+            assert np.array_equal(slots, os_) and np.array_equal(pos, op), (pl, he, k)
+        # a box holding more than 100 particles spread over many blocks
+        slots, _ = g.sand_player_box([10, 8, 10], [6, 6, 6])
+        assert len(slots) == 100 and np.all(np.diff(slots) > 0)

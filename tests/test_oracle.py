@@ -451,3 +451,19 @@ def test_sink_evict_reference_quirks(oracle):
     st.sink_evict([cell[0]])
     assert st.query_cells([0, 0, 0], [40, 16, 16], False) == 11
     assert st.query_cells([0, 0, 0], [40, 16, 16], True) == 12
+
+
+def test_player_box_closed_form(oracle):
+    # a unit lattice 10^3 at integer sites: the box of half extent 1.5 around
+    # a site holds its 3x3x3 neighbourhood, in slot (x-major here) order; a
+    # face exactly at a site (half extent 1.0) includes it (<=); the first
+    # max_out only, in slot order
+    g = np.arange(10, dtype=np.float32)
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    slots, p = oracle.player_box(pos, [4, 5, 6], [1.5, 1.5, 1.5])
+    want = [(x * 10 + y) * 10 + z for x in (3, 4, 5) for y in (4, 5, 6) for z in (5, 6, 7)]
+    assert slots.tolist() == want and np.array_equal(p, pos[want])
+    assert len(oracle.player_box(pos, [4, 5, 6], [1.0, 1.0, 1.0])[0]) == 27
+    assert len(oracle.player_box(pos, [4, 5, 6], [0.999, 1.0, 1.0])[0]) == 9
+    assert oracle.player_box(pos, [4, 5, 6], [1.5, 1.5, 1.5], max_out=5)[0].tolist() == want[:5]
+    assert len(oracle.player_box(pos, [40, 5, 6], [1.5, 1.5, 1.5])[0]) == 0
