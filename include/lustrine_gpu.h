@@ -295,6 +295,33 @@ int lus_sand_download(LusGpu* ctx, float* pos, float* pos_star, float* vel, int*
  * src/neighbors/Neighbors.cpp:262-364), reference slot indices. */
 int lus_sand_neighbors(LusGpu* ctx, long* off, int* idx, long cap, long* total);
 
+/* --- phase timing in the reference's profiling slots --- */
+/* The reference keeps LUSTRINE_MAX_NUM_MEASUREMENTS = 7 counters holding the
+ * last measurement of each phase and reads them with get_num_observation /
+ * get_cycles / get_duration (src/profiling/Profiling.hpp:14-77, filled in
+ * src/Lustrine.cpp:802-804 and src/Simulate.cpp:669-802).  With profiling on
+ * for a context, each lus_sand_step call, and the last step of each
+ * lus_pbf_step call, records hipEvents around its phases, waits for them, and
+ * stores the GPU times in the same slots, process-wide:
+ *   0  the step (the reference: simulate_fun, Bullet included)
+ *   1  Bullet -- on the host, not measured here: 0
+ *   2  the solver after Bullet (here: the whole GPU step)
+ *   3  neighbour search (sand: sort + lists; PBF: predict, sort, tiles)
+ *   4  constraint substeps (sand) / solver iterations with the list build,
+ *      features and network (PBF)
+ *   5  boundary collision -- inside the solver kernels here: 0
+ *   6  velocity update (sand) / XSPH and the rerun decision (PBF)
+ * Durations are seconds; cycles are duration x 2.7e9, the fixed rate the
+ * reference's Unix build divides rdtsc cycles by, so cycles / 2.7e9 is the
+ * duration as there.  Off by default (a profiled call waits for its
+ * kernels). */
+int       lus_profiling_enable(LusGpu* ctx, int on);
+/* src/profiling/Profiling.hpp:75-77 with a lus_ prefix: slot count (7), and
+ * the last measurement of slot `index` (0 outside 0..6). */
+long      lus_get_num_observation(void);
+long long lus_get_cycles(int index);
+double    lus_get_duration(int index);
+
 /* --- consumers of the grid state, served from the device cell tables --- */
 /* query_cell_num_particles (src/Lustrine.cpp:864-914): the number of ids in
  * the cell lists of the last step's grid (uniform_gird_cells: each cell's

@@ -20,6 +20,43 @@ namespace {
 int g_err_code = 0;
 char g_err_msg[512] = "";
 
+// The reference's profiling counters (src/profiling/Profiling.hpp:14-77):
+// LUSTRINE_MAX_NUM_MEASUREMENTS slots holding the last measurement, cycles
+// and seconds; its Unix build converts rdtsc cycles with a fixed 2.7 GHz.
+constexpr int kProfSlots = 7;
+constexpr double kProfHz = 2700000000.0;
+double g_prof_dur[kProfSlots] = {0, 0, 0, 0, 0, 0, 0};
+long long g_prof_cyc[kProfSlots] = {0, 0, 0, 0, 0, 0, 0};
+
+void prof_store(const double* sec) {
+    for (int i = 0; i < kProfSlots; ++i) {
+        g_prof_dur[i] = sec[i];
+        g_prof_cyc[i] = (long long)std::llround(sec[i] * kProfHz);
+    }
+}
+
+// hipEvents destroyed with the set
+struct EventSet {
+    std::vector<hipEvent_t> e;
+    ~EventSet() {
+        for (hipEvent_t x : e) (void)hipEventDestroy(x);
+    }
+    hipError_t create(int n) {
+        for (int i = 0; i < n; ++i) {
+            hipEvent_t x;
+            hipError_t r = hipEventCreate(&x);
+            if (r != hipSuccess) return r;
+            e.push_back(x);
+        }
+        return hipSuccess;
+    }
+    double seconds(int a, int b) const {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e[a], e[b]) != hipSuccess) return 0.0;
+        return (double)ms * 1e-3;
+    }
+};
+
 int set_error(int code, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -195,6 +232,7 @@ struct LusMlpTrainer;
 struct LusGpu {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool profiling = false;  // record phase events (lus_profiling_enable)
     PbfState pbf;
     NetState net;
     SandState sand;
@@ -943,8 +981,46 @@ int lus_pbf_step(LusGpu* c, float dt, int steps) {
     if (!c || !c->pbf.ready) return set_error(LUS_ESTATE, "lus_pbf_step: no PBF scene");
     if (!(dt > 0.0f) || steps < 0) return set_error(LUS_EINVAL, "lus_pbf_step: bad dt/steps");
     LUS_HIP(hipSetDevice(c->device));
-    return pbf_run(c, dt, steps);
+    if (!c->profiling || steps == 0) return pbf_run(c, dt, steps);
+    // profiling: the last step runs with an event after every kernel
+    int r = pbf_run(c, dt, steps - 1);
+    if (r) return r;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ph;
+    r = pbf_enqueue(c, dt, &ev, &ph);
+    if (!r && hipEventSynchronize(ev.back()) != hipSuccess) r = set_error(LUS_EHIP, "lus_pbf_step: event");
+    if (!r) {
+        // slots: 3 predict, sort and tiles; 4 the solver iterations (with the
+        // list build, features and network); 6 XSPH and the rerun decision;
+        // 0 and 2 the whole step; 1 (Bullet) and 5 (boundary, fused) 0
+        double sec[kProfSlots] = {0, 0, 0, 0, 0, 0, 0};
+        for (size_t q = 1; q < ev.size(); ++q) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[q - 1], ev[q]);
+            const double t = (double)ms * 1e-3;
+            const int p = ph[q];
+            const int slot = (p <= 3 || p == 11) ? 3 : (p == 6 || p == 10) ? 6 : 4;
+            sec[slot] += t;
+            sec[0] += t;
+        }
+        sec[2] = sec[0];
+        prof_store(sec);
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    return r;
 }
+
+int lus_profiling_enable(LusGpu* c, int on) {
+    if (!c) return set_error(LUS_EINVAL, "lus_profiling_enable: no context");
+    c->profiling = on != 0;
+    return LUS_OK;
+}
+
+long lus_get_num_observation(void) { return kProfSlots; }
+
+long long lus_get_cycles(int index) { return (index >= 0 && index < kProfSlots) ? g_prof_cyc[index] : 0; }
+
+double lus_get_duration(int index) { return (index >= 0 && index < kProfSlots) ? g_prof_dur[index] : 0.0; }
 
 int lus_pbf_step_latency(LusGpu* c, float dt, int steps, float* ms) {
     if (!c || !c->pbf.ready || !ms || steps <= 0 || !(dt > 0.0f))
@@ -1706,7 +1782,14 @@ int lus_sand_step(LusGpu* c, float dt, int attract_flag, int blow_flag, const fl
     k.blow = blow_flag ? 1 : 0;
     k.prev_attract = S.prev_attract;
     for (int a = 0; a < 3; ++a) k.player[a] = player ? player[a] : 0.0f;
+    // phase events (profiling): start, after predict, after the lists, after
+    // the substeps, after the velocity update
+    EventSet pe;
+    auto mark = [&](int q) -> hipError_t { return c->profiling ? hipEventRecord(pe.e[q], s) : hipSuccess; };
+    if (c->profiling) LUS_HIP(pe.create(5));
+    LUS_HIP(mark(0));
     LUS_HIP(lus::launch_sand_predict_key(S.x, S.v, S.att, n, k, dt, S.xs, S.key, S.slot, S.counts, s));
+    LUS_HIP(mark(1));
     LUS_HIP(lus::launch_exclusive_scan(S.counts, S.cs, S.C + 1, S.tiles, 1, s));
     LUS_HIP(lus::launch_scatter(S.key, S.slot, S.cs, n, S.tmp, s));
     LUS_HIP(lus::launch_sand_permute(S.tmp, S.key, S.cs, S.x, S.xs, S.v, S.att, n, S.x2, S.A, S.v2,
@@ -1727,13 +1810,26 @@ int lus_sand_step(LusGpu* c, float dt, int attract_flag, int blow_flag, const fl
     }
     LUS_HIP(lus::launch_sand_nbr(S.A, S.key2, S.cs, n, S.solid, S.ocs, k, S.nbr_off, S.nbr_cnt,
                                  S.nbr_cnt1, S.nbr_idx, 1, s));
+    LUS_HIP(mark(2));
     float4* A = S.A;
     float4* B = S.B;
     for (int it = 0; it < S.p.substeps; ++it) {
         LUS_HIP(lus::launch_sand_substep(A, S.x2, S.solid, S.nbr_off, S.nbr_idx, n, k, B, s));
         std::swap(A, B);
     }
+    LUS_HIP(mark(3));
     LUS_HIP(lus::launch_sand_vupdate(A, S.x2, S.v2, n, dt, s));
+    LUS_HIP(mark(4));
+    if (c->profiling) {
+        // slots of simulate_sand_v1 (src/Simulate.cpp:669-802): 0 the call,
+        // 2 the solver, 3 the neighbour search, 4 the substeps, 6 the velocity
+        // update; 1 (Bullet, on the host) and 5 (boundary, inside the
+        // substep kernels here) are 0
+        LUS_HIP(hipEventSynchronize(pe.e[4]));
+        const double all = pe.seconds(0, 4);
+        const double sec[kProfSlots] = {all, 0.0, all, pe.seconds(1, 2), pe.seconds(2, 3), 0.0, pe.seconds(3, 4)};
+        prof_store(sec);
+    }
     S.A = A;
     S.B = B;
     S.final_iter = A;

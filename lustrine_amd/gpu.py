@@ -117,6 +117,10 @@ _SIGS = {
     "lus_sand_query_cells": (C.c_int, [C.c_void_p, _f, _f, C.c_int, _i]),
     "lus_sand_sink_evict": (C.c_int, [C.c_void_p, _i, C.c_int, _i]),
     "lus_sand_player_box": (C.c_int, [C.c_void_p, _f, _f, C.c_int, _i, _f, _i]),
+    "lus_profiling_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "lus_get_num_observation": (C.c_long, []),
+    "lus_get_cycles": (C.c_longlong, [C.c_int]),
+    "lus_get_duration": (C.c_double, [C.c_int]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -484,6 +488,10 @@ class Gpu:
         return out.value
 
 
+    def profiling(self, on=True):
+        """Phase events into the reference's 7 profiling slots (see profile_slots)."""
+        _check(self._lib.lus_profiling_enable(self._h, int(bool(on))), "lus_profiling_enable")
+
     def sand_player_box(self, player, half_extent, max_out=100):
         """Slots and positions of the first max_out sand particles in the player's box."""
         pl, he = _f32(player, (3,)), _f32(half_extent, (3,))
@@ -493,6 +501,13 @@ class Gpu:
         _check(self._lib.lus_sand_player_box(self._h, _ptr_f(pl), _ptr_f(he), int(max_out), _ptr_i(slots),
                                              _ptr_f(pos), C.byref(cnt)), "lus_sand_player_box")
         return slots[: cnt.value], pos[: cnt.value]
+
+
+def profile_slots():
+    """(cycles, seconds) of the last profiled step in the reference's slots."""
+    lib = load()
+    n = int(lib.lus_get_num_observation())
+    return ([int(lib.lus_get_cycles(i)) for i in range(n)], [float(lib.lus_get_duration(i)) for i in range(n)])
 
 
 class Trainer:

@@ -18,7 +18,8 @@ def declared_symbols():
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"^\s*(?:int|void|long|float)\s+\**\s*(lus_\w+)\s*\(", text, flags=re.M):
+        for m in re.finditer(r"^\s*(?:int|void|long long|long|float|double)\s+\**\s*(lus_\w+)\s*\(", text,
+                             flags=re.M):
             names.add(m.group(1))
     return names
 
@@ -47,3 +48,13 @@ def test_argument_validation_without_gpu():
         G.counting_sort(np.array([0, 5, 2], np.int32), 3)
     with pytest.raises(G.LusError):
         G.neighbors_fluid(G.grid(2.0, 4, 4, 4), 2.0, np.array([[1.0, 1.0, 100.0]], np.float32))
+
+
+def test_profiling_slots_without_gpu():
+    # src/profiling/Profiling.hpp:75-77: 7 slots; nothing measured yet reads 0,
+    # an index outside the slots reads 0 (the reference asserts there)
+    cyc, sec = G.profile_slots()
+    assert len(cyc) == len(sec) == 7
+    assert all(c == 0 for c in cyc) and all(t == 0.0 for t in sec)
+    lib = G.load()
+    assert lib.lus_get_cycles(7) == 0 and lib.lus_get_duration(-1) == 0.0
