@@ -295,6 +295,31 @@ int lus_sand_download(LusGpu* ctx, float* pos, float* pos_star, float* vel, int*
  * src/neighbors/Neighbors.cpp:262-364), reference slot indices. */
 int lus_sand_neighbors(LusGpu* ctx, long* off, int* idx, long cap, long* total);
 
+/* --- scene construction --- */
+/* init_chunk_from_grid (src/Lustrine.cpp:676-722) on the device: grid cells
+ * X x Y x Z indexed x*Y*Z + y*Z + z (0 empty; with mask_out also cells
+ * holding exactly 1 are skipped), each occupied cell subdivided into
+ * subdivision^3 lattice points of spacing cell_size; point (x, y, z) of the
+ * subdivided lattice, in x-major order, lands at
+ * ((x * cell_size) + position) + cell_size / 2 per axis.  positions receives
+ * num_occupied * subdivision^3 points (3 floats each; num_occupied is the
+ * grid's num_occupied_grid_cells, which the chunk is sized by), the ones past
+ * the generated count zero as in the reference's chunk; colors (nullable, 4
+ * floats per point) the colour of each point's cell from cell_colors
+ * (nullable, 4 floats per cell).  *generated receives the number of points;
+ * more points than num_occupied * subdivision^3 is LUS_EINVAL (the reference
+ * would write past its vector). */
+int lus_chunk_from_grid(LusGpu* ctx, const int* cells, int X, int Y, int Z, const float* position, float cell_size,
+                        int subdivision, int mask_out, int num_occupied, const float* cell_colors, float* positions,
+                        float* colors, int* generated);
+/* init_grid_from_magika_voxel (src/VoxelLoader.cpp:47-100) after the .vox file
+ * is parsed: the model's sx*sy*sz colour indices (voxel (x, y, z) at
+ * x + y*sx + z*sx*sy, 0 empty) become grid cells (x*sy*sz + y*sz + z) holding
+ * the index, their colours (nullable, 4 floats per cell) palette[index] / 255
+ * (palette: 256 rgba bytes), *occupied the number of set voxels. */
+int lus_grid_from_voxels(LusGpu* ctx, const unsigned char* voxels, int sx, int sy, int sz,
+                         const unsigned char* palette, int* cells, float* colors, int* occupied);
+
 /* --- phase timing in the reference's profiling slots --- */
 /* The reference keeps LUSTRINE_MAX_NUM_MEASUREMENTS = 7 counters holding the
  * last measurement of each phase and reads them with get_num_observation /

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -2004,6 +2005,101 @@ int lus_sand_player_box(LusGpu* c, const float* player, const float* half_extent
         from_f4(h, pos);
     }
     *count = k;
+    return LUS_OK;
+}
+
+int lus_chunk_from_grid(LusGpu* c, const int* cells, int X, int Y, int Z, const float* position, float cell_size,
+                        int subdivision, int mask_out, int num_occupied, const float* cell_colors, float* positions,
+                        float* colors, int* generated) {
+    if (!c || !cells || !position || !generated || X <= 0 || Y <= 0 || Z <= 0 || subdivision <= 0 ||
+        num_occupied < 0)
+        return set_error(LUS_EINVAL, "lus_chunk_from_grid: bad arguments");
+    const long ncell = (long)X * Y * Z;
+    const long s3 = (long)subdivision * subdivision * subdivision;
+    const long F = ncell * s3;
+    const long cap = (long)num_occupied * s3;
+    if (F >= INT_MAX || cap >= INT_MAX || ncell >= INT_MAX)
+        return set_error(LUS_EINVAL, "lus_chunk_from_grid: %ld lattice points, at most %d", F, INT_MAX - 1);
+    if (cap > 0 && !positions) return set_error(LUS_EINVAL, "lus_chunk_from_grid: no positions buffer");
+    const bool want_col = cell_colors && colors;
+    LUS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    Arena tmp;
+    int *d_cells, *flags, *rank, *tiles;
+    float4* d_ccol = nullptr;
+    LUS_HIP(tmp.alloc(&d_cells, ncell));
+    LUS_HIP(tmp.alloc(&flags, F + 1));
+    LUS_HIP(tmp.alloc(&rank, F + 1));
+    LUS_HIP(tmp.alloc(&tiles, lus::scan_tile_count((int)F + 1) + 1));
+    LUS_HIP(hipMemcpyAsync(d_cells, cells, sizeof(int) * ncell, hipMemcpyHostToDevice, s));
+    if (want_col) {
+        std::vector<float4> hc((size_t)ncell);
+        for (long q = 0; q < ncell; ++q)
+            hc[q] = make_float4(cell_colors[4 * q], cell_colors[4 * q + 1], cell_colors[4 * q + 2], cell_colors[4 * q + 3]);
+        LUS_HIP(tmp.alloc(&d_ccol, ncell));
+        LUS_HIP(hipMemcpyAsync(d_ccol, hc.data(), sizeof(float4) * ncell, hipMemcpyHostToDevice, s));
+        LUS_HIP(hipStreamSynchronize(s));
+    }
+    LUS_HIP(hipMemsetAsync(flags + F, 0, sizeof(int), s));
+    LUS_HIP(lus::launch_chunk_flags(d_cells, X, Y, Z, subdivision, mask_out ? 1 : 0, F, flags, s));
+    LUS_HIP(lus::launch_exclusive_scan(flags, rank, (int)F + 1, tiles, 0, s));
+    int total = 0;
+    LUS_HIP(hipMemcpyAsync(&total, rank + F, sizeof(int), hipMemcpyDeviceToHost, s));
+    LUS_HIP(hipStreamSynchronize(s));
+    // the reference sizes the chunk by num_occupied_grid_cells and writes one
+    // entry per kept point: more points than that would write past its end
+    if (total > cap)
+        return set_error(LUS_EINVAL, "lus_chunk_from_grid: %d points for %ld entries (num_occupied too small)", total,
+                         cap);
+    float4 *d_pos = nullptr, *d_col = nullptr;
+    LUS_HIP(tmp.alloc(&d_pos, total));
+    if (want_col) LUS_HIP(tmp.alloc(&d_col, total));
+    LUS_HIP(lus::launch_chunk_write(d_cells, d_ccol, X, Y, Z, subdivision, mask_out ? 1 : 0, F, position, cell_size,
+                                    rank, d_pos, d_col, s));
+    std::vector<float4> h((size_t)std::max(total, 1));
+    LUS_HIP(hipMemcpyAsync(h.data(), d_pos, sizeof(float4) * total, hipMemcpyDeviceToHost, s));
+    LUS_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < total; ++q) {
+        positions[3 * (size_t)q] = h[q].x;
+        positions[3 * (size_t)q + 1] = h[q].y;
+        positions[3 * (size_t)q + 2] = h[q].z;
+    }
+    if (cap > total) memset(positions + 3 * (size_t)total, 0, sizeof(float) * 3 * (size_t)(cap - total));
+    if (want_col) {
+        LUS_HIP(hipMemcpy(h.data(), d_col, sizeof(float4) * total, hipMemcpyDeviceToHost));
+        memcpy(colors, h.data(), sizeof(float4) * (size_t)total);
+        if (cap > total) memset(colors + 4 * (size_t)total, 0, sizeof(float) * 4 * (size_t)(cap - total));
+    }
+    *generated = total;
+    return LUS_OK;
+}
+
+int lus_grid_from_voxels(LusGpu* c, const unsigned char* voxels, int sx, int sy, int sz,
+                         const unsigned char* palette, int* cells, float* colors, int* occupied) {
+    if (!c || !voxels || !palette || !cells || !occupied || sx <= 0 || sy <= 0 || sz <= 0)
+        return set_error(LUS_EINVAL, "lus_grid_from_voxels: bad arguments");
+    const long N = (long)sx * sy * sz;
+    if (N >= INT_MAX) return set_error(LUS_EINVAL, "lus_grid_from_voxels: model too large");
+    LUS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    Arena tmp;
+    unsigned char* d_vox;
+    uchar4* d_pal;
+    int *d_cells, *d_occ;
+    float4* d_col;
+    LUS_HIP(tmp.alloc(&d_vox, N));
+    LUS_HIP(tmp.alloc(&d_pal, 256));
+    LUS_HIP(tmp.alloc(&d_cells, N));
+    LUS_HIP(tmp.alloc(&d_col, N));
+    LUS_HIP(tmp.alloc(&d_occ, 1));
+    LUS_HIP(hipMemcpyAsync(d_vox, voxels, N, hipMemcpyHostToDevice, s));
+    LUS_HIP(hipMemcpyAsync(d_pal, palette, 256 * 4, hipMemcpyHostToDevice, s));
+    LUS_HIP(hipMemsetAsync(d_occ, 0, sizeof(int), s));
+    LUS_HIP(lus::launch_grid_from_voxels(d_vox, sx, sy, sz, d_pal, d_cells, d_col, d_occ, s));
+    LUS_HIP(hipMemcpyAsync(cells, d_cells, sizeof(int) * N, hipMemcpyDeviceToHost, s));
+    if (colors) LUS_HIP(hipMemcpyAsync(colors, d_col, sizeof(float4) * N, hipMemcpyDeviceToHost, s));
+    LUS_HIP(hipMemcpyAsync(occupied, d_occ, sizeof(int), hipMemcpyDeviceToHost, s));
+    LUS_HIP(hipStreamSynchronize(s));
     return LUS_OK;
 }
 

@@ -240,6 +240,17 @@ hipError_t launch_box_count(const float4* x, int n, const float* c, const float*
                             hipStream_t s);
 hipError_t launch_box_write(const float4* x, int n, const float* c, const float* h, const int* block_off, int k,
                             int* slots, float4* pos, hipStream_t s);
+// scene construction (init_chunk_from_grid): flags[t] of the F = X Y Z s^3
+// fine lattice points; after an exclusive scan into rank, chunk_write stores
+// the positions (and per-cell colours, col nullable) of the kept points
+hipError_t launch_chunk_flags(const int* cells, int X, int Y, int Z, int sub, int mask_out, long F, int* flags,
+                              hipStream_t s);
+hipError_t launch_chunk_write(const int* cells, const float4* cell_colors, int X, int Y, int Z, int sub,
+                              int mask_out, long F, const float* origin, float cs, const int* rank, float4* pos,
+                              float4* col, hipStream_t s);
+// voxel model -> grid cells and colours; *occupied += set voxels
+hipError_t launch_grid_from_voxels(const unsigned char* vox, int sx, int sy, int sz, const uchar4* palette,
+                                   int* cells, float4* colors, int* occupied, hipStream_t s);
 // a[mv[k].x] = a_old[mv[k].y] for the four arrays; t*: m staging entries each
 hipError_t launch_slot_moves(const int2* mv, int m, float4* a0, float4* a1, float4* a2, int* a3,
                              float4* t0, float4* t1, float4* t2, int* t3, hipStream_t s);

@@ -778,6 +778,109 @@ hipError_t launch_box_write(const float4* x, int n, const float* c, const float*
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ scene construction
+// init_chunk_from_grid (src/Lustrine.cpp:676-722).  Fine lattice point t, in
+// the x-major order of the reference's loops over X*s x Y*s x Z*s points,
+// belongs to grid cell (x/s, y/s, z/s) (cells indexed x*Y*Z + y*Z + z); it
+// becomes a particle when the cell is occupied (and, with mask_out, does not
+// hold exactly 1).  Ranks come from an exclusive scan of the flags.
+__device__ __forceinline__ int chunk_cell(long t, int X, int Y, int Z, int sub, int* xf, int* yf, int* zf) {
+    const long Zf = (long)Z * sub, Yf = (long)Y * sub;
+    *zf = (int)(t % Zf);
+    const long r = t / Zf;
+    *yf = (int)(r % Yf);
+    *xf = (int)(r / Yf);
+    return (*xf / sub) * Y * Z + (*yf / sub) * Z + (*zf / sub);
+}
+
+__device__ __forceinline__ bool chunk_keep(int v, int mask_out) { return v != 0 && (!mask_out || v != 1); }
+
+__global__ __launch_bounds__(kBlock) void k_chunk_flags(const int* __restrict__ cells, int X, int Y, int Z, int sub,
+                                                        int mask_out, long F, int* __restrict__ flags) {
+    const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= F) return;
+    int xf, yf, zf;
+    flags[t] = chunk_keep(cells[chunk_cell(t, X, Y, Z, sub, &xf, &yf, &zf)], mask_out) ? 1 : 0;
+}
+
+// position = ((x * cs) + grid position) + cs/2 per axis, each operation
+// rounded on its own as the reference's vec3 arithmetic does
+__global__ __launch_bounds__(kBlock) void k_chunk_write(const int* __restrict__ cells,
+                                                        const float4* __restrict__ cell_colors, int X, int Y,
+                                                        int Z, int sub, int mask_out, long F, float4 origin,
+                                                        float cs, const int* __restrict__ rank,
+                                                        float4* __restrict__ pos, float4* __restrict__ col) {
+    const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= F) return;
+    int xf, yf, zf;
+    const int c = chunk_cell(t, X, Y, Z, sub, &xf, &yf, &zf);
+    if (!chunk_keep(cells[c], mask_out)) return;
+    const int o = rank[t];
+    const float half = __fmul_rn(cs, 0.5f);
+    pos[o] = make_float4(__fadd_rn(__fadd_rn(__fmul_rn((float)xf, cs), origin.x), half),
+                         __fadd_rn(__fadd_rn(__fmul_rn((float)yf, cs), origin.y), half),
+                         __fadd_rn(__fadd_rn(__fmul_rn((float)zf, cs), origin.z), half), 0.0f);
+    if (col) col[o] = cell_colors[c];
+}
+
+hipError_t launch_chunk_flags(const int* cells, int X, int Y, int Z, int sub, int mask_out, long F, int* flags,
+                              hipStream_t s) {
+    if (F <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chunk_flags, dim3((unsigned)((F + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cells, X, Y,
+                       Z, sub, mask_out, F, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_chunk_write(const int* cells, const float4* cell_colors, int X, int Y, int Z, int sub,
+                              int mask_out, long F, const float* origin, float cs, const int* rank, float4* pos,
+                              float4* col, hipStream_t s) {
+    if (F <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chunk_write, dim3((unsigned)((F + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cells,
+                       cell_colors, X, Y, Z, sub, mask_out, F, make_float4(origin[0], origin[1], origin[2], 0.0f),
+                       cs, rank, pos, col);
+    return hipGetLastError();
+}
+
+// init_grid_from_magika_voxel (src/VoxelLoader.cpp:66-90) after parsing:
+// grid cell (x, y, z) of an sx x sy x sz model takes voxel x + y sx + z sx sy
+// (colour index, 0 empty) and the palette colour / 255; *occupied counts the
+// set voxels (zeroed by the caller)
+__global__ __launch_bounds__(kBlock) void k_grid_from_voxels(const unsigned char* __restrict__ vox, int sx, int sy,
+                                                             int sz, const uchar4* __restrict__ palette,
+                                                             int* __restrict__ cells, float4* __restrict__ colors,
+                                                             int* __restrict__ occupied) {
+    const long g = (long)blockIdx.x * kBlock + threadIdx.x;
+    const long N = (long)sx * sy * sz;
+    int hit = 0;
+    if (g < N) {
+        const int z = (int)(g % sz);
+        const long r = g / sz;
+        const int y = (int)(r % sy);
+        const int x = (int)(r / sy);
+        const int v = vox[(long)x + (long)y * sx + (long)z * sx * sy];
+        cells[g] = v;
+        float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (v) {
+            const uchar4 p = palette[v];
+            c = make_float4(__fdiv_rn((float)p.x, 255.0f), __fdiv_rn((float)p.y, 255.0f),
+                            __fdiv_rn((float)p.z, 255.0f), __fdiv_rn((float)p.w, 255.0f));
+            hit = 1;
+        }
+        colors[g] = c;
+    }
+    const unsigned long long m = __ballot(hit);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(occupied, __popcll(m));
+}
+
+hipError_t launch_grid_from_voxels(const unsigned char* vox, int sx, int sy, int sz, const uchar4* palette,
+                                   int* cells, float4* colors, int* occupied, hipStream_t s) {
+    const long N = (long)sx * sy * sz;
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grid_from_voxels, dim3((unsigned)((N + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, vox, sx,
+                       sy, sz, palette, cells, colors, occupied);
+    return hipGetLastError();
+}
+
 hipError_t launch_slot_moves(const int2* mv, int m, float4* a0, float4* a1, float4* a2, int* a3,
                              float4* t0, float4* t1, float4* t2, int* t3, hipStream_t s) {
     if (m <= 0) return hipSuccess;

@@ -118,6 +118,10 @@ _SIGS = {
     "lus_sand_sink_evict": (C.c_int, [C.c_void_p, _i, C.c_int, _i]),
     "lus_sand_player_box": (C.c_int, [C.c_void_p, _f, _f, C.c_int, _i, _f, _i]),
     "lus_profiling_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "lus_chunk_from_grid": (C.c_int, [C.c_void_p, _i, C.c_int, C.c_int, C.c_int, _f, C.c_float, C.c_int, C.c_int,
+                                      C.c_int, _f, _f, _f, _i]),
+    "lus_grid_from_voxels": (C.c_int, [C.c_void_p, _P(C.c_ubyte), C.c_int, C.c_int, C.c_int, _P(C.c_ubyte), _i, _f,
+                                       _i]),
     "lus_get_num_observation": (C.c_long, []),
     "lus_get_cycles": (C.c_longlong, [C.c_int]),
     "lus_get_duration": (C.c_double, [C.c_int]),
@@ -487,6 +491,35 @@ class Gpu:
         self.n_sand = out.value
         return out.value
 
+
+    def chunk_from_grid(self, cells, X, Y, Z, position, cell_size, subdivision, mask_out=False, num_occupied=None,
+                        colors=None):
+        """init_chunk_from_grid on the device: (positions, colors or None, generated)."""
+        c = np.ascontiguousarray(cells, np.int32).reshape(-1)
+        n_occ = int(np.count_nonzero(c)) if num_occupied is None else int(num_occupied)
+        cap = n_occ * int(subdivision) ** 3
+        pos = np.zeros((max(cap, 1), 3), np.float32)
+        cc = None if colors is None else _f32(colors, (-1, 4))
+        col = None if colors is None else np.zeros((max(cap, 1), 4), np.float32)
+        gen = C.c_int(0)
+        _check(self._lib.lus_chunk_from_grid(self._h, _ptr_i(c), X, Y, Z, _ptr_f(_f32(position, (3,))),
+                                             float(cell_size), int(subdivision), int(bool(mask_out)), n_occ,
+                                             _ptr_f(cc), _ptr_f(pos), _ptr_f(col), C.byref(gen)),
+               "lus_chunk_from_grid")
+        return pos[:cap], (None if col is None else col[:cap]), gen.value
+
+    def grid_from_voxels(self, voxels, sx, sy, sz, palette):
+        """init_grid_from_magika_voxel after parsing: (cells, colors, occupied)."""
+        v = np.ascontiguousarray(voxels, np.uint8).reshape(-1)
+        pal = np.ascontiguousarray(palette, np.uint8).reshape(-1)
+        n = sx * sy * sz
+        cells = np.zeros(n, np.int32)
+        colors = np.zeros((n, 4), np.float32)
+        occ = C.c_int(0)
+        u8 = C.POINTER(C.c_ubyte)
+        _check(self._lib.lus_grid_from_voxels(self._h, v.ctypes.data_as(u8), sx, sy, sz, pal.ctypes.data_as(u8),
+                                              _ptr_i(cells), _ptr_f(colors), C.byref(occ)), "lus_grid_from_voxels")
+        return cells, colors, occ.value
 
     def profiling(self, on=True):
         """Phase events into the reference's 7 profiling slots (see profile_slots)."""

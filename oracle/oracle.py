@@ -289,6 +289,51 @@ def player_box(positions, player, half_extent, max_out=100):
     return slots, p[slots]
 
 
+def chunk_from_grid(cells, X, Y, Z, position, cell_size, subdivision, mask_out=False, num_occupied=None,
+                    colors=None):
+    """init_chunk_from_grid (src/Lustrine.cpp:676-722): one particle per
+    subdivided lattice point of an occupied cell (cells[x*Y*Z + y*Z + z] != 0,
+    and != 1 with mask_out), in x-major order over the subdivided lattice, at
+    ((x * cs) + position) + cs/2 per axis in float32; the positions array has
+    num_occupied * subdivision^3 entries (num_occupied: the grid's
+    num_occupied_grid_cells, which the random box overcounts) and entries past
+    the generated ones stay 0.  colors: the per-cell colours, copied per
+    particle.  Returns (positions, colors or None, generated count)."""
+    cells = np.asarray(cells).reshape(X, Y, Z)
+    s = int(subdivision)
+    occ = (cells != 0) if not mask_out else ((cells != 0) & (cells != 1))
+    fine = np.repeat(np.repeat(np.repeat(occ, s, 0), s, 1), s, 2)
+    xf, yf, zf = np.nonzero(fine)  # x-major, as the loops
+    cs = np.float32(cell_size)
+    off = cs * np.float32(0.5)
+    pos = np.asarray(position, np.float32)
+    p = np.stack([(xf.astype(np.float32) * cs + pos[0]) + off, (yf.astype(np.float32) * cs + pos[1]) + off,
+                  (zf.astype(np.float32) * cs + pos[2]) + off], axis=1).astype(np.float32)
+    n_occ = int((cells != 0).sum()) if num_occupied is None else int(num_occupied)
+    total = n_occ * s ** 3
+    out = np.zeros((total, 3), np.float32)
+    out[: p.shape[0]] = p[:total]
+    col = None
+    if colors is not None:
+        c = np.asarray(colors, np.float32).reshape(X, Y, Z, 4)
+        col = np.zeros((total, 4), np.float32)
+        col[: p.shape[0]] = c[xf // s, yf // s, zf // s][:total]
+    return out, col, min(p.shape[0], total)
+
+
+def grid_from_voxels(voxels, sx, sy, sz, palette):
+    """init_grid_from_magika_voxel (src/VoxelLoader.cpp:47-100) after the file
+    is parsed: voxel (x, y, z) of a model at voxels[x + y*sx + z*sx*sy]
+    (colour index, 0 = empty) goes to grid cell x*Y*Z + y*Z + z of a grid with
+    X = sx, Y = sy, Z = sz; the cell keeps the colour index and its colour is
+    the palette's rgba / 255.  Returns (cells, colors, occupied)."""
+    v = np.asarray(voxels, np.uint8).reshape(sz, sy, sx)  # z slowest
+    cells = np.transpose(v, (2, 1, 0)).astype(np.int32)   # [x][y][z]
+    pal = np.asarray(palette, np.uint8).reshape(256, 4)
+    colors = np.where((cells != 0)[..., None], pal[cells].astype(np.float32) / np.float32(255.0), np.float32(0.0))
+    return cells.reshape(-1), colors.reshape(-1, 4).astype(np.float32), int((cells != 0).sum())
+
+
 def simulate_fluid(s, p, ps, v, lambdas, sand, solid, dt):
     lib().orc_simulate_fluid(C.byref(s), _fp(p), _fp(ps), _fp(v), _fp(lambdas),
                              sand[0], sand[1], solid[0], solid[1], dt)

@@ -467,3 +467,43 @@ def test_player_box_closed_form(oracle):
     assert len(oracle.player_box(pos, [4, 5, 6], [0.999, 1.0, 1.0])[0]) == 9
     assert oracle.player_box(pos, [4, 5, 6], [1.5, 1.5, 1.5], max_out=5)[0].tolist() == want[:5]
     assert len(oracle.player_box(pos, [40, 5, 6], [1.5, 1.5, 1.5])[0]) == 0
+
+
+# ------------------------------------------------------------------ scene construction
+def test_chunk_from_grid_closed_form(oracle):
+    # a full 2x3x2 grid, subdivision 2, cell 0.5: a 4x6x4 lattice of points at
+    # (i + 0.5) * 0.5 + position, in x-major order
+    cells = np.ones(2 * 3 * 2, np.int32)
+    pos, _, k = oracle.chunk_from_grid(cells, 2, 3, 2, (1.0, 2.0, 3.0), 0.5, 2)
+    g = [(x, y, z) for x in range(4) for y in range(6) for z in range(4)]
+    want = np.array([[x * 0.5 + 1.25, y * 0.5 + 2.25, z * 0.5 + 3.25] for x, y, z in g], np.float32)
+    assert k == 96 and np.array_equal(pos, want)
+    # empty cells are skipped; mask_out also drops cells holding exactly 1
+    cells = np.array([0, 1, 2, 0, 3, 1, 5, 0, 0, 1, 0, 7], np.int32)  # x*Y*Z + y*Z + z over 2x3x2
+    pos, _, k = oracle.chunk_from_grid(cells, 2, 3, 2, (0, 0, 0), 1.0, 1)
+    assert k == 7 and pos[:, 0].tolist() == [0.5, 0.5, 0.5, 0.5, 1.5, 1.5, 1.5]
+    pos, _, k = oracle.chunk_from_grid(cells, 2, 3, 2, (0, 0, 0), 1.0, 1, mask_out=True)
+    # mask_out keeps the array length of num_occupied * s^3 = 7: the tail is zeros
+    assert k == 4 and pos.shape == (7, 3) and np.all(pos[4:] == 0)
+    # kept: cells 2 (x0 y1 z0), 4 (x0 y2 z0), 6 (x1 y0 z0), 11 (x1 y2 z1)
+    assert pos[:4].tolist() == [[0.5, 1.5, 0.5], [0.5, 2.5, 0.5], [1.5, 0.5, 0.5], [1.5, 2.5, 1.5]]
+    # init_grid_box_random counts every cell as occupied: the tail is zeros
+    pos, _, k = oracle.chunk_from_grid(cells, 2, 3, 2, (0, 0, 0), 1.0, 2, num_occupied=12)
+    assert k == 7 * 8 and pos.shape == (96, 3) and np.all(pos[56:] == 0)
+
+
+def test_grid_from_voxels_closed_form(oracle):
+    # a 3x2x4 model: voxel (x, y, z) at x + 3y + 6z; two set voxels
+    vox = np.zeros(24, np.uint8)
+    vox[1 + 3 * 1 + 6 * 2] = 9   # (1, 1, 2)
+    vox[2 + 3 * 0 + 6 * 3] = 255  # (2, 0, 3)
+    pal = np.zeros((256, 4), np.uint8)
+    pal[9] = [255, 0, 51, 255]
+    pal[255] = [10, 20, 30, 40]
+    cells, colors, occ = oracle.grid_from_voxels(vox, 3, 2, 4, pal)
+    assert occ == 2 and cells.shape == (24,)
+    c1, c2 = 1 * 2 * 4 + 1 * 4 + 2, 2 * 2 * 4 + 0 * 4 + 3  # x*Y*Z + y*Z + z
+    assert cells[c1] == 9 and cells[c2] == 255 and np.count_nonzero(cells) == 2
+    assert colors[c1].tolist() == [1.0, 0.0, np.float32(51) / np.float32(255), 1.0]
+    assert np.array_equal(colors[c2], np.array([10, 20, 30, 40], np.float32) / np.float32(255))
+    assert np.all(colors[[i for i in range(24) if i not in (c1, c2)]] == 0)
