@@ -282,6 +282,10 @@ lus::PbfK pbf_constants(const LusPbfParams& p) {
     volatile float tq3 = tq2 * tq;
     k.inv_tq3 = 1.0f / tq3;
     k.scorr_k = p.scorr_k;
+    const double iq = (double)k.inv_tq3;
+    k.f_scorr12 = (float)(-(double)p.scorr_k * iq * iq * iq * iq);
+    k.f_spf_neg_inv_r0 = (float)((double)k.SPf * (double)k.neg_inv_r0);
+    k.f_mr0sq_spf2 = (float)((double)k.mr0sq * (double)k.SPf * (double)k.SPf);
     k.n_scorr = p.scorr_n;
     k.cW = p.c_xsph * P6;
     k.eps = p.epsilon;

@@ -37,6 +37,9 @@ struct PbfK {
     const int* ph;              // phase (0 fluid, 1 sand, 2 solid), input order
     int* ph_out;                // phase in cell order
     float mix_d, mix_collision, mix_friction, mix_mu_s, mix_mu_k;
+    // fused mode: spiky's SPf and s_corr's 1 / t(dq)^3 moved out of the
+    // neighbour loops: -scorr_k / t(dq)^12, SPf * neg_inv_r0, mr0sq * SPf^2
+    float f_scorr12, f_spf_neg_inv_r0, f_mr0sq_spf2;
 };
 
 // simulate_sand constants (src/Simulate.cpp:159-163 and the Simulation fields).

@@ -670,12 +670,55 @@ __device__ __forceinline__ void block_max_sum_store(float cmax, float csum, int*
 // per slot (flat) or one tile per block with LDS staging (tiled, suffix _t).
 // `visit(f)` calls f(j, cur_j, aux_j) over N(i) in ascending slot order.
 
+// A uniform kernel constant held in a vector register for the neighbour loop
+// (an instruction takes one scalar operand: a second one would be copied to
+// a vector register on every iteration).
+__device__ __forceinline__ float vreg(float x) {
+    float v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+    return v;
+}
+
+// Fused-mode spiky factor without SPf: g^2 / r with g = hf - f r,
+// from rinv = 1 / sqrt(r2) (v_rsq_f32)
+__device__ __forceinline__ float spiky_unit(float r2, float mf, float hf) {
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float g = __builtin_fmaf(mf, r2 * rinv, hf);
+    return g * (g * rinv);
+}
+
 // lambda pass: density constraint and its gradient norm (PBF eq. 1, 8, 11)
 template <bool kFast, class Visit>
 __device__ __forceinline__ float lambda_body(int i, const float4* __restrict__ xa, float4* __restrict__ xl,
                                              const PbfK& k, Visit&& visit) {
     const float4 pi = xa[i];
     float rho = 0.0f, a = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    if constexpr (kFast) {
+        // SPf leaves the sums: a and |grad|^2 scale by SPf^2 (f_mr0sq_spf2)
+        const float mf = vreg(-k.f), hf = vreg(k.hf), mf2 = vreg(-k.f2), hf2 = vreg(k.hf2);
+        visit([&](int, const float4 pj, const float4) {
+            const float dx = pi.x - pj.x;
+            const float dy = pi.y - pj.y;
+            const float dz = pi.z - pj.z;
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            if (r2 <= k.h2) {
+                const float t = __builtin_fmaf(mf2, r2, hf2);
+                rho = __builtin_fmaf(t * t, t, rho);
+                if (r2 > 0.0f) {
+                    const float s = spiky_unit(r2, mf, hf);
+                    a = __builtin_fmaf(s * s, r2, a);
+                    gx = __builtin_fmaf(s, dx, gx);
+                    gy = __builtin_fmaf(s, dy, gy);
+                    gz = __builtin_fmaf(s, dz, gz);
+                }
+            }
+        });
+        const float Ci = rho * k.c_rho - 1.0f;
+        const float gg = gx * gx + gy * gy + gz * gz;
+        const float den = k.f_mr0sq_spf2 * (a + gg) + k.eps;
+        xl[i] = make_float4(pi.x, pi.y, pi.z, -Ci / den);
+        return Ci;
+    }
     visit([&](int, const float4 pj, const float4) {
         const float dx = pi.x - pj.x;
         const float dy = pi.y - pj.y;
@@ -791,7 +834,30 @@ __device__ __forceinline__ void deltap_body(int i, const float4* __restrict__ xl
     const float4 pi = xl[i];
     const float li = pi.w;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    visit([&](int, const float4 pj, const float4) {
+    float step = k.neg_inv_r0;
+    if constexpr (kFast && kNS == 4) {
+        // SPf leaves the sums (f_spf_neg_inv_r0); s_corr = -k (t / t(dq))^12
+        // as one constant times t^12 (f_scorr12)
+        const float mf = vreg(-k.f), hf = vreg(k.hf), mf2 = vreg(-k.f2), hf2 = vreg(k.hf2);
+        visit([&](int, const float4 pj, const float4) {
+            const float dx = pi.x - pj.x;
+            const float dy = pi.y - pj.y;
+            const float dz = pi.z - pj.z;
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            if (r2 <= k.h2 && r2 > 0.0f) {
+                const float s = spiky_unit(r2, mf, hf);
+                const float t = __builtin_fmaf(mf2, r2, hf2);
+                const float t2 = t * t;
+                const float t4 = t2 * t2;
+                const float t12 = (t4 * t4) * t4;
+                const float c = __builtin_fmaf(k.f_scorr12, t12, li + pj.w) * s;
+                ax = __builtin_fmaf(c, dx, ax);
+                ay = __builtin_fmaf(c, dy, ay);
+                az = __builtin_fmaf(c, dz, az);
+            }
+        });
+        step = k.f_spf_neg_inv_r0;
+    } else visit([&](int, const float4 pj, const float4) {
         const float dx = pi.x - pj.x;
         const float dy = pi.y - pj.y;
         const float dz = pi.z - pj.z;
@@ -819,9 +885,9 @@ __device__ __forceinline__ void deltap_body(int i, const float4* __restrict__ xl
     });
     const Bounds b = bounds_of(k, k.scenes ? k.sid_out[i] : 0);
     float4 r;
-    r.x = rclamp(pi.x + ax * k.neg_inv_r0, b.lo[0], b.hi[0]);
-    r.y = rclamp(pi.y + ay * k.neg_inv_r0, b.lo[1], b.hi[1]);
-    r.z = rclamp(pi.z + az * k.neg_inv_r0, b.lo[2], b.hi[2]);
+    r.x = rclamp(pi.x + ax * step, b.lo[0], b.hi[0]);
+    r.y = rclamp(pi.y + ay * step, b.lo[1], b.hi[1]);
+    r.z = rclamp(pi.z + az * step, b.lo[2], b.hi[2]);
     r.w = 0.0f;
     xa[i] = r;
     if (vout) {
