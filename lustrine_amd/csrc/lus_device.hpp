@@ -39,25 +39,31 @@ __device__ __forceinline__ int cell_of(float x, float y, float z, float cs, int 
     return ((int)py) * gX * gZ + ((int)px) * gZ + ((int)pz);
 }
 
-// Histogram slot of key c: one atomicAdd per distinct key of the wave instead
-// of one per lane (particles arrive in the previous step's cell order, so a
-// wave holds few distinct keys).  Every lane of the wave must call it.
+// Histogram slot of key c: one atomicAdd per run of equal keys in consecutive
+// active lanes instead of one per lane (particles arrive in the previous
+// step's cell order, so a wave holds few runs).  The run heads issue their
+// atomics in one instruction and the wave waits once; each lane then takes its
+// head's base plus its rank inside the run.  A key split over several runs
+// gets one disjoint range per run.  Every lane of the wave must call it.
 __device__ __forceinline__ int wave_atomic_slot(int* __restrict__ counts, int c, bool active) {
     const int lane = threadIdx.x & 63;
-    unsigned long long pending = __ballot(active);
-    int slot = 0;
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const int lk = __shfl(c, leader, 64);
-        const bool mine = active && c == lk;
-        const unsigned long long same = __ballot(mine);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&counts[lk], __popcll(same));
-        base = __shfl(base, leader, 64);
-        if (mine) slot = base + __popcll(same & ((1ull << lane) - 1ull));
-        pending &= ~same;
+    const unsigned long long act = __ballot(active);
+    const int pc = __shfl_up(c, 1, 64);
+    const bool pact = lane > 0 && ((act >> (lane - 1)) & 1ull);
+    const bool head = active && (!pact || pc != c);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long below = (1ull << lane) - 1ull;  // lanes < lane
+    const unsigned long long upto = (2ull << lane) - 1ull;   // lanes <= lane (all for 63)
+    int base = 0;
+    if (head) {
+        const unsigned long long after = heads & ~upto;
+        const unsigned long long end = after ? ((1ull << (__ffsll(after) - 1)) - 1ull) : ~0ull;
+        base = atomicAdd(&counts[c], __popcll(act & ~below & end));
     }
-    return slot;
+    const unsigned long long mine = heads & upto;
+    const int h = mine ? 63 - __clzll(mine) : lane;
+    const int b = __shfl(base, h, 64);
+    return b + __popcll(act & below & ~((1ull << h) - 1ull));
 }
 
 // float -> int preserving order, for atomicMax on floats

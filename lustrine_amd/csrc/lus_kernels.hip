@@ -26,8 +26,15 @@
 namespace lus {
 
 // ------------------------------------------------------------------ scan
+// Two passes over tiles of kScanTile ints: k_scan_reduce writes each tile's
+// sum, k_scan_down rescans the tile and adds the sum of all earlier tiles.
+// Up to kScanFuseTiles tiles every down block sums the earlier tile sums
+// itself (at most 16 KB, L2 resident); above that k_scan_tiles scans them
+// first in one block.  Each thread owns 8 consecutive ints, moved as two
+// 16-byte accesses when the arrays are 16-byte aligned.
 constexpr int kScanV = 8;
 constexpr int kScanTile = kBlock * kScanV;
+constexpr int kScanFuseTiles = 4096;
 
 int scan_tile_count(int m) { return (m + kScanTile - 1) / kScanTile; }
 
@@ -56,13 +63,29 @@ __device__ __forceinline__ int block_excl_scan(int v, int* lds, int* total) {
     return woff + incl - v;
 }
 
+template <bool kVec>
+__device__ __forceinline__ void scan_load8(const int* __restrict__ in, int base, int m, int v[kScanV]) {
+    if (kVec && base + kScanV <= m) {
+        const int4 a = *reinterpret_cast<const int4*>(in + base);
+        const int4 b = *reinterpret_cast<const int4*>(in + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanV; ++k) v[k] = (base + k < m) ? in[base + k] : 0;
+    }
+}
+
+template <bool kVec>
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int* __restrict__ in, int m,
                                                          int* __restrict__ tile_sums) {
     __shared__ int lds[16];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanV;
+    int v[kScanV];
+    scan_load8<kVec>(in, base, m, v);
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanV; ++k) s += (base + k < m) ? in[base + k] : 0;
+    for (int k = 0; k < kScanV; ++k) s += v[k];
     int tot;
     block_excl_scan(s, lds, &tot);
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
@@ -83,40 +106,84 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(int* __restrict__ tile_sums
     if (threadIdx.x == 0) tile_sums[nt] = carry;
 }
 
+// kFuse: tile_sums holds plain tile sums and the block adds those of tiles
+// 0..blockIdx.x-1; otherwise tile_sums holds their exclusive scan.
+template <bool kVec, bool kFuse>
 __global__ __launch_bounds__(kBlock) void k_scan_down(int* in, int m, const int* __restrict__ tile_sums,
                                                        int* out, int zero_in) {
     __shared__ int lds[16];
+    __shared__ int pre_lds[kBlock / 64];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanV;
     int v[kScanV];
+    scan_load8<kVec>(in, base, m, v);
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanV; ++k) {
-        v[k] = (base + k < m) ? in[base + k] : 0;
-        s += v[k];
+    for (int k = 0; k < kScanV; ++k) s += v[k];
+    int pre;
+    if (kFuse) {
+        int t = 0;
+        for (int q = threadIdx.x; q < (int)blockIdx.x; q += kBlock) t += tile_sums[q];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
+        if ((threadIdx.x & 63) == 0) pre_lds[threadIdx.x >> 6] = t;
+        // block_excl_scan's first barrier publishes pre_lds
     }
     int tot;
-    int e = block_excl_scan(s, lds, &tot) + tile_sums[blockIdx.x];
+    int e = block_excl_scan(s, lds, &tot);
+    if (kFuse) {
+        pre = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) pre += pre_lds[w];
+    } else {
+        pre = tile_sums[blockIdx.x];
+    }
+    e += pre;
+    int o[kScanV];
 #pragma unroll
     for (int k = 0; k < kScanV; ++k) {
-        if (base + k < m) {
-            out[base + k] = e;
-            if (zero_in) in[base + k] = 0;
-        }
+        o[k] = e;
         e += v[k];
+    }
+    if (kVec && base + kScanV <= m) {
+        *reinterpret_cast<int4*>(out + base) = make_int4(o[0], o[1], o[2], o[3]);
+        *reinterpret_cast<int4*>(out + base + 4) = make_int4(o[4], o[5], o[6], o[7]);
+        if (zero_in) {
+            *reinterpret_cast<int4*>(in + base) = make_int4(0, 0, 0, 0);
+            *reinterpret_cast<int4*>(in + base + 4) = make_int4(0, 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanV; ++k) {
+            if (base + k < m) {
+                out[base + k] = o[k];
+                if (zero_in) in[base + k] = 0;
+            }
+        }
+    }
+}
+
+template <bool kVec>
+static void scan_launch(int* in, int* out, int m, int* tile_sums, int zero_in, hipStream_t s) {
+    const int nt = scan_tile_count(m);
+    hipLaunchKernelGGL(k_scan_reduce<kVec>, dim3(nt), dim3(kBlock), 0, s, in, m, tile_sums);
+    if (nt <= kScanFuseTiles) {
+        hipLaunchKernelGGL((k_scan_down<kVec, true>), dim3(nt), dim3(kBlock), 0, s, in, m,
+                           tile_sums, out, zero_in);
+    } else {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tile_sums, nt);
+        hipLaunchKernelGGL((k_scan_down<kVec, false>), dim3(nt), dim3(kBlock), 0, s, in, m,
+                           tile_sums, out, zero_in);
     }
 }
 
 hipError_t launch_exclusive_scan(const int* in, int* out, int m, int* tile_sums, int zero_in,
                                  hipStream_t s) {
-    const int nt = scan_tile_count(m);
-    if (nt > 0) {
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(kBlock), 0, s, in, m, tile_sums);
-    }
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tile_sums, nt);
-    if (nt > 0) {
-        hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(kBlock), 0, s, const_cast<int*>(in), m,
-                           tile_sums, out, zero_in);
-    }
+    if (m <= 0) return hipSuccess;
+    // out may alias in (only when zero_in is 0: every thread reads its 8 ints
+    // before it writes them)
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    if (vec) scan_launch<true>(const_cast<int*>(in), out, m, tile_sums, zero_in, s);
+    else scan_launch<false>(const_cast<int*>(in), out, m, tile_sums, zero_in, s);
     return hipGetLastError();
 }
 

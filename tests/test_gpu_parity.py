@@ -60,7 +60,10 @@ def test_counting_sort_reference_unit_test(gpu_mod, oracle):
     assert np.array_equal(keys[perm], np.sort(keys))
 
 
-@pytest.mark.parametrize("n,c", [(0, 7), (1, 1), (5000, 1), (33, 2), (100000, 4096)])
+# the last two: 4096 scan tiles of 2048 cells (down blocks sum the tile sums
+# themselves) and 4395 tiles (the tile sums are scanned by a separate kernel)
+@pytest.mark.parametrize("n,c", [(0, 7), (1, 1), (5000, 1), (33, 2), (100000, 4096),
+                                 (200000, 8388607), (200000, 9000001)])
 def test_counting_sort_edges(gpu_mod, oracle, n, c):
     keys = np.random.default_rng(n).integers(0, c, n).astype(np.int32)
     perm, counts = gpu_mod.counting_sort(keys, c)
