@@ -175,9 +175,11 @@ int lus_pbf_download(LusGpu* ctx, int order, float* pos, float* vel, int* ids);
 int lus_pbf_last_max_constraint(LusGpu* ctx, float* out);
 /* Runs warmup + steps substeps; *ms = elapsed of the `steps` (hipEvents on the
  * context stream, one before and one after).  When phase_ms is non-NULL
- * (array of LUS_PBF_NPHASES) the `steps` instead record an event after every
- * kernel: phase_ms accumulates per-phase device time, phase_launches the
- * launch count per phase, and *ms their sum. */
+ * (array of LUS_PBF_NPHASES) the `steps` instead run with plain launches and
+ * every kernel timed by an event pair stamped at its own dispatch's start and
+ * end (hipExtLaunchKernelGGL; the gaps between kernels are not counted):
+ * phase_ms accumulates per-phase kernel time, phase_launches the number of
+ * times each phase ran, and *ms the sum of all kernel times. */
 #define LUS_PBF_NPHASES 12
 /* phases: 0 predict+cell key, 1 cell scan, 2 scatter, 3 stable permute,
  *         4 lambda pass (iterations 2..K), 5 delta-p pass, 6 XSPH + commit,

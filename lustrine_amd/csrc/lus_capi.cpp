@@ -324,6 +324,9 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         if (r != hipSuccess) return r;
         ev->push_back(e);
         ev_phase->push_back(phase);
+        // label the kernels timed since the previous mark (per-kernel timing)
+        lus::KernelTimes& kt = lus::kernel_times();
+        if (kt.on) kt.phase.resize(kt.start.size(), phase);
         return hipEventRecord(e, s);
     };
     // per-step constants: the scene-id buffers of a batch ping-pong like id
@@ -1165,21 +1168,46 @@ int lus_pbf_bench(LusGpu* c, float dt, int warmup, int steps, float* ms, float* 
             phase_ms[k] = 0.0f;
             if (phase_launches) phase_launches[k] = 0;
         }
-        for (int s = 0; s < steps; ++s) {
-            std::vector<hipEvent_t> ev;
-            std::vector<int> ph;
-            r = pbf_enqueue(c, dt, &ev, &ph);
-            if (r) return r;
-            LUS_HIP(hipEventSynchronize(ev.back()));
-            for (size_t k = 1; k < ev.size(); ++k) {
-                float t = 0.0f;
-                LUS_HIP(hipEventElapsedTime(&t, ev[k - 1], ev[k]));
-                phase_ms[ph[k]] += t;
-                *ms += t;
-                if (phase_launches) phase_launches[ph[k]] += 1;
-            }
-            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        // All `steps` are enqueued before one synchronisation, so the host
+        // stays ahead of the device as in the untimed run.  Every kernel is
+        // timed by an event pair stamped at its own dispatch's start and end
+        // (these agree with a kernel trace's durations; events recorded
+        // between launches would add the dispatch gaps, 2-4 us per kernel);
+        // the phase marks label the kernels and count the phases.
+        lus::KernelTimes& kt = lus::kernel_times();
+        std::vector<std::vector<hipEvent_t>> ev((size_t)steps);
+        std::vector<std::vector<int>> ph((size_t)steps);
+        kt.start.clear();
+        kt.stop.clear();
+        kt.phase.clear();
+        kt.failed = false;
+        for (int s = 0; s < steps && !r; ++s) {
+            kt.on = true;
+            r = pbf_enqueue(c, dt, &ev[s], &ph[s]);
+            kt.on = false;
+            kt.phase.resize(kt.start.size(), -1);
         }
+        if (!r && kt.failed) r = set_error(LUS_EHIP, "lus_pbf_bench: kernel event");
+        if (!r && hipStreamSynchronize(c->stream) != hipSuccess) r = set_error(LUS_EHIP, "lus_pbf_bench: sync");
+        for (size_t q = 0; !r && q < kt.start.size(); ++q) {
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, kt.start[q], kt.stop[q]) != hipSuccess) {
+                r = set_error(LUS_EHIP, "lus_pbf_bench: elapsed time");
+                break;
+            }
+            if (kt.phase[q] >= 0) phase_ms[kt.phase[q]] += t;
+            *ms += t;
+        }
+        for (int s = 0; !r && phase_launches && s < steps; ++s)
+            for (size_t k = 1; k < ph[s].size(); ++k) phase_launches[ph[s][k]] += 1;
+        for (auto& v : ev)
+            for (hipEvent_t e : v) (void)hipEventDestroy(e);
+        for (hipEvent_t e : kt.start) (void)hipEventDestroy(e);
+        for (hipEvent_t e : kt.stop) (void)hipEventDestroy(e);
+        kt.start.clear();
+        kt.stop.clear();
+        kt.phase.clear();
+        if (r) return r;
     }
     return LUS_OK;
 }

@@ -6,10 +6,23 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "lustrine_gpu.h"
 
 namespace lus {
+
+// Per-kernel timing of the calling host thread.  While `on`, every kernel the
+// library launches records an event pair stamped at that dispatch's start and
+// end (start[q], stop[q]; phase[q] is the caller's label); `failed` is set if
+// an event could not be created (that launch then goes untimed).
+struct KernelTimes {
+    bool on = false;
+    bool failed = false;
+    std::vector<hipEvent_t> start, stop;
+    std::vector<int> phase;
+};
+KernelTimes& kernel_times();
 
 // Derived constants of the PBF substep, all float32, computed once on the host
 // (pbf_constants in lus_capi.cpp).  The kernels evaluate

@@ -22,8 +22,14 @@
 
 #include "lus_device.hpp"
 #include "lus_internal.hpp"
+#include "lus_launch.hpp"
 
 namespace lus {
+
+KernelTimes& kernel_times() {
+    static thread_local KernelTimes t;
+    return t;
+}
 
 // ------------------------------------------------------------------ scan
 // Two passes over tiles of kScanTile ints: k_scan_reduce writes each tile's
@@ -165,13 +171,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(int* in, int m, const int*
 template <bool kVec>
 static void scan_launch(int* in, int* out, int m, int* tile_sums, int zero_in, hipStream_t s) {
     const int nt = scan_tile_count(m);
-    hipLaunchKernelGGL(k_scan_reduce<kVec>, dim3(nt), dim3(kBlock), 0, s, in, m, tile_sums);
+    launch_kernel(k_scan_reduce<kVec>, dim3(nt), dim3(kBlock), 0, s, in, m, tile_sums);
     if (nt <= kScanFuseTiles) {
-        hipLaunchKernelGGL((k_scan_down<kVec, true>), dim3(nt), dim3(kBlock), 0, s, in, m,
+        launch_kernel((k_scan_down<kVec, true>), dim3(nt), dim3(kBlock), 0, s, in, m,
                            tile_sums, out, zero_in);
     } else {
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tile_sums, nt);
-        hipLaunchKernelGGL((k_scan_down<kVec, false>), dim3(nt), dim3(kBlock), 0, s, in, m,
+        launch_kernel(k_scan_tiles, dim3(1), dim3(1024), 0, s, tile_sums, nt);
+        launch_kernel((k_scan_down<kVec, false>), dim3(nt), dim3(kBlock), 0, s, in, m,
                            tile_sums, out, zero_in);
     }
 }
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_keys(const float4* __restrict__
 }
 
 hipError_t launch_cell_keys(const float4* pos, int n, LusGrid g, int* key, hipStream_t s) {
-    hipLaunchKernelGGL(k_cell_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, pos, n, g, key);
+    launch_kernel(k_cell_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, pos, n, g, key);
     return hipGetLastError();
 }
 
@@ -212,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_count_keys(const int* __restrict__ k
 }
 
 hipError_t launch_count_keys(const int* key, int n, int* counts, int* slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_count_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, key, n, counts, slot);
+    launch_kernel(k_count_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, key, n, counts, slot);
     return hipGetLastError();
 }
 
@@ -227,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const int* __restrict__ key,
 
 hipError_t launch_scatter(const int* key, const int* slot, const int* cell_start, int n, int* tmp,
                           hipStream_t s) {
-    hipLaunchKernelGGL(k_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, key, slot, cell_start, n,
+    launch_kernel(k_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, key, slot, cell_start, n,
                        tmp);
     return hipGetLastError();
 }
@@ -248,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_stable_rank(const int* __restrict__ 
 
 hipError_t launch_stable_rank(const int* tmp, const int* key, const int* cell_start, int n,
                               int* perm, int* key_sorted, hipStream_t s) {
-    hipLaunchKernelGGL(k_stable_rank, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start, n,
+    launch_kernel(k_stable_rank, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start, n,
                        perm, key_sorted);
     return hipGetLastError();
 }
@@ -262,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_gather4(const float4* __restrict__ s
 }
 
 hipError_t launch_gather4(const float4* src, const int* perm, int n, float4* dst, hipStream_t s) {
-    hipLaunchKernelGGL(k_gather4, dim3(grid_for(n)), dim3(kBlock), 0, s, src, perm, n, dst);
+    launch_kernel(k_gather4, dim3(grid_for(n)), dim3(kBlock), 0, s, src, perm, n, dst);
     return hipGetLastError();
 }
 
@@ -275,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_gather1(const int* __restrict__ src,
 }
 
 hipError_t launch_gather1(const int* src, const int* perm, int n, int* dst, hipStream_t s) {
-    hipLaunchKernelGGL(k_gather1, dim3(grid_for(n)), dim3(kBlock), 0, s, src, perm, n, dst);
+    launch_kernel(k_gather1, dim3(grid_for(n)), dim3(kBlock), 0, s, src, perm, n, dst);
     return hipGetLastError();
 }
 
@@ -299,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(const float4* __restrict
 hipError_t launch_stream_copy(const float4* src, float4* dst, size_t n4, hipStream_t s) {
     size_t blocks = (n4 + (size_t)kBlock * 4 - 1) / ((size_t)kBlock * 4);
     if (blocks > 256 * 64) blocks = 256 * 64;
-    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(kBlock), 0, s, src, dst, n4);
+    launch_kernel(k_stream_copy, dim3((unsigned)blocks), dim3(kBlock), 0, s, src, dst, n4);
     return hipGetLastError();
 }
 
@@ -368,7 +374,7 @@ hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, cons
                             const float4* solid_sorted, const int* solid_cs, const int* solid_ref,
                             LusGrid g, float h2, const int* inv_perm, const long* off, int* cnt,
                             int* idx, int fill, hipStream_t s) {
-    hipLaunchKernelGGL(k_fluid_nbr, dim3(grid_for(n_sand)), dim3(kBlock), 0, s, sand_sorted,
+    launch_kernel(k_fluid_nbr, dim3(grid_for(n_sand)), dim3(kBlock), 0, s, sand_sorted,
                        sand_key, sand_cs, sand_orig, n_sand, sand_start, solid_sorted, solid_cs,
                        solid_ref, g, h2, inv_perm, off, cnt, idx, fill);
     return hipGetLastError();
@@ -450,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_sand_predict_key(float4* __restrict_
 
 hipError_t launch_sand_predict_key(float4* x, float4* v, int* att, int n, SandK k, float dt,
                                    float4* xs, int* key, int* slot, int* counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_sand_predict_key, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, att, n, k,
+    launch_kernel(k_sand_predict_key, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, att, n, k,
                        dt, xs, key, slot, counts);
     return hipGetLastError();
 }
@@ -480,7 +486,7 @@ hipError_t launch_sand_permute(const int* tmp, const int* key, const int* cell_s
                                const float4* x, const float4* xs, const float4* v, const int* att,
                                int n, float4* x2, float4* xs2, float4* v2, int* att2, int* perm,
                                int* key2, hipStream_t s) {
-    hipLaunchKernelGGL(k_sand_permute, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start,
+    launch_kernel(k_sand_permute, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start,
                        x, xs, v, att, n, x2, xs2, v2, att2, perm, key2);
     return hipGetLastError();
 }
@@ -565,7 +571,7 @@ __global__ __launch_bounds__(kBlock) void k_sand_nbr(const float4* __restrict__ 
 hipError_t launch_sand_nbr(const float4* xs, const int* key2, const int* sand_cs, int n,
                            const float4* solid, const int* solid_cs, SandK k, const int* off,
                            int* cnt, int* cnt1, int* idx, int fill, hipStream_t s) {
-    hipLaunchKernelGGL(k_sand_nbr, dim3(grid_for(n)), dim3(kBlock), 0, s, xs, key2, sand_cs, n, solid,
+    launch_kernel(k_sand_nbr, dim3(grid_for(n)), dim3(kBlock), 0, s, xs, key2, sand_cs, n, solid,
                        solid_cs, k, off, cnt, cnt1, idx, fill);
     return hipGetLastError();
 }
@@ -662,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restric
 hipError_t launch_sand_substep(const float4* A, const float4* xold, const float4* solid,
                                const int* off, const int* idx, int n, SandK k, float4* B,
                                hipStream_t s) {
-    hipLaunchKernelGGL(k_sand_substep, dim3(grid_for(n)), dim3(kBlock), 0, s, A, xold, solid, off,
+    launch_kernel(k_sand_substep, dim3(grid_for(n)), dim3(kBlock), 0, s, A, xold, solid, off,
                        idx, n, k, B);
     return hipGetLastError();
 }
@@ -682,7 +688,7 @@ __global__ __launch_bounds__(kBlock) void k_sand_vupdate(const float4* __restric
 
 hipError_t launch_sand_vupdate(const float4* A, float4* xold_to_new, float4* vout, int n, float dt,
                                hipStream_t s) {
-    hipLaunchKernelGGL(k_sand_vupdate, dim3(grid_for(n)), dim3(kBlock), 0, s, A, xold_to_new, vout,
+    launch_kernel(k_sand_vupdate, dim3(grid_for(n)), dim3(kBlock), 0, s, A, xold_to_new, vout,
                        n, dt);
     return hipGetLastError();
 }
@@ -725,7 +731,7 @@ hipError_t launch_cell_box_count(const int* cs, const int* ocs, const int* solid
                                  hipStream_t s) {
     const int rows = (hi[1] - lo[1] + 1) * (hi[0] - lo[0] + 1);
     const int4 l = make_int4(lo[0], lo[1], lo[2], 0), h = make_int4(hi[0], hi[1], hi[2], 0);
-    hipLaunchKernelGGL(k_cell_box_count, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cs, ocs,
+    launch_kernel(k_cell_box_count, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cs, ocs,
                        solid_ref, l, h, gX, gZ, sand_end, all_ids, out);
     return hipGetLastError();
 }
@@ -742,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_ranges(const int* __restrict__ 
 
 hipError_t launch_cell_ranges(const int* cs, const int* cells, int n, int2* out, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cell_ranges, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cs, cells, n,
+    launch_kernel(k_cell_ranges, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, cs, cells, n,
                        out);
     return hipGetLastError();
 }
@@ -831,7 +837,7 @@ hipError_t launch_box_count(const float4* x, int n, const float* c, const float*
                             hipStream_t s) {
     const int nb = (n + kBlock - 1) / kBlock;
     if (nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_box_count, dim3(nb), dim3(kBlock), 0, s, x, n, make_float4(c[0], c[1], c[2], 0.0f),
+    launch_kernel(k_box_count, dim3(nb), dim3(kBlock), 0, s, x, n, make_float4(c[0], c[1], c[2], 0.0f),
                        make_float4(h[0], h[1], h[2], 0.0f), block_counts);
     return hipGetLastError();
 }
@@ -840,7 +846,7 @@ hipError_t launch_box_write(const float4* x, int n, const float* c, const float*
                             int* slots, float4* pos, hipStream_t s) {
     const int nb = (n + kBlock - 1) / kBlock;
     if (nb == 0 || k <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_box_write, dim3(nb), dim3(kBlock), 0, s, x, n, make_float4(c[0], c[1], c[2], 0.0f),
+    launch_kernel(k_box_write, dim3(nb), dim3(kBlock), 0, s, x, n, make_float4(c[0], c[1], c[2], 0.0f),
                        make_float4(h[0], h[1], h[2], 0.0f), block_off, k, slots, pos);
     return hipGetLastError();
 }
@@ -893,7 +899,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk_write(const int* __restrict__ 
 hipError_t launch_chunk_flags(const int* cells, int X, int Y, int Z, int sub, int mask_out, long F, int* flags,
                               hipStream_t s) {
     if (F <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_chunk_flags, dim3((unsigned)((F + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cells, X, Y,
+    launch_kernel(k_chunk_flags, dim3((unsigned)((F + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cells, X, Y,
                        Z, sub, mask_out, F, flags);
     return hipGetLastError();
 }
@@ -902,7 +908,7 @@ hipError_t launch_chunk_write(const int* cells, const float4* cell_colors, int X
                               int mask_out, long F, const float* origin, float cs, const int* rank, float4* pos,
                               float4* col, hipStream_t s) {
     if (F <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_chunk_write, dim3((unsigned)((F + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cells,
+    launch_kernel(k_chunk_write, dim3((unsigned)((F + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, cells,
                        cell_colors, X, Y, Z, sub, mask_out, F, make_float4(origin[0], origin[1], origin[2], 0.0f),
                        cs, rank, pos, col);
     return hipGetLastError();
@@ -943,7 +949,7 @@ hipError_t launch_grid_from_voxels(const unsigned char* vox, int sx, int sy, int
                                    int* cells, float4* colors, int* occupied, hipStream_t s) {
     const long N = (long)sx * sy * sz;
     if (N <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_grid_from_voxels, dim3((unsigned)((N + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, vox, sx,
+    launch_kernel(k_grid_from_voxels, dim3((unsigned)((N + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, vox, sx,
                        sy, sz, palette, cells, colors, occupied);
     return hipGetLastError();
 }
@@ -952,8 +958,8 @@ hipError_t launch_slot_moves(const int2* mv, int m, float4* a0, float4* a1, floa
                              float4* t0, float4* t1, float4* t2, int* t3, hipStream_t s) {
     if (m <= 0) return hipSuccess;
     const dim3 g((m + kBlock - 1) / kBlock), b(kBlock);
-    hipLaunchKernelGGL(k_moves_gather, g, b, 0, s, mv, m, a0, a1, a2, a3, t0, t1, t2, t3);
-    hipLaunchKernelGGL(k_moves_scatter, g, b, 0, s, mv, m, a0, a1, a2, a3, t0, t1, t2, t3);
+    launch_kernel(k_moves_gather, g, b, 0, s, mv, m, a0, a1, a2, a3, t0, t1, t2, t3);
+    launch_kernel(k_moves_scatter, g, b, 0, s, mv, m, a0, a1, a2, a3, t0, t1, t2, t3);
     return hipGetLastError();
 }
 

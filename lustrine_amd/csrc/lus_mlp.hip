@@ -17,6 +17,7 @@
 
 #include "lus_device.hpp"
 #include "lus_internal.hpp"
+#include "lus_launch.hpp"
 
 namespace lus {
 
@@ -58,7 +59,7 @@ __global__ void k_mlp_pack(const float* __restrict__ p, bf16x8* __restrict__ w) 
 }
 
 hipError_t launch_mlp_pack(const float* params, void* packed, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_pack, dim3(4), dim3(256), 0, s, params, (bf16x8*)packed);
+    launch_kernel(k_mlp_pack, dim3(4), dim3(256), 0, s, params, (bf16x8*)packed);
     return hipGetLastError();
 }
 
@@ -167,7 +168,7 @@ hipError_t launch_mlp_forward(const void* X, int n, const float* params, const v
     int blocks = (ntiles + kMlpWaves - 1) / kMlpWaves;
     if (blocks > 256 * 16) blocks = 256 * 16;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(64 * kMlpWaves), 0, s, (const bf16x8*)X, n,
+    launch_kernel(k_mlp_forward, dim3(blocks), dim3(64 * kMlpWaves), 0, s, (const bf16x8*)X, n,
                        params, (const bf16x8*)packed, out_scale, out, xp, xa, k);
     return hipGetLastError();
 }
@@ -218,7 +219,7 @@ hipError_t launch_gemm_bf16(const void* A, long sam, long sak, const void* B, lo
     if (kchunk <= 0 || kchunk % 32) return hipErrorInvalidValue;
     const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
     const int chunks = (K + kchunk - 1) / kchunk;
-    hipLaunchKernelGGL(k_gemm_bf16, dim3(tiles, chunks), dim3(64), 0, s, (const __bf16*)A, sam, sak,
+    launch_kernel(k_gemm_bf16, dim3(tiles, chunks), dim3(64), 0, s, (const __bf16*)A, sam, sak,
                        (const __bf16*)B, sbk, sbn, C, ldc, M, N, K, kchunk, ones_row);
     return hipGetLastError();
 }
@@ -233,7 +234,7 @@ __global__ void k_sum_partials(const float* __restrict__ part, int chunks, int M
 }
 
 hipError_t launch_sum_partials(const float* part, int chunks, int MN, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_sum_partials, dim3((MN + 255) / 256), dim3(256), 0, s, part, chunks, MN, out);
+    launch_kernel(k_sum_partials, dim3((MN + 255) / 256), dim3(256), 0, s, part, chunks, MN, out);
     return hipGetLastError();
 }
 
@@ -249,7 +250,7 @@ __global__ void k_bias_silu(float* __restrict__ z, const float* __restrict__ bia
 
 hipError_t launch_bias_silu(float* z, const float* bias, int rows, int N, void* h, hipStream_t s) {
     const long tot = (long)rows * N;
-    hipLaunchKernelGGL(k_bias_silu, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, z, bias, rows, N,
+    launch_kernel(k_bias_silu, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, z, bias, rows, N,
                        (__bf16*)h);
     return hipGetLastError();
 }
@@ -273,7 +274,7 @@ __global__ void k_out_loss(const float* __restrict__ z3, const float* __restrict
 
 hipError_t launch_out_loss(const float* z3, const float* b3, const float* y, int rows, float inv_scale,
                            void* dbf, float* err, hipStream_t s) {
-    hipLaunchKernelGGL(k_out_loss, dim3((rows + 255) / 256), dim3(256), 0, s, z3, b3, y, rows, inv_scale,
+    launch_kernel(k_out_loss, dim3((rows + 255) / 256), dim3(256), 0, s, z3, b3, y, rows, inv_scale,
                        (__bf16*)dbf, err);
     return hipGetLastError();
 }
@@ -287,7 +288,7 @@ __global__ void k_silu_back(const float* __restrict__ dh, const float* __restric
 }
 
 hipError_t launch_silu_back(const float* dh, const float* z, long tot, void* dzb, hipStream_t s) {
-    hipLaunchKernelGGL(k_silu_back, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, dh, z, tot,
+    launch_kernel(k_silu_back, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, dh, z, tot,
                        (__bf16*)dzb);
     return hipGetLastError();
 }
@@ -298,7 +299,7 @@ __global__ void k_to_bf16(const float* __restrict__ a, long tot, __bf16* __restr
 }
 
 hipError_t launch_to_bf16(const float* a, long tot, void* b, hipStream_t s) {
-    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, a, tot, (__bf16*)b);
+    launch_kernel(k_to_bf16, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, a, tot, (__bf16*)b);
     return hipGetLastError();
 }
 
@@ -317,7 +318,7 @@ __global__ void k_normalize_rows(const float* __restrict__ x, const int* __restr
 hipError_t launch_normalize_rows(const float* x, const int* rows, long nrows, const float* mu,
                                  const float* inv_sigma, void* xb, hipStream_t s) {
     const long tot = nrows * kFeat;
-    hipLaunchKernelGGL(k_normalize_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, x, rows, nrows,
+    launch_kernel(k_normalize_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, x, rows, nrows,
                        mu, inv_sigma, (__bf16*)xb);
     return hipGetLastError();
 }
@@ -333,7 +334,7 @@ __global__ void k_gather_rows(const float* __restrict__ y, const int* __restrict
 
 hipError_t launch_gather_rows(const float* y, const int* rows, long nrows, int w, float* out, hipStream_t s) {
     const long tot = nrows * w;
-    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, y, rows, nrows, w,
+    launch_kernel(k_gather_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, y, rows, nrows, w,
                        out);
     return hipGetLastError();
 }
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(256) void k_sum_into(const float* __restrict__ v, i
 }
 
 hipError_t launch_sum_into(const float* v, int n, float* acc, hipStream_t s) {
-    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(256), 0, s, v, n, acc);
+    launch_kernel(k_sum_into, dim3(1), dim3(256), 0, s, v, n, acc);
     return hipGetLastError();
 }
 
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(256) void k_col_stats(const float* __restrict__ x, 
 }
 
 hipError_t launch_col_stats(const float* x, long rows, int w, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_col_stats, dim3(w), dim3(256), 0, s, x, rows, w, out);
+    launch_kernel(k_col_stats, dim3(w), dim3(256), 0, s, x, rows, w, out);
     return hipGetLastError();
 }
 
@@ -406,7 +407,7 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, int np, float lr, float b1, float b2,
                        float eps, float c1, float c2, hipStream_t s) {
-    hipLaunchKernelGGL(k_adam, dim3((np + 255) / 256), dim3(256), 0, s, p, g, m, v, np, lr, b1, b2, eps, c1, c2);
+    launch_kernel(k_adam, dim3((np + 255) / 256), dim3(256), 0, s, p, g, m, v, np, lr, b1, b2, eps, c1, c2);
     return hipGetLastError();
 }
 

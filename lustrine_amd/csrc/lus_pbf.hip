@@ -36,6 +36,7 @@
 
 #include "lus_device.hpp"
 #include "lus_internal.hpp"
+#include "lus_launch.hpp"
 
 namespace lus {
 
@@ -507,12 +508,12 @@ hipError_t launch_tiles(const int* cs, const int* key2, PbfK k, int zt, int* tco
     const int nzc = (k.gZ + zt - 1) / zt;
     const int nchunks = k.gX * k.gY * nzc;
     const dim3 g((nchunks + kBlock - 1) / kBlock), b(kBlock);
-    hipLaunchKernelGGL(k_tile_count, g, b, 0, s, cs, k, zt, nzc, nchunks, tcount);
+    launch_kernel(k_tile_count, g, b, 0, s, cs, k, zt, nzc, nchunks, tcount);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = launch_exclusive_scan(tcount, toff, nchunks + 1, scan_tiles, 1, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_tile_write, g, b, 0, s, cs, key2, k, zt, nzc, nchunks, toff, tiles);
+    launch_kernel(k_tile_write, g, b, 0, s, cs, key2, k, zt, nzc, nchunks, toff, tiles);
     return hipGetLastError();
 }
 
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_key(const float4* __restrict__ x
 
 hipError_t launch_pbf_key(const float4* x, const float4* v, int n, PbfK k, float dt, int* key,
                           int* slot, int* counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_key, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, n, k, dt, key, slot,
+    launch_kernel(k_pbf_key, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, n, k, dt, key, slot,
                        counts);
     return hipGetLastError();
 }
@@ -570,7 +571,7 @@ hipError_t launch_pbf_permute(const int* tmp, const int* key, const int* cell_st
                               const float4* x, const float4* v, const int* id, int n, PbfK k,
                               float dt, float4* x2, float4* xp, int* id2, int* key2,
                               const int* gate, hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_permute, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start,
+    launch_kernel(k_pbf_permute, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start,
                        x, v, id, n, k, dt, x2, xp, id2, key2, gate);
     return hipGetLastError();
 }
@@ -592,7 +593,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_restore_old(const int* __restric
 
 hipError_t launch_pbf_restore_old(const int* tmp, const int* key, const int* cell_start,
                                   const float4* x, int n, float4* x2, const int* gate, hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_restore_old, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start, x, n,
+    launch_kernel(k_pbf_restore_old, dim3(grid_for(n)), dim3(kBlock), 0, s, tmp, key, cell_start, x, n,
                        x2, gate);
     return hipGetLastError();
 }
@@ -800,28 +801,28 @@ hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, 
         const dim3 g(ta.grid), b(kTileT);
         const size_t sh = (size_t)ta.lcap * sizeof(float4);
         if (build && k.exact)
-            hipLaunchKernelGGL((k_pbf_lambda_t<true, false>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
+            launch_kernel((k_pbf_lambda_t<true, false>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
                                gate, ta.tiles, ta.ntiles, ta.lcap);
         else if (build)
-            hipLaunchKernelGGL((k_pbf_lambda_t<true, true>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
+            launch_kernel((k_pbf_lambda_t<true, true>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
                                gate, ta.tiles, ta.ntiles, ta.lcap);
         else if (k.exact)
-            hipLaunchKernelGGL((k_pbf_lambda_t<false, false>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
+            launch_kernel((k_pbf_lambda_t<false, false>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
                                gate, ta.tiles, ta.ntiles, ta.lcap);
         else
-            hipLaunchKernelGGL((k_pbf_lambda_t<false, true>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
+            launch_kernel((k_pbf_lambda_t<false, true>), g, b, sh, s, xa, key2, cs, n, k, xl, maxc, masks, xp,
                                gate, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
     const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (build && k.exact)
-        hipLaunchKernelGGL((k_pbf_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
+        launch_kernel((k_pbf_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
     else if (build)
-        hipLaunchKernelGGL((k_pbf_lambda<true, true>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
+        launch_kernel((k_pbf_lambda<true, true>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
     else if (k.exact)
-        hipLaunchKernelGGL((k_pbf_lambda<false, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
+        launch_kernel((k_pbf_lambda<false, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
     else
-        hipLaunchKernelGGL((k_pbf_lambda<false, true>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
+        launch_kernel((k_pbf_lambda<false, true>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
     return hipGetLastError();
 }
 
@@ -949,28 +950,28 @@ hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, 
         const dim3 g(ta.grid), b(kTileT);
         const size_t sh = (size_t)ta.lcap * sizeof(float4);
         if (k.exact && n4)
-            hipLaunchKernelGGL((k_pbf_deltap_t<false, 4>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
+            launch_kernel((k_pbf_deltap_t<false, 4>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
                                xp, gate, ta.tiles, ta.ntiles, ta.lcap);
         else if (k.exact)
-            hipLaunchKernelGGL((k_pbf_deltap_t<false, 0>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
+            launch_kernel((k_pbf_deltap_t<false, 0>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
                                xp, gate, ta.tiles, ta.ntiles, ta.lcap);
         else if (n4)
-            hipLaunchKernelGGL((k_pbf_deltap_t<true, 4>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
+            launch_kernel((k_pbf_deltap_t<true, 4>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
                                xp, gate, ta.tiles, ta.ntiles, ta.lcap);
         else
-            hipLaunchKernelGGL((k_pbf_deltap_t<true, 0>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
+            launch_kernel((k_pbf_deltap_t<true, 0>), g, b, sh, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks,
                                xp, gate, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
     const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (k.exact && n4)
-        hipLaunchKernelGGL((k_pbf_deltap<false, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
+        launch_kernel((k_pbf_deltap<false, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
     else if (k.exact)
-        hipLaunchKernelGGL((k_pbf_deltap<false, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
+        launch_kernel((k_pbf_deltap<false, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
     else if (n4)
-        hipLaunchKernelGGL((k_pbf_deltap<true, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
+        launch_kernel((k_pbf_deltap<true, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
     else
-        hipLaunchKernelGGL((k_pbf_deltap<true, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
+        launch_kernel((k_pbf_deltap<true, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
     return hipGetLastError();
 }
 
@@ -986,7 +987,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_velocity(const float4* __restric
 
 hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, float dt,
                                float4* vout, hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_velocity, dim3(grid_for(n)), dim3(kBlock), 0, s, xa, xold, n, dt, vout);
+    launch_kernel(k_pbf_velocity, dim3(grid_for(n)), dim3(kBlock), 0, s, xa, xold, n, dt, vout);
     return hipGetLastError();
 }
 
@@ -1085,28 +1086,28 @@ hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const 
         const dim3 g(ta.grid), b(kTileT);
         const size_t sh = 2 * (size_t)ta.lcap * sizeof(float4);
         if (k.exact && maxd)
-            hipLaunchKernelGGL((k_pbf_xsph_t<false, true>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
+            launch_kernel((k_pbf_xsph_t<false, true>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
                                xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
         else if (k.exact)
-            hipLaunchKernelGGL((k_pbf_xsph_t<false, false>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
+            launch_kernel((k_pbf_xsph_t<false, false>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
                                xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
         else if (maxd)
-            hipLaunchKernelGGL((k_pbf_xsph_t<true, true>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
+            launch_kernel((k_pbf_xsph_t<true, true>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
                                xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
         else
-            hipLaunchKernelGGL((k_pbf_xsph_t<true, false>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
+            launch_kernel((k_pbf_xsph_t<true, false>), g, b, sh, s, xa, vt, key2, cs, n, k, vout, xout, masks,
                                xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
     const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (k.exact && maxd)
-        hipLaunchKernelGGL((k_pbf_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+        launch_kernel((k_pbf_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
     else if (k.exact)
-        hipLaunchKernelGGL((k_pbf_xsph<false, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+        launch_kernel((k_pbf_xsph<false, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
     else if (maxd)
-        hipLaunchKernelGGL((k_pbf_xsph<true, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+        launch_kernel((k_pbf_xsph<true, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
     else
-        hipLaunchKernelGGL((k_pbf_xsph<true, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+        launch_kernel((k_pbf_xsph<true, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
     return hipGetLastError();
 }
 
@@ -1227,13 +1228,13 @@ hipError_t launch_mix_lambda(const float4* xa, int* key2, const int* cs, int n, 
                              unsigned long long* masks, const float4* xp, int build, hipStream_t s) {
     const dim3 g(grid_for(n)), b(kBlock);
     if (build && k.exact)
-        hipLaunchKernelGGL((k_mix_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
+        launch_kernel((k_mix_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
     else if (build)
-        hipLaunchKernelGGL((k_mix_lambda<true, true>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
+        launch_kernel((k_mix_lambda<true, true>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
     else if (k.exact)
-        hipLaunchKernelGGL((k_mix_lambda<false, false>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
+        launch_kernel((k_mix_lambda<false, false>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
     else
-        hipLaunchKernelGGL((k_mix_lambda<false, true>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
+        launch_kernel((k_mix_lambda<false, true>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
     return hipGetLastError();
 }
 
@@ -1297,9 +1298,9 @@ hipError_t launch_mix_correct(const float4* xl, int* key2, const int* cs, int n,
                               const float4* xp, hipStream_t s) {
     const dim3 g(grid_for(n)), b(kBlock);
     if (k.exact)
-        hipLaunchKernelGGL((k_mix_correct<false>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+        launch_kernel((k_mix_correct<false>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
     else
-        hipLaunchKernelGGL((k_mix_correct<true>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+        launch_kernel((k_mix_correct<true>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
     return hipGetLastError();
 }
 
@@ -1351,13 +1352,13 @@ hipError_t launch_mix_xsph(const float4* xa, const float4* vt, int* key2, const 
                            hipStream_t s) {
     const dim3 g(grid_for(n)), b(kBlock);
     if (k.exact && maxd)
-        hipLaunchKernelGGL((k_mix_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
+        launch_kernel((k_mix_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
     else if (k.exact)
-        hipLaunchKernelGGL((k_mix_xsph<false, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
+        launch_kernel((k_mix_xsph<false, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
     else if (maxd)
-        hipLaunchKernelGGL((k_mix_xsph<true, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
+        launch_kernel((k_mix_xsph<true, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
     else
-        hipLaunchKernelGGL((k_mix_xsph<true, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
+        launch_kernel((k_mix_xsph<true, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd);
     return hipGetLastError();
 }
 
@@ -1495,22 +1496,22 @@ hipError_t launch_pbf_features(int* key2, const int* cs, int n, PbfK k, unsigned
     if (!frec && ta.tiles) {
         const size_t sh = (size_t)ta.lcap * sizeof(float4);
         if (k.exact)
-            hipLaunchKernelGGL((k_pbf_features_t<false>), dim3(ta.grid), dim3(kTileT), sh, s, key2, cs, n, k, masks,
+            launch_kernel((k_pbf_features_t<false>), dim3(ta.grid), dim3(kTileT), sh, s, key2, cs, n, k, masks,
                                xp, xold, mu, inv_sigma, (bf16x8_t*)fbf, inv_h, ta.tiles, ta.ntiles, ta.lcap);
         else
-            hipLaunchKernelGGL((k_pbf_features_t<true>), dim3(ta.grid), dim3(kTileT), sh, s, key2, cs, n, k, masks,
+            launch_kernel((k_pbf_features_t<true>), dim3(ta.grid), dim3(kTileT), sh, s, key2, cs, n, k, masks,
                                xp, xold, mu, inv_sigma, (bf16x8_t*)fbf, inv_h, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
     if (frec) {
         if (rec_stride < 1) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_pbf_features<false, true>), g, b, 0, s, key2, cs, n, k, masks, xp, xold, mu,
+        launch_kernel((k_pbf_features<false, true>), g, b, 0, s, key2, cs, n, k, masks, xp, xold, mu,
                            inv_sigma, (bf16x8_t*)fbf, frec, rec_stride, rec_off, inv_h);
     } else if (k.exact) {
-        hipLaunchKernelGGL((k_pbf_features<false, false>), g, b, 0, s, key2, cs, n, k, masks, xp, xold, mu,
+        launch_kernel((k_pbf_features<false, false>), g, b, 0, s, key2, cs, n, k, masks, xp, xold, mu,
                            inv_sigma, (bf16x8_t*)fbf, frec, 1, 0, inv_h);
     } else {
-        hipLaunchKernelGGL((k_pbf_features<true, false>), g, b, 0, s, key2, cs, n, k, masks, xp, xold, mu,
+        launch_kernel((k_pbf_features<true, false>), g, b, 0, s, key2, cs, n, k, masks, xp, xold, mu,
                            inv_sigma, (bf16x8_t*)fbf, frec, 1, 0, inv_h);
     }
     return hipGetLastError();
@@ -1534,7 +1535,7 @@ hipError_t launch_pbf_record_target(const float4* xa, const float4* xp, int n, i
                                     int rec_off, float* tgt, hipStream_t s) {
     if (rec_stride < 1) return hipErrorInvalidValue;
     const int rows = n / rec_stride + 1;
-    hipLaunchKernelGGL(k_pbf_record_target, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, xa, xp,
+    launch_kernel(k_pbf_record_target, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, xa, xp,
                        n, rec_stride, rec_off, tgt);
     return hipGetLastError();
 }
@@ -1588,7 +1589,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_decide(const int* __restrict__ m
 
 hipError_t launch_pbf_decide(const int* maxd, int nblk, int n, float bound, int learned, int* flag,
                              int* counters, float* hist, int cap, hipStream_t s) {
-    hipLaunchKernelGGL(k_pbf_decide, dim3(1), dim3(kBlock), 0, s, maxd, nblk, n, bound, learned, flag,
+    launch_kernel(k_pbf_decide, dim3(1), dim3(kBlock), 0, s, maxd, nblk, n, bound, learned, flag,
                        counters, hist, cap);
     return hipGetLastError();
 }
