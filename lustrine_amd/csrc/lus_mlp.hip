@@ -35,10 +35,20 @@ __device__ __forceinline__ float silu_grad(float z) {
 }
 
 // ------------------------------------------------------------------ inference
-// B-operand images of the weights (one 16-byte load per lane and tile):
-//   [0, 4)    W1 n-tile j:            lane l: W1[8(l/16)+e][16j + l%16]
-//   [4, 12)   W2 k-step s, n-tile j:  lane l: W2[32s + 8(l/16)+e][16j + l%16]
-//   [12, 14)  W3 k-step s (n < 3):    lane l: W3[32s + 8(l/16)+e][l%16], 0 for l%16 >= 3
+// Inference runs each layer transposed, Y^T = W^T X^T: the weights are the A
+// operand and the activations the B operand, so the C tile of a layer holds,
+// in lane l, units 16j + 4(l/16) + i (i = 0..3) of sample l%16.  The next
+// layer's reduction index is permuted to take those values as they lie:
+// k-step s, lane group g = l/16, element e  <->  hidden unit
+//   pi(s, g, e) = 16 (2s + e/4) + 4g + e%4,
+// i.e. the B operand of k-step s is C tiles 2s and 2s+1 of the lane itself,
+// and no activation leaves its lane.  A-operand images of the weights (one
+// 16-byte load per lane and image):
+//   [0, 4)    W1 m-tile j:            lane l: W1[8(l/16)+e][16j + l%16]
+//   [4, 12)   W2 k-step s, m-tile j:  lane l: W2[pi(s, l/16, e)][16j + l%16]
+//   [12, 14)  W3 k-step s (m < 3):    lane l: W3[pi(s, l/16, e)][l%16], 0 for l%16 >= 3
+__device__ __forceinline__ int mlp_pi(int s, int g, int e) { return 16 * (2 * s + e / 4) + 4 * g + e % 4; }
+
 __global__ void k_mlp_pack(const float* __restrict__ p, bf16x8* __restrict__ w) {
     const int t = threadIdx.x + blockIdx.x * blockDim.x;
     const int img = t / 64, l = t % 64;
@@ -49,11 +59,11 @@ __global__ void k_mlp_pack(const float* __restrict__ p, bf16x8* __restrict__ w) 
         for (int e = 0; e < 8; ++e) v[e] = (__bf16)p[kMlpW1 + (8 * grp + e) * 64 + 16 * img + col];
     } else if (img < 12) {
         const int s = (img - 4) / 4, j = (img - 4) % 4;
-        for (int e = 0; e < 8; ++e) v[e] = (__bf16)p[kMlpW2 + (32 * s + 8 * grp + e) * 64 + 16 * j + col];
+        for (int e = 0; e < 8; ++e) v[e] = (__bf16)p[kMlpW2 + mlp_pi(s, grp, e) * 64 + 16 * j + col];
     } else {
         const int s = img - 12;
         for (int e = 0; e < 8; ++e)
-            v[e] = col < 3 ? (__bf16)p[kMlpW3 + (32 * s + 8 * grp + e) * 3 + col] : (__bf16)0.0f;
+            v[e] = col < 3 ? (__bf16)p[kMlpW3 + mlp_pi(s, grp, e) * 3 + col] : (__bf16)0.0f;
     }
     w[t] = v;
 }
@@ -63,17 +73,23 @@ hipError_t launch_mlp_pack(const float* params, void* packed, hipStream_t s) {
     return hipGetLastError();
 }
 
-// One wave per 16-sample tile (grid-stride over tiles).  X: bf16 [n][32].
-// A layer's C tile goes to LDS as [sample][64] bf16 and comes back as the
-// next layer's A operand (8 consecutive features of one sample per lane).
+// One wave per 16-sample tile (grid-stride over tiles).  X: bf16 [n][32]; a
+// lane loads 8 features of one sample, which is the B operand of layer 1 as
+// it lies.  The activations stay in registers (see the pack layout above) and
+// the lanes of group 0 hold the three outputs of their sample at the end.  The
+// next tile's features are requested before the current tile's layers run.
 constexpr int kMlpWaves = 4;
+
+// SiLU for inference with the hardware reciprocal (1 ulp): the activation is
+// rounded to bf16 right after, far coarser than that error.
+__device__ __forceinline__ float silu_fast(float z) {
+    return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 
 __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(
     const bf16x8* __restrict__ X, int n, const float* __restrict__ params,
     const bf16x8* __restrict__ wpk, float out_scale, float* __restrict__ out,
     const float4* __restrict__ xp, float4* __restrict__ xa, PbfK k) {
-    __shared__ __bf16 act[kMlpWaves][16][64 + 8];
-    __shared__ float res[kMlpWaves][16][4];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int col = lane % 16, grp = lane / 16;
@@ -86,66 +102,63 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(
         for (int j = 0; j < 4; ++j) W2[s][j] = wpk[(4 + s * 4 + j) * 64 + lane];
         W3[s] = wpk[(12 + s) * 64 + lane];
     }
-    float B1[4], B2[4];
+    // biases of the units this lane's C values belong to: 16j + 4 grp + i
+    f32x4 B1[4], B2[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        B1[j] = params[kMlpB1 + 16 * j + col];
-        B2[j] = params[kMlpB2 + 16 * j + col];
-    }
-    const float B3 = col < 3 ? params[kMlpB3 + col] : 0.0f;
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            B1[j][i] = params[kMlpB1 + 16 * j + 4 * grp + i];
+            B2[j][i] = params[kMlpB2 + 16 * j + 4 * grp + i];
+        }
+    float B3[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) B3[c] = params[kMlpB3 + c];
     const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     const int ntiles = (n + 15) / 16;
-    const int wave_id = blockIdx.x * kMlpWaves + w;
     const int nwaves = gridDim.x * kMlpWaves;
-    // every wave of the block runs the same number of rounds (block barriers)
-    const int rounds = (ntiles + nwaves - 1) / nwaves;
-    for (int r = 0; r < rounds; ++r) {
-        const int tile = r * nwaves + wave_id;
-        const int p0 = tile * 16;
-        bf16x8 a = {};
-        if (tile < ntiles && p0 + col < n) a = X[(size_t)(p0 + col) * 4 + grp];
+    int tile = blockIdx.x * kMlpWaves + w;  // uniform per wave: the loop below never diverges
+    bf16x8 a = {};
+    if (tile < ntiles && tile * 16 + col < n) a = X[(size_t)(tile * 16 + col) * 4 + grp];
+    for (; tile < ntiles; tile += nwaves) {
+        const int nt = tile + nwaves;
+        bf16x8 an = {};
+        if (nt < ntiles && nt * 16 + col < n) an = X[(size_t)(nt * 16 + col) * 4 + grp];
+        const int i = tile * 16 + col;
+        const bool writer = grp == 0 && i < n;
+        float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (xa && writer) p = xp[i];
         f32x4 z[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) z[j] = mfma(a, W1[j], zero);
+        for (int j = 0; j < 4; ++j) z[j] = mfma(W1[j], a, zero);
+        bf16x8 h[2];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) act[w][4 * grp + i][16 * j + col] = (__bf16)silu(z[j][i] + B1[j]);
-        __syncthreads();
-        bf16x8 a2[2];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) a2[s] = *(const bf16x8*)&act[w][col][32 * s + 8 * grp];
-        __syncthreads();
+            for (int e = 0; e < 8; ++e)
+                h[s][e] = (__bf16)silu_fast(z[2 * s + e / 4][e % 4] + B1[2 * s + e / 4][e % 4]);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            z[j] = mfma(a2[0], W2[0][j], zero);
-            z[j] = mfma(a2[1], W2[1][j], z[j]);
+            z[j] = mfma(W2[0][j], h[0], zero);
+            z[j] = mfma(W2[1][j], h[1], z[j]);
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) act[w][4 * grp + i][16 * j + col] = (__bf16)silu(z[j][i] + B2[j]);
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < 2; ++s) a2[s] = *(const bf16x8*)&act[w][col][32 * s + 8 * grp];
-        __syncthreads();
-        f32x4 o = mfma(a2[0], W3[0], zero);
-        o = mfma(a2[1], W3[1], o);
-        if (col < 3) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) res[w][4 * grp + i][col] = (o[i] + B3) * out_scale;
-        }
-        __syncthreads();
-        if (lane < 16 && tile < ntiles && p0 + lane < n) {
-            const int i = p0 + lane;
-            const float dx = res[w][lane][0], dy = res[w][lane][1], dz = res[w][lane][2];
+            for (int e = 0; e < 8; ++e)
+                h[s][e] = (__bf16)silu_fast(z[2 * s + e / 4][e % 4] + B2[2 * s + e / 4][e % 4]);
+        f32x4 o = mfma(W3[0], h[0], zero);
+        o = mfma(W3[1], h[1], o);
+        if (writer) {
+            const float dx = (o[0] + B3[0]) * out_scale;
+            const float dy = (o[1] + B3[1]) * out_scale;
+            const float dz = (o[2] + B3[2]) * out_scale;
             if (out) {
                 out[(size_t)i * 3 + 0] = dx;
                 out[(size_t)i * 3 + 1] = dy;
                 out[(size_t)i * 3 + 2] = dz;
             }
             if (xa) {
-                const float4 p = xp[i];
                 const float* lo = k.lo;
                 const float* hi = k.hi;
                 if (k.scenes) {
@@ -157,7 +170,7 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(
                                     rclamp(p.z + dz, lo[2], hi[2]), 0.0f);
             }
         }
-        __syncthreads();
+        a = an;
     }
 }
 
