@@ -589,9 +589,19 @@ __global__ __launch_bounds__(kBlock) void k_pbf_permute(
     const int o = stable_slot(tmp, cell_start, c, i);
     const float4 xi = x[i];
     const int s = k.scenes ? k.sid[i] : 0;
-    x2[o] = xi;
     const int ph = k.ph ? k.ph[i] : 0;
-    xp[o] = ph == 2 ? xi : pbf_predict(xi, v[i], k, bounds_of(k, s), dt);
+    float4 xq = ph == 2 ? xi : pbf_predict(xi, v[i], k, bounds_of(k, s), dt);
+    if (k.ph) {
+        // mixed scene: the phase rides in w of the old and predicted positions
+        // (and of the corrected ones), so neighbour loops read it with the
+        // position instead of gathering ph_out[j]
+        const float fph = (float)ph;
+        x2[o] = make_float4(xi.x, xi.y, xi.z, fph);
+        xq.w = fph;
+    } else {
+        x2[o] = xi;
+    }
+    xp[o] = xq;
     id2[o] = id[i];
     key2[o] = c;
     if (k.scenes) k.sid_out[o] = s;
@@ -1144,7 +1154,8 @@ hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const 
 
 // ---------------------------------------------------------------------------
 // Mixed sand + fluid over static solids.  One cell-ordered array holds every
-// phase (k.ph_out[slot]: 0 fluid, 1 sand, 2 solid) and shares the neighbour
+// phase (k.ph_out[slot]: 0 fluid, 1 sand, 2 solid; a neighbour's phase is read
+// from w of its old, predicted or corrected position) and shares the neighbour
 // sets.  Fluid: density over every neighbour, the other-particle gradient term
 // over fluid neighbours only, lambda = 0 for sand and solids, delta-p as the
 // PBF pass.  Sand: the contact + friction projection of simulate_sand
@@ -1242,7 +1253,7 @@ __global__ __launch_bounds__(kBlock) void k_mix_lambda(const float4* __restrict_
             rho = acc<kFast>(rho, t * t, t);
             if (r2 > 0.0f) {
                 const float s = spiky_s<kFast>(r2, k);
-                if (k.ph_out[j] == 0) a = acc<kFast>(a, s * s, r2);
+                if (pj.w == 0.0f) a = acc<kFast>(a, s * s, r2);
                 gx = acc<kFast>(gx, s, dx);
                 gy = acc<kFast>(gy, s, dy);
                 gz = acc<kFast>(gz, s, dz);
@@ -1280,13 +1291,14 @@ __global__ __launch_bounds__(kBlock) void k_mix_correct(const float4* __restrict
     if (i >= n) return;
     const int ph = k.ph_out[i];
     const float4 pi = xl[i];
-    float4 r = make_float4(pi.x, pi.y, pi.z, 0.0f);
+    float4 r = make_float4(pi.x, pi.y, pi.z, (float)ph);
     if (ph == 1) {
         const float4 xoi = xold[i];
         float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
         for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k, [&](int j, const float4 pj, const float4) {
             if (j == i) return;
-            mix_sand_pair(pi, xoi, pj, xold[j], k.ph_out[j], k, dpx, dpy, dpz);
+            const float4 xoj = xold[j];
+            mix_sand_pair(pi, xoi, pj, xoj, (int)xoj.w, k, dpx, dpy, dpz);
         });
         r.x = rclamp(pi.x + dpx, k.lo[0], k.hi[0]);
         r.y = rclamp(pi.y + dpy, k.lo[1], k.hi[1]);
@@ -1361,7 +1373,7 @@ __global__ __launch_bounds__(kBlock) void k_mix_xsph(const float4* __restrict__ 
                     const float t = poly_t<kFast>(r2, k);
                     const float w = (t * t) * t;
                     rho += w;
-                    if (k.ph_out[j] == 0) {
+                    if (pj.w == 0.0f) {
                         sx = acc<kFast>(sx, vj.x - vi.x, w);
                         sy = acc<kFast>(sy, vj.y - vi.y, w);
                         sz = acc<kFast>(sz, vj.z - vi.z, w);
