@@ -133,6 +133,38 @@ def test_pbf_one_step_small(gpu_mod, oracle, K):
         assert abs(gmc - omc) <= 1e-6 * max(1.0, abs(omc))
 
 
+# particle counts that are not multiples of a wave (64) or of a block (256):
+# every kernel's tail lanes, the XCD block remap's padding blocks and the
+# neighbour-list runs of a partial 64-slot group
+RAGGED = [((1, 1, 1), (16, 16, 16)), ((1, 1, 2), (16, 16, 16)), ((3, 3, 7), (16, 16, 16)),
+          ((5, 13, 1), (16, 16, 16)), ((10, 10, 10), (16, 16, 16)), ((17, 17, 17), (24, 24, 24))]
+
+
+@pytest.mark.parametrize("lat,box", RAGGED)
+def test_pbf_ragged_sizes_exact_bits(gpu_mod, oracle, lat, box):
+    cfg = scenes.small(*lat, box=box, seed=sum(lat))
+    (gx, gv, gid, _), (x, v, ids, _) = _pbf_compare(gpu_mod, oracle, cfg, steps=3)
+    assert np.array_equal(gid, ids)
+    assert np.array_equal(gx, x) and np.array_equal(gv, v), (max_ulp(gx, x), max_ulp(gv, v))
+
+
+@pytest.mark.parametrize("lat,box", RAGGED[2:])
+def test_pbf_ragged_sizes_fused_and_graph(gpu_mod, oracle, lat, box):
+    cfg = scenes.small(*lat, box=box, seed=sum(lat))
+    (gx, gv, gid, _), (x, v, ids, _) = _pbf_compare(gpu_mod, oracle, cfg, steps=3, exact=False)
+    assert np.array_equal(gid, ids)
+    assert rel_err(gx, x) <= REL
+    # three steps under hipGraph replay (one captured pair plus one plain step)
+    # give the plain launches' bits
+    x0 = cfg.positions()
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_init(gpu_mod.pbf_params(cfg, exact=False), x0, None)
+        g.pbf_set_graph(True)
+        g.pbf_step(np.float32(cfg.dt), 3)
+        hx, hv, hid = g.pbf_download(order=0)
+    assert np.array_equal(hid, gid) and np.array_equal(hx, gx) and np.array_equal(hv, gv)
+
+
 def test_pbf_one_step_dam_break_32k(gpu_mod, oracle):
     # the 32,768-particle dam break, one step from the initial state
     cfg = scenes.cfg0()
