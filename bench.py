@@ -271,6 +271,69 @@ def bench_learned(args, G, scenes) -> int:
     return 0
 
 
+def bench_sand(args, G, scenes) -> int:
+    """The reference's default simulate_fun, simulate_sand (src/Simulate.cpp:156-325:
+    predict, sorted-grid lists, 4 Jacobi contact + friction substeps, velocity update),
+    bit-exact on the GPU, on the 1,048,576-particle dam-break lattice as sand (no
+    solids; h = 3.1 r, init_simulation's default kernel radius).  Sand particle-steps/s
+    timed on the host around synchronised batches of steps, the phase slots of one
+    profiled step, and the oracle's simulate_sand on one host core."""
+    cfg = scenes.cfg2()
+    pos = cfg.positions()
+    sp = G.sand_params(*cfg.box)
+    dt = np.float32(cfg.dt)
+    n = pos.shape[0]
+    with G.Gpu(0) as g:
+        g.sand_init(sp, pos)
+        for _ in range(args.warmup):
+            g.sand_step(dt)
+        g.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.sand_step(dt)
+        g.sync()
+        sec = time.perf_counter() - t0
+        slots = None
+        if not args.no_phases:
+            g.profiling(True)
+            g.sand_step(dt)
+            g.sync()
+            _, slots = G.profile_slots()
+            g.profiling(False)
+    ms = sec * 1e3
+    out = {
+        "metric": "simulate_sand particle-steps/s at 1M sand particles",
+        "value": n * args.steps / sec, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "sand_dam_break_1m", "particles": n, "lattice": list(cfg.lattice),
+                   "box": list(cfg.box), "h": float(sp.kernelRadius), "cells": sp.grid.gridX * sp.grid.gridY * sp.grid.gridZ,
+                   "substeps": int(sp.substeps), "dt": cfg.dt, "arithmetic": "exact (bit-identical to the oracle)",
+                   "timing": "host clock around synchronised step batches", "parallelism": "single-gpu"},
+    }
+    if slots is not None:
+        out["phases_ms"] = {"step": slots[0] * 1e3, "neighbour_search": slots[3] * 1e3,
+                            "substeps": slots[4] * 1e3, "velocity_update": slots[6] * 1e3,
+                            "window": f"step {args.warmup + args.steps + 1}"}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        O.build()
+        s = O.init_constants(*cfg.box, 0.5, 1.0, 3.1)
+        st = O.SandStream(s, pos, None)
+        st.step(dt)  # the first call also copies the solids (src/Simulate.cpp:217-223)
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            st.step(dt)
+            done += 1
+        csec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * done / csec, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"steps 2-{done + 1} of the sand scene ({csec:.1f} s), FP32 oracle "
+                                         f"simulate_sand, 1 thread of {cpu_model()}"}
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+    return 0
+
+
 def bench_mixed(args, G, scenes) -> int:
     """Mixed sand + fluid over the voxel heightfield: moving particle-steps/s
     (solids are not counted), 3 iterations, hipGraph replay."""
@@ -334,7 +397,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg1", "cfg2", "cfg2_tail", "cfg3", "cfg4"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg1", "cfg2", "cfg2_tail", "cfg3", "cfg4", "sand_1m"])
     ap.add_argument("--network", default=None, help="warm-start network file (cfg3)")
     ap.add_argument("--iterations", type=int, default=None)
     ap.add_argument("--exact", action="store_true",
@@ -365,6 +428,8 @@ def main(argv=None):
         return bench_learned(args, G, scenes)
     if args.workload == "cfg1":
         return bench_mixed(args, G, scenes)
+    if args.workload == "sand_1m":
+        return bench_sand(args, G, scenes)
     cfg = {"cfg0": scenes.cfg0, "cfg2": scenes.cfg2, "cfg2_tail": scenes.cfg2_tail}[args.workload]()
     if args.iterations is not None:
         cfg.iterations = args.iterations
