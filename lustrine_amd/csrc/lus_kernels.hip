@@ -576,7 +576,80 @@ hipError_t launch_sand_nbr(const float4* xs, const int* key2, const int* sand_cs
     return hipGetLastError();
 }
 
-// One Jacobi substep of contact + friction (src/Simulate.cpp:228-308)
+// The contact + friction projection of particle (pi, xoi) against neighbour j
+// at pj (old position xoj, sand neighbours only) into dp (src/Simulate.cpp:
+// 240-300).
+__device__ __forceinline__ void sand_pair(const float4 pi, const float4 xoi, const float4 pj, const float4 xoj,
+                                          bool sand, const SandK& k, float& dpx, float& dpy, float& dpz) {
+    float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
+    if (sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz)) == 0.0f) {
+        ijx = 0.0f;
+        ijy = 0.00001f;
+        ijz = 0.0f;
+    }
+    const float len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
+    if (len > k.d) return;
+    float sc;
+    if (sand) {
+        sc = k.collision * k.mass_n;
+        sc = sc / k.mass_sum;
+        sc = sc * (len - k.d);
+    } else {
+        sc = k.collision * (len - k.d);
+    }
+    const float tx = (sc * ijx) / len;
+    const float ty = (sc * ijy) / len;
+    const float tz = (sc * ijz) / len;
+    dpx = dpx - tx;
+    dpy = dpy - ty;
+    dpz = dpz - tz;
+    const float d = sqrtf(dot3(tx, ty, tz, tx, ty, tz));
+    float xidx = (pi.x - tx) - xoi.x;
+    float xidy = (pi.y - ty) - xoi.y;
+    float xidz = (pi.z - tz) - xoi.z;
+    float xjdx, xjdy, xjdz, nx, ny, nz;
+    if (sand) {
+        xjdx = (pj.x + tx) - xoj.x;
+        xjdy = (pj.y + ty) - xoj.y;
+        xjdz = (pj.z + tz) - xoj.z;
+        nx = (pi.x - tx) - (pj.x + tx);
+        ny = (pi.y - ty) - (pj.y + ty);
+        nz = (pi.z - tz) - (pj.z + tz);
+    } else {
+        xjdx = 0.0f;
+        xjdy = 0.0f;
+        xjdz = 0.0f;
+        nx = (pi.x - tx) - pj.x;
+        ny = (pi.y - ty) - pj.y;
+        nz = (pi.z - tz) - pj.z;
+    }
+    const float inv = 1.0f / sqrtf(dot3(nx, ny, nz, nx, ny, nz));
+    nx = nx * inv;
+    ny = ny * inv;
+    nz = nz * inv;
+    const float rx = xidx - xjdx, ry = xidy - xjdy, rz = xidz - xjdz;
+    const float rd = dot3(rx, ry, rz, nx, ny, nz);
+    const float xtx = rx - rd * nx, xty = ry - rd * ny, xtz = rz - rd * nz;
+    const float lt = sqrtf(dot3(xtx, xty, xtz, xtx, xty, xtz));
+    if ((d * k.mu_s) > (lt + 1e-9f)) {
+        dpx = dpx - k.friction * xtx;
+        dpy = dpy - k.friction * xty;
+        dpz = dpz - k.friction * xtz;
+    } else {
+        const float q = (k.mu_k * d) / (lt + 1e-9f);
+        const float ratio = q < 1.0f ? q : 1.0f;
+        dpx = dpx - (k.friction * xtx) * ratio;
+        dpy = dpy - (k.friction * xty) * ratio;
+        dpz = dpz - (k.friction * xtz) * ratio;
+    }
+}
+
+// One Jacobi substep of contact + friction (src/Simulate.cpp:228-308).  The
+// list is walked four entries at a time: their indices, positions and old
+// positions are all requested before the four are projected in list order, so
+// a wave waits once per four neighbours instead of twice per neighbour.
+// Places past the list end (and solid entries, for the old position) read
+// slot i, in bounds and unused.
 __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restrict__ A,
                                                           const float4* __restrict__ xold,
                                                           const float4* __restrict__ solid,
@@ -589,73 +662,20 @@ __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restric
     const float4 xoi = xold[i];
     float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
     const int t1 = off[i + 1];
-    for (int t = off[i]; t < t1; ++t) {
-        const int j = idx[t];
-        if (j == i) continue;
-        const bool sand = j < n;
-        const float4 pj = sand ? A[j] : solid[j - n];
-        float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
-        if (sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz)) == 0.0f) {
-            ijx = 0.0f;
-            ijy = 0.00001f;
-            ijz = 0.0f;
+    for (int t = off[i]; t < t1; t += 4) {
+        int j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = (t + u < t1) ? idx[t + u] : i;
+        float4 pj[4], xoj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool sand = j[u] < n;
+            pj[u] = sand ? A[j[u]] : solid[j[u] - n];
+            xoj[u] = xold[sand ? j[u] : i];
         }
-        const float len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
-        if (len > k.d) continue;
-        float sc;
-        if (sand) {
-            sc = k.collision * k.mass_n;
-            sc = sc / k.mass_sum;
-            sc = sc * (len - k.d);
-        } else {
-            sc = k.collision * (len - k.d);
-        }
-        const float tx = (sc * ijx) / len;
-        const float ty = (sc * ijy) / len;
-        const float tz = (sc * ijz) / len;
-        dpx = dpx - tx;
-        dpy = dpy - ty;
-        dpz = dpz - tz;
-        const float d = sqrtf(dot3(tx, ty, tz, tx, ty, tz));
-        float xidx = (pi.x - tx) - xoi.x;
-        float xidy = (pi.y - ty) - xoi.y;
-        float xidz = (pi.z - tz) - xoi.z;
-        float xjdx, xjdy, xjdz, nx, ny, nz;
-        if (sand) {
-            const float4 xoj = xold[j];
-            xjdx = (pj.x + tx) - xoj.x;
-            xjdy = (pj.y + ty) - xoj.y;
-            xjdz = (pj.z + tz) - xoj.z;
-            nx = (pi.x - tx) - (pj.x + tx);
-            ny = (pi.y - ty) - (pj.y + ty);
-            nz = (pi.z - tz) - (pj.z + tz);
-        } else {
-            xjdx = 0.0f;
-            xjdy = 0.0f;
-            xjdz = 0.0f;
-            nx = (pi.x - tx) - pj.x;
-            ny = (pi.y - ty) - pj.y;
-            nz = (pi.z - tz) - pj.z;
-        }
-        const float inv = 1.0f / sqrtf(dot3(nx, ny, nz, nx, ny, nz));
-        nx = nx * inv;
-        ny = ny * inv;
-        nz = nz * inv;
-        const float rx = xidx - xjdx, ry = xidy - xjdy, rz = xidz - xjdz;
-        const float rd = dot3(rx, ry, rz, nx, ny, nz);
-        const float xtx = rx - rd * nx, xty = ry - rd * ny, xtz = rz - rd * nz;
-        const float lt = sqrtf(dot3(xtx, xty, xtz, xtx, xty, xtz));
-        if ((d * k.mu_s) > (lt + 1e-9f)) {
-            dpx = dpx - k.friction * xtx;
-            dpy = dpy - k.friction * xty;
-            dpz = dpz - k.friction * xtz;
-        } else {
-            const float q = (k.mu_k * d) / (lt + 1e-9f);
-            const float ratio = q < 1.0f ? q : 1.0f;
-            dpx = dpx - (k.friction * xtx) * ratio;
-            dpy = dpy - (k.friction * xty) * ratio;
-            dpz = dpz - (k.friction * xtz) * ratio;
-        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (t + u < t1 && j[u] != i) sand_pair(pi, xoi, pj[u], xoj[u], j[u] < n, k, dpx, dpy, dpz);
     }
     float4 r;
     r.x = rclamp(pi.x + dpx, k.r, k.hi[0]);
