@@ -529,32 +529,48 @@ __global__ __launch_bounds__(kBlock) void k_sand_nbr(const float4* __restrict__ 
         for (int x = xl; x <= xu; ++x) {
             for (int z = zl; z <= zu; ++z) {
                 const int nc = (yy + y) * k.gXZ + (xx + x) * k.gZ + (zz + z);
+                // candidates four at a time: the four loads are requested
+                // together, then tested in slot order (the last candidate of
+                // the cell stands in for places past its end)
                 if (solid) {
-                    for (int j = ocs[nc]; j < ocs[nc + 1]; ++j) {
-                        const float4 q = solid[j];
-                        const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
-                        if (dot3(dx, dy, dz, dx, dy, dz) <= k.h2) {
-                            if (fill) idx[o2 + n2] = n + j;
-                            ++n2;
+                    const int je = ocs[nc + 1];
+                    for (int j0 = ocs[nc]; j0 < je; j0 += 4) {
+                        float4 q[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) q[u] = solid[j0 + u < je ? j0 + u : je - 1];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const float dx = me.x - q[u].x, dy = me.y - q[u].y, dz = me.z - q[u].z;
+                            if (j0 + u < je && dot3(dx, dy, dz, dx, dy, dz) <= k.h2) {
+                                if (fill) idx[o2 + n2] = n + j0 + u;
+                                ++n2;
+                            }
                         }
                     }
                 }
-                for (int j = scs[nc]; j < scs[nc + 1]; ++j) {
-                    const float4 q = xs[j];
-                    const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
-                    if (dot3(dx, dy, dz, dx, dy, dz) <= k.h2) {
-                        if (j < i) {
-                            // pushed into slot i by slot j's scan
-                            if (i < k.sym_limit) {
-                                if (fill) idx[o1 + n1] = j;
-                                ++n1;
-                            }
-                        } else {
-                            if (fill) idx[o2 + n2] = j;
-                            ++n2;
-                            if (j == i && i < k.sym_limit) {
+                const int je = scs[nc + 1];
+                for (int j0 = scs[nc]; j0 < je; j0 += 4) {
+                    float4 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) q[u] = xs[j0 + u < je ? j0 + u : je - 1];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int j = j0 + u;
+                        const float dx = me.x - q[u].x, dy = me.y - q[u].y, dz = me.z - q[u].z;
+                        if (j < je && dot3(dx, dy, dz, dx, dy, dz) <= k.h2) {
+                            if (j < i) {
+                                // pushed into slot i by slot j's scan
+                                if (i < k.sym_limit) {
+                                    if (fill) idx[o1 + n1] = j;
+                                    ++n1;
+                                }
+                            } else {
                                 if (fill) idx[o2 + n2] = j;
                                 ++n2;
+                                if (j == i && i < k.sym_limit) {
+                                    if (fill) idx[o2 + n2] = j;
+                                    ++n2;
+                                }
                             }
                         }
                     }
@@ -576,10 +592,10 @@ hipError_t launch_sand_nbr(const float4* xs, const int* key2, const int* sand_cs
     return hipGetLastError();
 }
 
-// The contact + friction projection of particle (pi, xoi) against neighbour j
-// at pj (old position xoj, sand neighbours only) into dp (src/Simulate.cpp:
-// 240-300).
-__device__ __forceinline__ void sand_pair(const float4 pi, const float4 xoi, const float4 pj, const float4 xoj,
+// The contact + friction projection of particle (pi, xoi) against neighbour jj
+// at pj (its old position xold[jj] is read for a sand contact only) into dp
+// (src/Simulate.cpp:240-300).
+__device__ __forceinline__ void sand_pair(const float4 pi, const float4 xoi, const float4 pj, const float4* __restrict__ xold, int jj,
                                           bool sand, const SandK& k, float& dpx, float& dpy, float& dpz) {
     float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
     if (sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz)) == 0.0f) {
@@ -609,6 +625,7 @@ __device__ __forceinline__ void sand_pair(const float4 pi, const float4 xoi, con
     float xidz = (pi.z - tz) - xoi.z;
     float xjdx, xjdy, xjdz, nx, ny, nz;
     if (sand) {
+        const float4 xoj = xold[jj];
         xjdx = (pj.x + tx) - xoj.x;
         xjdy = (pj.y + ty) - xoj.y;
         xjdz = (pj.z + tz) - xoj.z;
@@ -645,11 +662,10 @@ __device__ __forceinline__ void sand_pair(const float4 pi, const float4 xoi, con
 }
 
 // One Jacobi substep of contact + friction (src/Simulate.cpp:228-308).  The
-// list is walked four entries at a time: their indices, positions and old
-// positions are all requested before the four are projected in list order, so
-// a wave waits once per four neighbours instead of twice per neighbour.
-// Places past the list end (and solid entries, for the old position) read
-// slot i, in bounds and unused.
+// list is walked four entries at a time: their indices and positions are all
+// requested before the four are projected in list order (a sand contact's old
+// position is loaded inside the projection, ~a quarter of the entries).
+// Places past the list end read slot i, in bounds and unused.
 __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restrict__ A,
                                                           const float4* __restrict__ xold,
                                                           const float4* __restrict__ solid,
@@ -666,16 +682,15 @@ __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restric
         int j[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) j[u] = (t + u < t1) ? idx[t + u] : i;
-        float4 pj[4], xoj[4];
+        float4 pj[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const bool sand = j[u] < n;
             pj[u] = sand ? A[j[u]] : solid[j[u] - n];
-            xoj[u] = xold[sand ? j[u] : i];
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (t + u < t1 && j[u] != i) sand_pair(pi, xoi, pj[u], xoj[u], j[u] < n, k, dpx, dpy, dpz);
+            if (t + u < t1 && j[u] != i) sand_pair(pi, xoi, pj[u], xold, j[u], j[u] < n, k, dpx, dpy, dpz);
     }
     float4 r;
     r.x = rclamp(pi.x + dpx, k.r, k.hi[0]);
