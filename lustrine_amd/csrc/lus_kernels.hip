@@ -382,15 +382,12 @@ hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, cons
 
 // ------------------------------------------------------------------ sand
 // integration loop of simulate_sand, src/Simulate.cpp:188-213
-__global__ __launch_bounds__(kBlock) void k_sand_predict_key(float4* __restrict__ x,
-                                                              float4* __restrict__ v,
-                                                              int* __restrict__ att, int n, SandK k,
-                                                              float dt, float4* __restrict__ xs,
-                                                              int* __restrict__ key,
-                                                              int* __restrict__ slot,
-                                                              int* __restrict__ counts) {
-    const int i = xcd_index();
-    if (i >= n) return;
+// Predicted position of sand slot i with gravity and the attract / blow
+// forces (src/Simulate.cpp:185-212); writes v, attracted, x*, key; returns
+// the cell.
+__device__ __forceinline__ int sand_predict(int i, float4* __restrict__ x, float4* __restrict__ v,
+                                            int* __restrict__ att, const SandK& k, float dt,
+                                            float4* __restrict__ xs, int* __restrict__ key) {
     const float4 p = x[i];
     float4 vi = v[i];
     const float w = k.w;
@@ -451,7 +448,22 @@ __global__ __launch_bounds__(kBlock) void k_sand_predict_key(float4* __restrict_
     xs[i] = q;
     const int c = cell_of(q.x, q.y, q.z, k.cs, k.gX, k.gZ);
     key[i] = c;
-    slot[i] = atomicAdd(&counts[c], 1);
+    return c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sand_predict_key(float4* __restrict__ x,
+                                                              float4* __restrict__ v,
+                                                              int* __restrict__ att, int n, SandK k,
+                                                              float dt, float4* __restrict__ xs,
+                                                              int* __restrict__ key,
+                                                              int* __restrict__ slot,
+                                                              int* __restrict__ counts) {
+    const int i = xcd_index();
+    const bool active = i < n;
+    const int c = active ? sand_predict(i, x, v, att, k, dt, xs, key) : 0;
+    // one atomic per run of equal keys in the wave (every lane calls it)
+    const int sl = wave_atomic_slot(counts, c, active);
+    if (active) slot[i] = sl;
 }
 
 hipError_t launch_sand_predict_key(float4* x, float4* v, int* att, int n, SandK k, float dt,

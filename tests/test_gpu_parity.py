@@ -330,6 +330,22 @@ def test_sand_steps_match_reference(gpu_mod, oracle):
     print(f"sand x5 ulp(x)={max_ulp(got[0], want[0])} ulp(v)={max_ulp(got[2], want[2])}")
 
 
+def test_sand_dense_clump_long_lists(gpu_mod, oracle):
+    # a clump at spacing 0.3 (about 200 neighbours within h, lists of 216
+    # entries) beside a sparse layer (lists of 5 to 20), over solids
+    clump = scenes.jittered_lattice(6, 6, 6, cell=0.3, origin=(8.0, 8.0, 8.0), amplitude=0.02, seed=11)
+    layer = scenes.jittered_lattice(10, 2, 10, origin=(2.0, 3.0, 2.0), amplitude=0.2, seed=12)
+    pos = np.concatenate([layer, clump]).astype(np.float32)
+    sol = scenes.lattice(20, 2, 20, origin=(0.0, 0.0, 0.0))
+    # one step: the lists are those of the first prediction, the clump intact
+    got, want, (goff, gidx), (ooff, oidx) = _sand_compare(gpu_mod, oracle, pos, sol, steps=1)
+    assert int(np.diff(ooff).max()) > 64
+    assert np.array_equal(got[4], want[4])
+    assert np.array_equal(goff, ooff) and np.array_equal(gidx, oidx)
+    for a, b in zip(got[:3], want[:3]):
+        assert rel_err(a, b) <= REL
+
+
 def test_sand_attract_and_blow_flags(gpu_mod, oracle):
     pos, sol = _scene_with_solids(n_side=8, seed=9)
     player = np.array([6.0, 6.0, 6.0], np.float32)
