@@ -610,12 +610,15 @@ hipError_t launch_sand_nbr(const float4* xs, const int* key2, const int* sand_cs
 __device__ __forceinline__ void sand_pair(const float4 pi, const float4 xoi, const float4 pj, const float4* __restrict__ xold, int jj,
                                           bool sand, const SandK& k, float& dpx, float& dpy, float& dpz) {
     float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
-    if (sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz)) == 0.0f) {
+    // |ij| == 0 -> ij = (0, 1e-5, 0) (src/Simulate.cpp:236-239); the length
+    // is computed again only then
+    float len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
+    if (len == 0.0f) {
         ijx = 0.0f;
         ijy = 0.00001f;
         ijz = 0.0f;
+        len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
     }
-    const float len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
     if (len > k.d) return;
     float sc;
     if (sand) {

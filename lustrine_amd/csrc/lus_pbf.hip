@@ -1167,12 +1167,15 @@ hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const 
 __device__ __forceinline__ void mix_sand_pair(const float4 pi, const float4 xoi, const float4 pj, const float4 xoj,
                                               int jph, const PbfK& k, float& dpx, float& dpy, float& dpz) {
     float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
-    if (sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz)) == 0.0f) {
+    // |ij| == 0 -> ij = (0, 1e-5, 0) (src/Simulate.cpp:236-239); the length
+    // is computed again only then
+    float len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
+    if (len == 0.0f) {
         ijx = 0.0f;
         ijy = 0.00001f;
         ijz = 0.0f;
+        len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
     }
-    const float len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
     if (len > k.mix_d) return;
     float sc;
     if (jph == 2) {
