@@ -539,14 +539,25 @@ __global__ __launch_bounds__(kBlock) void k_sand_nbr(const float4* __restrict__ 
     int n1 = 0, n2 = 0;
     for (int y = yl; y <= yu; ++y) {
         for (int x = xl; x <= xu; ++x) {
-            for (int z = zl; z <= zu; ++z) {
-                const int nc = (yy + y) * k.gXZ + (xx + x) * k.gZ + (zz + z);
+            // the row's z cells are consecutive: their 2 to 4 boundaries in
+            // both cell tables are requested together
+            const int row = (yy + y) * k.gXZ + (xx + x) * k.gZ + zz;
+            int sb[4], ob[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool in = zl + q <= zu + 1;
+                sb[q] = in ? scs[row + zl + q] : 0;
+                ob[q] = (in && solid) ? ocs[row + zl + q] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (zl + q > zu) break;
                 // candidates four at a time: the four loads are requested
                 // together, then tested in slot order (the last candidate of
                 // the cell stands in for places past its end)
                 if (solid) {
-                    const int je = ocs[nc + 1];
-                    for (int j0 = ocs[nc]; j0 < je; j0 += 4) {
+                    const int je = ob[q + 1];
+                    for (int j0 = ob[q]; j0 < je; j0 += 4) {
                         float4 q[4];
 #pragma unroll
                         for (int u = 0; u < 4; ++u) q[u] = solid[j0 + u < je ? j0 + u : je - 1];
@@ -560,8 +571,8 @@ __global__ __launch_bounds__(kBlock) void k_sand_nbr(const float4* __restrict__ 
                         }
                     }
                 }
-                const int je = scs[nc + 1];
-                for (int j0 = scs[nc]; j0 < je; j0 += 4) {
+                const int je = sb[q + 1];
+                for (int j0 = sb[q]; j0 < je; j0 += 4) {
                     float4 q[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) q[u] = xs[j0 + u < je ? j0 + u : je - 1];
