@@ -287,7 +287,9 @@ int lus_sand_init(LusGpu* ctx, const LusSandParams* p, int n_sand, const float* 
                   const float* vel, const int* attracted, int n_solid,
                   const float* solid_pos, const int* solid_index);
 /* One simulate_sand call.  player = Bullet player position (3 floats, may be
- * NULL when both flags are 0). */
+ * NULL when both flags are 0).  The call waits once for the device mid-step:
+ * the neighbour lists' total length sizes their buffer before they are
+ * written (the lists have no fixed capacity, as the reference's vectors). */
 int lus_sand_step(LusGpu* ctx, float dt, int attract_flag, int blow_flag, const float* player);
 /* State in the reference's host order (sorted slots of the last step).
  * perm (n_sand): the last counting-sort permutation, new slot -> old slot. */
