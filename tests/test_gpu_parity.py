@@ -346,6 +346,18 @@ def test_sand_dense_clump_long_lists(gpu_mod, oracle):
         assert rel_err(a, b) <= REL
 
 
+@pytest.mark.parametrize("lat", [(1, 1, 1), (1, 1, 2), (5, 13, 1)])
+def test_sand_ragged_sizes(gpu_mod, oracle, lat):
+    # 1, 2 and 65 sand particles over solids: tail lanes of every sand kernel
+    pos = scenes.jittered_lattice(*lat, origin=(6.0, 3.0, 6.0), amplitude=0.2, seed=sum(lat))
+    sol = scenes.lattice(20, 2, 20, origin=(0.0, 0.0, 0.0))
+    got, want, (goff, gidx), (ooff, oidx) = _sand_compare(gpu_mod, oracle, pos, sol, steps=3)
+    assert np.array_equal(got[4], want[4])
+    assert np.array_equal(goff, ooff) and np.array_equal(gidx, oidx)
+    for a, b in zip(got[:3], want[:3]):
+        assert rel_err(a, b) <= REL
+
+
 def test_sand_attract_and_blow_flags(gpu_mod, oracle):
     pos, sol = _scene_with_solids(n_side=8, seed=9)
     player = np.array([6.0, 6.0, 6.0], np.float32)
