@@ -93,18 +93,25 @@ hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, cons
                             int* idx, int fill, hipStream_t s);
 
 // Neighbour sets of a step, one buffer of nbr_words(n) 64-bit words.  Flat
-// passes: neighbour lists, cnt[i] (n ints, padded to 16 bytes), then per run
-// of 64 slots (w = i / 64, l = i % 64) kNbrCap / 4 groups of 64 int4, entries
-// 4g .. 4g+3 of slot i in int4 number (w * kNbrCap / 4 + g) * 64 + l; entries
-// in ascending slot order.  A particle with more than kNbrCap neighbours
-// re-tests its candidates in every pass.  Tiled passes: 9 row masks per
-// particle, masks[r * n + i].  A step uses one of the two forms.
+// passes: neighbour lists in ascending slot order.  First a header per slot,
+// hdr[i] = int4(b0, b1, b2, w): b_L is the slot of the first neighbour in the
+// stencil's y layer L (cell y - 1, y, y + 1; slot i when the layer has none),
+// w = |N(i)| | n0 << 16 | n01 << 24 with n0 the neighbours in layer 0 and n01
+// those in layers 0 and 1.  Then per run of 64 slots (w = i / 64, l = i % 64)
+// kNbrGroups groups of 64 int4, each int4 eight 16-bit entries: entry t of
+// slot i is in int4 number (w * kNbrGroups + t / 8) * 64 + l, bits
+// 16 (t % 8) .. 16 (t % 8) + 15 of the 128, and holds j - b_L for neighbour j
+// of layer L.  A particle with more than kNbrCap neighbours, or one whose
+// neighbours lie more than 65535 slots past their layer's first, has
+// w = kNbrRetest and re-tests its candidates in every pass.  Tiled passes: 9
+// row masks per particle, masks[r * n + i].  A step uses one of the two forms.
 constexpr int kNbrCap = 128;  // about 4x the rest-state count (a K = 1 solve compresses 2.5x)
-constexpr int kNbrGroups = kNbrCap / 4;
+constexpr int kNbrGroups = kNbrCap / 8;
+constexpr int kNbrRetest = 0xFFFF;
 // particles of one PBF context: float4 data of any slot lies within 4 GiB
 constexpr int kMaxParticles = (1 << 28) - 1;
 inline size_t nbr_words(size_t n) {
-    const size_t int4s = (n + 3) / 4 + (n + 63) / 64 * (size_t)kNbrGroups * 64;
+    const size_t int4s = n + (n + 63) / 64 * (size_t)kNbrGroups * 64;
     return 2 * int4s > 9 * n ? 2 * int4s : 9 * n;
 }
 

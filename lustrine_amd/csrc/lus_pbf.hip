@@ -18,7 +18,7 @@
 // the lists, one entry per iteration in every lane, so a wave runs
 // max_i |N(i)| iterations instead of the sum over the 9 rows of the longest
 // row (about half as many on a dam break).  A particle with more than
-// kNbrCap neighbours re-tests its candidates against the stored predicted
+// kNbrCap neighbours (or with an entry beyond 16 bits) re-tests its candidates against the stored predicted
 // positions in every pass, which gives the same set in the same order.  The
 // tiled passes keep the row masks instead (masks[r * n + i], bit t = slot
 // j0_r + t) and flag a particle whose stencil has a row longer than 64 slots
@@ -100,6 +100,14 @@ __device__ __forceinline__ bool within_h(const float4 a, const float4 b, const P
     const float r2 = dx * dx + dy * dy + dz * dz;
     return r2 <= k.h2;
 }
+__device__ __forceinline__ bool within_h(const float4 a, const float3 b, const PbfK& k) {
+    return within_h(a, make_float4(b.x, b.y, b.z, 0.0f), k);
+}
+// x, y, z of slot t of a particle array (one 12-byte load: a candidate test
+// reads no more)
+__device__ __forceinline__ float3 xyz_at(const float4* __restrict__ a, int t) {
+    return *reinterpret_cast<const float3*>(a + t);
+}
 
 __device__ __forceinline__ int low_bit(unsigned int m) { return __builtin_ctz(m); }
 __device__ __forceinline__ int low_bit(unsigned long long m) { return __builtin_ctzll(m); }
@@ -142,11 +150,12 @@ __device__ __forceinline__ float4 at4(const float4* __restrict__ a, int j) {
     return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(a) + (unsigned)j * 16u);
 }
 
-// Neighbour-list buffer (layout in lus_internal.hpp): counts, then int4 groups.
-__device__ __forceinline__ int* nbr_cnt(unsigned long long* nb) { return reinterpret_cast<int*>(nb); }
+// Neighbour-list buffer (layout in lus_internal.hpp): headers, then int4
+// groups of eight 16-bit entries.
+__device__ __forceinline__ int4* nbr_hdr(unsigned long long* nb) { return reinterpret_cast<int4*>(nb); }
 // group 0 of slot i's list; group g is 64 int4 further
 __device__ __forceinline__ int4* nbr_list(unsigned long long* nb, int n, int i) {
-    int4* lists = reinterpret_cast<int4*>(nb) + ((size_t)n + 3) / 4;
+    int4* lists = reinterpret_cast<int4*>(nb) + (size_t)n;
     return lists + (size_t)(i >> 6) * (kNbrGroups * 64) + (i & 63);
 }
 
@@ -154,12 +163,12 @@ __device__ __forceinline__ int4* nbr_list(unsigned long long* nb, int n, int i) 
 // ascending slot order -- a row of at most 64 slots through a 64-bit hit mask
 // (32-bit mask arithmetic when no lane of the wave has a hit above bit 31), a
 // longer row slot by slot -- and call f(j, cur[j], aux[j]) on every hit while
-// appending it to i's list four at a time (one 16-byte store per group, so
-// the lanes of a wave write whole segments); entries past kNbrCap are
-// counted, not stored.  Writes cnt[i] = |N(i)|.
+// appending it to i's list, eight 16-bit entries at a time (one 16-byte store
+// per group, so the lanes of a wave write whole segments); entries past
+// kNbrCap are counted, not stored.  Writes the header hdr[i].
 template <class F>
 __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, const int* __restrict__ cs,
-                                           const float4* __restrict__ xp, const PbfK& k, int* __restrict__ cnt,
+                                           const float4* __restrict__ xp, const PbfK& k, int4* __restrict__ hdr,
                                            int4* __restrict__ lst, const float4* __restrict__ cur,
                                            const float4* __restrict__ aux, F& f) {
     const int key = key2[i] & ~kOvf;
@@ -171,18 +180,38 @@ __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, 
     const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
     const float4 pi = xp[i];
     int c = 0;
-    int4 q = make_int4(i, i, i, i);
+    int lb = -1;                     // first neighbour of the current layer (-1: none yet)
+    int b0 = i, b1 = i;
+    // min(n0, 255) | min(n01, 255) << 8, bit 16: an entry does not fit 16 bits
+    int meta = 0;
+    unsigned int q0 = 0u, q1 = 0u, q2 = 0u, q3 = 0u;  // the group being filled, entry 0 lowest
+    auto push = [&](unsigned int e) {
+        q0 = (q0 >> 16) | (q1 << 16);
+        q1 = (q1 >> 16) | (q2 << 16);
+        q2 = (q2 >> 16) | (q3 << 16);
+        q3 = (q3 >> 16) | (e << 16);
+    };
     auto emit = [&](int j, const float4 pj, const float4 aj) {
-        q.x = q.y;
-        q.y = q.z;
-        q.z = q.w;
-        q.w = j;
+        lb = lb < 0 ? j : lb;
+        const unsigned int e = (unsigned int)(j - lb);
+        meta |= e > 0xFFFFu ? 0x10000 : 0;
+        push(e & 0xFFFFu);
         ++c;
-        if ((c & 3) == 0 && c <= kNbrCap) lst[(size_t)((c >> 2) - 1) * 64] = q;
+        if ((c & 7) == 0 && c <= kNbrCap)
+            lst[(size_t)((c >> 3) - 1) * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
         f(j, pj, aj);
     };
 #pragma unroll 1
     for (int r = 0; r < 9; ++r) {
+        if (r == 3) {
+            meta |= c < 255 ? c : 255;
+            b0 = lb < 0 ? i : lb;
+            lb = -1;
+        } else if (r == 6) {
+            meta |= (c < 255 ? c : 255) << 8;
+            b1 = lb < 0 ? i : lb;
+            lb = -1;
+        }
         const int y = cy + r / 3 - 1;
         const int x = cx + r % 3 - 1;
         if (y < 0 || y >= k.gY || x < 0 || x >= k.gX) continue;
@@ -199,98 +228,120 @@ __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, 
         const float4* __restrict__ base = xp + j0;
         int t = 0;
         for (; t + 4 <= L; t += 4) {
-            const float4 q0 = base[t], q1 = base[t + 1], q2 = base[t + 2], q3 = base[t + 3];
-            mm |= within_h(pi, q0, k) ? (1ull << t) : 0ull;
-            mm |= within_h(pi, q1, k) ? (2ull << t) : 0ull;
-            mm |= within_h(pi, q2, k) ? (4ull << t) : 0ull;
-            mm |= within_h(pi, q3, k) ? (8ull << t) : 0ull;
+            const float3 c0 = xyz_at(base, t), c1 = xyz_at(base, t + 1), c2 = xyz_at(base, t + 2),
+                         c3 = xyz_at(base, t + 3);
+            mm |= within_h(pi, c0, k) ? (1ull << t) : 0ull;
+            mm |= within_h(pi, c1, k) ? (2ull << t) : 0ull;
+            mm |= within_h(pi, c2, k) ? (4ull << t) : 0ull;
+            mm |= within_h(pi, c3, k) ? (8ull << t) : 0ull;
         }
-        for (; t < L; ++t) mm |= within_h(pi, base[t], k) ? (1ull << t) : 0ull;
+        for (; t < L; ++t) mm |= within_h(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
         if (__any((int)((mm >> 32) != 0ull)))
             visit_row<unsigned long long>(mm, j0, 0, cur, aux, emit);
         else
             visit_row<unsigned int>((unsigned int)mm, j0, 0, cur, aux, emit);
     }
-    // the last, partial group: entries to the front, slot i in the unused places
-    const int rem = c & 3;
+    const int b2 = lb < 0 ? i : lb;
+    // the last, partial group: zero entries in the unused places (they decode
+    // to b2, a particle's slot, loaded by a walk but not visited)
+    const int rem = c & 7;
     if (rem != 0 && c < kNbrCap) {
-        for (int s = rem; s < 4; ++s) {
-            q.x = q.y;
-            q.y = q.z;
-            q.z = q.w;
-            q.w = i;
-        }
-        lst[(size_t)(c >> 2) * 64] = q;
+        for (int u = rem; u < 8; ++u) push(0u);
+        lst[(size_t)(c >> 3) * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
     }
-    cnt[i] = c;
+    // c <= kNbrCap: n0 and n01 are below 255
+    const int w = (c > kNbrCap || (meta & 0x10000)) ? kNbrRetest : (c | (meta & 0xFFFF) << 16);
+    hdr[i] = make_int4(b0, b1, b2, w);
 }
 
-// Walk a list of c <= kNbrCap entries, calling f(j, cur[j], aux[j]) in list
-// order, four entries (one group) at a time: the group's neighbour data are
-// requested, then the next group's entries (so waiting for the data never
-// waits for the list), then f runs on the four.  Every lane of a wave is in
-// the same group, so a group load is one whole 1 KB segment.  Places of the
-// last group past c hold slot i (in-bounds data, not visited); the next
-// group's entries are used only when the walk goes on.  (Masking the loads
-// of places past c, or requesting the next group's data before f runs on
-// the current one, measured slower.)  kWide walks two groups per step: the
-// plain lambda and delta-p passes stay within 64 VGPRs with it, so the eight
-// gathers in flight cost no occupancy (about 1.4 % less time per launch);
-// XSPH (two gathers per neighbour) and the mixed-scene passes walk one.
-template <bool kWide, class F>
-__device__ __forceinline__ void visit_list(int c, const int4* __restrict__ lst, const float4* __restrict__ cur,
-                                           const float4* __restrict__ aux, F& f) {
+// Walk forms of visit_list: kWalkWide, one group of eight entries per step
+// with all eight gathers in flight (the plain lambda and delta-p passes: one
+// gather per neighbour, within 64 VGPRs); kWalkPairs, one group per step in
+// two halves of four (XSPH: two gathers per neighbour); kWalkHalf, one half
+// per loop step (the mixed-scene passes, whose per-neighbour bodies are long:
+// four copies of the body instead of eight).
+constexpr int kWalkHalf = 0, kWalkPairs = 1, kWalkWide = 2;
+
+// Walk the list of header hd (|N(i)| <= kNbrCap), calling f(j, cur[j],
+// aux[j]) in list order.  A step's neighbour data are requested, then the
+// next group's entries (so waiting for the data never waits for the list),
+// then f runs.  Every lane of a wave is in the same group, so a group load is
+// one whole 1 KB segment.  Places of the last group past |N(i)| decode to
+// slot b2 (in-bounds data, not visited); the next group is used only when the
+// walk goes on.
+template <int kWalk, class F>
+__device__ __forceinline__ void visit_list(const int4 hd, const int4* __restrict__ lst,
+                                           const float4* __restrict__ cur, const float4* __restrict__ aux,
+                                           F& f) {
+    const int c = hd.w & 0xFFFF;
     if (c <= 0) return;
+    const int n0 = (hd.w >> 16) & 0xFF, n01 = (int)((unsigned int)hd.w >> 24);
+    // slot of entry t holding e
+    auto slot = [&](int t, unsigned int e) { return (t < n0 ? hd.x : (t < n01 ? hd.y : hd.z)) + (int)e; };
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    int4 e = lst[0];
-    if constexpr (kWide) {
-        // kWide (passes with one gather per neighbour and registers to
-        // spare): two groups, eight gathers, in flight per step.  The second
-        // group's entries stand in for the first group's (in-bounds) when
-        // the list ends inside the first.
-        int4 e2 = c > 4 ? lst[64] : e;
-        for (int t = 0; t < c; t += 8) {
-            const float4 p0 = at4(cur, e.x), p1 = at4(cur, e.y), p2 = at4(cur, e.z), p3 = at4(cur, e.w);
-            const float4 p4 = at4(cur, e2.x), p5 = at4(cur, e2.y), p6 = at4(cur, e2.z), p7 = at4(cur, e2.w);
-            const int g = (t >> 2) + 2 < kNbrGroups ? (t >> 2) + 2 : kNbrGroups - 1;
-            const int g2 = g + 1 < kNbrGroups ? g + 1 : kNbrGroups - 1;
-            const int4 en = lst[(size_t)g * 64];
-            const int4 en2 = (t + 12 < c) ? lst[(size_t)g2 * 64] : en;
-            asm volatile("" ::"v"(p0.x), "v"(p0.y), "v"(p0.z), "v"(p0.w));
-            asm volatile("" ::"v"(p4.x), "v"(p4.y), "v"(p4.z), "v"(p4.w));
-            f(e.x, p0, z);
-            if (t + 1 < c) f(e.y, p1, z);
-            if (t + 2 < c) f(e.z, p2, z);
-            if (t + 3 < c) f(e.w, p3, z);
-            if (t + 4 < c) f(e2.x, p4, z);
-            if (t + 5 < c) f(e2.y, p5, z);
-            if (t + 6 < c) f(e2.z, p6, z);
-            if (t + 7 < c) f(e2.w, p7, z);
-            e = en;
-            e2 = en2;
-        }
-        return;
-    }
-    for (int t = 0; t < c; t += 4) {
-        const float4 p0 = at4(cur, e.x), p1 = at4(cur, e.y), p2 = at4(cur, e.z), p3 = at4(cur, e.w);
-        float4 q0 = z, q1 = z, q2 = z, q3 = z;
+    // four entries from the two words wa (t, t + 1) and wb (t + 2, t + 3)
+    auto visit4 = [&](int t, unsigned int wa, unsigned int wb, auto&& between) {
+        const int j0 = slot(t, wa & 0xFFFFu), j1 = slot(t + 1, wa >> 16);
+        const int j2 = slot(t + 2, wb & 0xFFFFu), j3 = slot(t + 3, wb >> 16);
+        const float4 p0 = at4(cur, j0), p1 = at4(cur, j1), p2 = at4(cur, j2), p3 = at4(cur, j3);
+        float4 a0 = z, a1 = z, a2 = z, a3 = z;
         if (aux) {
-            q0 = at4(aux, e.x);
-            q1 = at4(aux, e.y);
-            q2 = at4(aux, e.z);
-            q3 = at4(aux, e.w);
+            a0 = at4(aux, j0);
+            a1 = at4(aux, j1);
+            a2 = at4(aux, j2);
+            a3 = at4(aux, j3);
         }
-        const int g = (t >> 2) + 1 < kNbrGroups ? (t >> 2) + 1 : kNbrGroups - 1;
-        const int4 en = lst[(size_t)g * 64];
-        // keep every component of the first neighbour in the group's loads
+        between();
+        // keep every component of the first neighbour in the step's loads
         // (a component used only under f's branch would be loaded there,
         // after the next group's entries, and wait for them)
         asm volatile("" ::"v"(p0.x), "v"(p0.y), "v"(p0.z), "v"(p0.w));
-        f(e.x, p0, q0);
-        if (t + 1 < c) f(e.y, p1, q1);
-        if (t + 2 < c) f(e.z, p2, q2);
-        if (t + 3 < c) f(e.w, p3, q3);
-        e = en;
+        f(j0, p0, a0);
+        if (t + 1 < c) f(j1, p1, a1);
+        if (t + 2 < c) f(j2, p2, a2);
+        if (t + 3 < c) f(j3, p3, a3);
+    };
+    int4 g = lst[0];
+    if constexpr (kWalk == kWalkHalf) {
+        int4 gn = g;
+        for (int t = 0; t < c; t += 4) {
+            const bool lo = (t & 4) == 0;  // uniform
+            const int gn_i = (t >> 3) + 1 < kNbrGroups ? (t >> 3) + 1 : kNbrGroups - 1;
+            visit4(t, (unsigned int)(lo ? g.x : g.z), (unsigned int)(lo ? g.y : g.w), [&] {
+                if (lo) gn = lst[(size_t)gn_i * 64];
+            });
+            if (!lo) g = gn;
+        }
+        return;
+    }
+    for (int t = 0; t < c; t += 8) {
+        const int gn_i = (t >> 3) + 1 < kNbrGroups ? (t >> 3) + 1 : kNbrGroups - 1;
+        const unsigned int w0 = (unsigned int)g.x, w1 = (unsigned int)g.y;
+        const unsigned int w2 = (unsigned int)g.z, w3 = (unsigned int)g.w;
+        int4 gn;
+        if constexpr (kWalk == kWalkWide) {
+            const int j0 = slot(t, w0 & 0xFFFFu), j1 = slot(t + 1, w0 >> 16);
+            const int j2 = slot(t + 2, w1 & 0xFFFFu), j3 = slot(t + 3, w1 >> 16);
+            const int j4 = slot(t + 4, w2 & 0xFFFFu), j5 = slot(t + 5, w2 >> 16);
+            const int j6 = slot(t + 6, w3 & 0xFFFFu), j7 = slot(t + 7, w3 >> 16);
+            const float4 p0 = at4(cur, j0), p1 = at4(cur, j1), p2 = at4(cur, j2), p3 = at4(cur, j3);
+            const float4 p4 = at4(cur, j4), p5 = at4(cur, j5), p6 = at4(cur, j6), p7 = at4(cur, j7);
+            gn = lst[(size_t)gn_i * 64];
+            asm volatile("" ::"v"(p0.x), "v"(p0.y), "v"(p0.z), "v"(p0.w));
+            asm volatile("" ::"v"(p4.x), "v"(p4.y), "v"(p4.z), "v"(p4.w));
+            f(j0, p0, z);
+            if (t + 1 < c) f(j1, p1, z);
+            if (t + 2 < c) f(j2, p2, z);
+            if (t + 3 < c) f(j3, p3, z);
+            if (t + 4 < c) f(j4, p4, z);
+            if (t + 5 < c) f(j5, p5, z);
+            if (t + 6 < c) f(j6, p6, z);
+            if (t + 7 < c) f(j7, p7, z);
+        } else {
+            visit4(t, w0, w1, [&] { gn = lst[(size_t)gn_i * 64]; });
+            if (t + 4 < c) visit4(t + 4, w2, w3, [] {});
+        }
+        g = gn;
     }
 }
 
@@ -299,7 +350,7 @@ __device__ __forceinline__ void visit_list(int c, const int4* __restrict__ lst, 
 // walk the stored list.  A particle whose list overflowed re-tests its candidates
 // against the stored predicted positions xp (the same set in the same order).
 // nb: the neighbour-list buffer.
-template <bool kBuild, bool kWide = false, class F>
+template <bool kBuild, int kWalk = kWalkHalf, class F>
 __device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict__ key2,
                                                   const int* __restrict__ cs,
                                                   unsigned long long* __restrict__ nb,
@@ -307,15 +358,14 @@ __device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict_
                                                   const float4* __restrict__ cur,
                                                   const float4* __restrict__ aux, const PbfK& k,
                                                   F&& f) {
-    int* __restrict__ cnt = nbr_cnt(nb);
     int4* __restrict__ lst = nbr_list(nb, n, i);
     if (kBuild) {
-        build_list(i, key2, cs, xp, k, cnt, lst, cur, aux, f);
+        build_list(i, key2, cs, xp, k, nbr_hdr(nb), lst, cur, aux, f);
         return;
     }
-    const int c = cnt[i];
-    if (c <= kNbrCap) {
-        visit_list<kWide>(c, lst, cur, aux, f);
+    const int4 hd = nbr_hdr(nb)[i];
+    if ((hd.w & 0xFFFF) <= kNbrCap) {
+        visit_list<kWalk>(hd, lst, cur, aux, f);
         return;
     }
     const int key = key2[i] & ~kOvf;
@@ -799,7 +849,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict_
     float Ci = -1e30f;
     if (i < n)
         Ci = lambda_body<kFast>(i, xa, xl, k, [&](auto&& f) {
-            for_each_neighbor<kBuild, true>(i, n, key2, cs, masks, xp, xa, nullptr, k, f);
+            for_each_neighbor<kBuild, kWalkWide>(i, n, key2, cs, masks, xp, xa, nullptr, k, f);
         });
     block_max_store(Ci, maxc);
 }
@@ -952,7 +1002,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_deltap(const float4* __restrict_
     const int i = xcd_index();
     if (i >= n) return;
     deltap_body<kFast, kNS>(i, xl, k, xa, xold, vout, dt, [&](auto&& f) {
-        for_each_neighbor<false, true>(i, n, key2, cs, masks, xp, xl, nullptr, k, f);
+        for_each_neighbor<false, kWalkWide>(i, n, key2, cs, masks, xp, xl, nullptr, k, f);
     });
 }
 
@@ -1082,7 +1132,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_xsph(const float4* __restrict__ 
     float Ci = -1e30f;
     if (i < n)
         Ci = xsph_body<kFast, kCheck>(i, xa, vt, k, vout, xout, [&](auto&& f) {
-            for_each_neighbor<false>(i, n, key2, cs, masks, xp, xa, vt, k, f);
+            for_each_neighbor<false, kWalkPairs>(i, n, key2, cs, masks, xp, xa, vt, k, f);
         });
     if (kCheck) block_max_sum_store(Ci, Ci > 0.0f ? Ci : 0.0f, maxd);
 }
@@ -1366,7 +1416,7 @@ __global__ __launch_bounds__(kBlock) void k_mix_xsph(const float4* __restrict__ 
             xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
         } else {
             float sx = 0.0f, sy = 0.0f, sz = 0.0f, rho = 0.0f;
-            for_each_neighbor<false>(i, n, key2, cs, masks, xp, xa, vt, k,
+            for_each_neighbor<false, kWalkPairs>(i, n, key2, cs, masks, xp, xa, vt, k,
                                      [&](int j, const float4 pj, const float4 vj) {
                 const float dx = pi.x - pj.x;
                 const float dy = pi.y - pj.y;

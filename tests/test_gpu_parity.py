@@ -282,6 +282,33 @@ def test_pbf_neighbour_list_overflow_mixed(gpu_mod, oracle):
     assert np.array_equal(gx, x) and np.array_equal(gv, v)
 
 
+def test_pbf_list_entries_beyond_16_bits(gpu_mod, oracle):
+    # a slab three cell columns wide (x), one cell high (y) and 4,200 cells deep
+    # (z): in cell order a particle of the middle column has neighbours in the
+    # columns either side whose slots lie one column's population (33,600)
+    # before and after, 67,200 apart -- more than a 16-bit list entry holds
+    # past its layer's first neighbour.  Those particles re-test their
+    # candidates in every pass while the outer columns walk lists; exact mode,
+    # bit for bit over two steps.
+    cfg = scenes.small(6, 2, 8400, box=(6, 8, 16800))
+    x0 = cfg.positions()
+    cell = np.floor(x0 / np.float32(cfg.h)).astype(np.int64)
+    assert set(cell[:, 1]) == {0} and set(cell[:, 0]) == {0, 1, 2}
+    pop = np.bincount(cell[:, 0])
+    assert pop[0] + pop[1] > 65535, pop
+    p = oracle.pbf_params(cfg)
+    x, v = x0.copy(), np.zeros_like(x0)
+    ids = np.arange(x0.shape[0], dtype=np.int32)
+    for _ in range(2):
+        oracle.pbf_step(p, x, v, ids, np.float32(cfg.dt))
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_init(gpu_mod.pbf_params(cfg), x0, None)
+        g.pbf_step(np.float32(cfg.dt), 2)
+        gx, gv, gid = g.pbf_download(order=0)
+    assert np.array_equal(gid, ids)
+    assert np.array_equal(gx, x) and np.array_equal(gv, v), (max_ulp(gx, x), max_ulp(gv, v))
+
+
 def test_pbf_original_order_download(gpu_mod):
     cfg = scenes.small(10, 10, 10, box=(20, 20, 20))
     x0 = cfg.positions()
