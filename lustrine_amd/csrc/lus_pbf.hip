@@ -257,9 +257,10 @@ __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, 
 // Walk forms of visit_list: kWalkWide, one group of eight entries per step
 // with all eight gathers in flight (the plain lambda and delta-p passes: one
 // gather per neighbour, within 64 VGPRs); kWalkPairs, one group per step in
-// two halves of four (XSPH: two gathers per neighbour); kWalkHalf, one half
-// per loop step (the mixed-scene passes, whose per-neighbour bodies are long:
-// four copies of the body instead of eight).
+// two halves of four (the XSPH passes of both scenes: two gathers per
+// neighbour); kWalkHalf, one half per loop step (the mixed scene's lambda and
+// correction passes, whose per-neighbour bodies are long: four copies of the
+// body instead of eight).
 constexpr int kWalkHalf = 0, kWalkPairs = 1, kWalkWide = 2;
 
 // Walk the list of header hd (|N(i)| <= kNbrCap), calling f(j, cur[j],
