@@ -96,49 +96,55 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        _lib = C.CDLL(build())
-        L = _lib
-        L.orc_init_constants.argtypes = [C.POINTER(OrcSim), C.c_int, C.c_int, C.c_int,
-                                         C.c_float, C.c_float, C.c_float]
-        L.orc_cubic_kernel.restype = C.c_float
-        L.orc_cubic_kernel.argtypes = [C.POINTER(OrcSim), C.c_float]
-        L.orc_cubic_kernel_grad.argtypes = [C.POINTER(OrcSim), C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.orc_poly6_kernel.restype = C.c_float
-        L.orc_poly6_kernel.argtypes = [C.POINTER(OrcSim), C.c_float]
-        L.orc_poly6_kernel_vec.restype = C.c_float
-        L.orc_poly6_kernel_vec.argtypes = [C.POINTER(OrcSim), C.POINTER(C.c_float)]
-        L.orc_spiky_kernel.argtypes = [C.POINTER(OrcSim), C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.orc_cell_ids.argtypes = [C.POINTER(OrcSim), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]
-        L.orc_counting_sort.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_size_t,
-                                        C.c_size_t, C.POINTER(C.c_int)]
-        L.orc_neighbors_fluid.restype = C.c_long
-        L.orc_neighbors_fluid.argtypes = [C.POINTER(OrcSim), C.POINTER(C.c_float), C.c_int, C.c_int,
-                                          C.c_int, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_int),
-                                          C.c_long]
-        L.orc_simulate_sand.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState), C.c_float]
-        L.orc_sand_state_free.argtypes = [C.POINTER(OrcSandState)]
-        L.orc_query_cell_num_particles.restype = C.c_int
-        L.orc_query_cell_num_particles.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState),
-                                                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]
-        L.orc_sink_evict.restype = C.c_int
-        L.orc_sink_evict.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState), C.POINTER(C.c_int), C.c_int]
-        L.orc_simulate_fluid.argtypes = [C.POINTER(OrcSim)] + [C.POINTER(C.c_float)] * 4 + \
-            [C.c_int] * 4 + [C.c_float]
-        L.orc_pbf_step_f32.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_float),
-                                       C.POINTER(C.c_float), C.POINTER(C.c_int), C.c_float,
-                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.orc_pbf_step_f64.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_double),
-                                       C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_double,
-                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        L.orc_mixed_step_f32.argtypes = [C.POINTER(OrcPbfParams), C.POINTER(OrcMixParams), C.c_int,
-                                         C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int),
-                                         C.POINTER(C.c_int), C.c_float, C.POINTER(C.c_float)]
-        L.orc_mixed_step_f64.argtypes = [C.POINTER(OrcPbfParams), C.POINTER(OrcMixParams), C.c_int,
-                                         C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int),
-                                         C.POINTER(C.c_int), C.c_double, C.POINTER(C.c_double)]
-        L.orc_pbf_density_error_f64.restype = C.c_double
-        L.orc_pbf_density_error_f64.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_double)]
+        _lib = load(build())
     return _lib
+
+
+def load(path: str):
+    """A configured handle on an oracle library at `path` (the built one, or a
+    variant a test compiled from modified sources)."""
+    L = C.CDLL(path)
+    L.orc_init_constants.argtypes = [C.POINTER(OrcSim), C.c_int, C.c_int, C.c_int,
+                                     C.c_float, C.c_float, C.c_float]
+    L.orc_cubic_kernel.restype = C.c_float
+    L.orc_cubic_kernel.argtypes = [C.POINTER(OrcSim), C.c_float]
+    L.orc_cubic_kernel_grad.argtypes = [C.POINTER(OrcSim), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_poly6_kernel.restype = C.c_float
+    L.orc_poly6_kernel.argtypes = [C.POINTER(OrcSim), C.c_float]
+    L.orc_poly6_kernel_vec.restype = C.c_float
+    L.orc_poly6_kernel_vec.argtypes = [C.POINTER(OrcSim), C.POINTER(C.c_float)]
+    L.orc_spiky_kernel.argtypes = [C.POINTER(OrcSim), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_cell_ids.argtypes = [C.POINTER(OrcSim), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]
+    L.orc_counting_sort.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_size_t,
+                                    C.c_size_t, C.POINTER(C.c_int)]
+    L.orc_neighbors_fluid.restype = C.c_long
+    L.orc_neighbors_fluid.argtypes = [C.POINTER(OrcSim), C.POINTER(C.c_float), C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_int),
+                                      C.c_long]
+    L.orc_simulate_sand.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState), C.c_float]
+    L.orc_sand_state_free.argtypes = [C.POINTER(OrcSandState)]
+    L.orc_query_cell_num_particles.restype = C.c_int
+    L.orc_query_cell_num_particles.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState),
+                                               C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]
+    L.orc_sink_evict.restype = C.c_int
+    L.orc_sink_evict.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState), C.POINTER(C.c_int), C.c_int]
+    L.orc_simulate_fluid.argtypes = [C.POINTER(OrcSim)] + [C.POINTER(C.c_float)] * 4 + \
+        [C.c_int] * 4 + [C.c_float]
+    L.orc_pbf_step_f32.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_int), C.c_float,
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_pbf_step_f64.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_double,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_mixed_step_f32.argtypes = [C.POINTER(OrcPbfParams), C.POINTER(OrcMixParams), C.c_int,
+                                     C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.c_float, C.POINTER(C.c_float)]
+    L.orc_mixed_step_f64.argtypes = [C.POINTER(OrcPbfParams), C.POINTER(OrcMixParams), C.c_int,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.c_double, C.POINTER(C.c_double)]
+    L.orc_pbf_density_error_f64.restype = C.c_double
+    L.orc_pbf_density_error_f64.argtypes = [C.POINTER(OrcPbfParams), C.c_int, C.POINTER(C.c_double)]
+    return L
 
 
 # ---------------------------------------------------------------- reference API
@@ -362,19 +368,20 @@ def pbf_params(cfg, iterations=None) -> OrcPbfParams:
     return p
 
 
-def pbf_step(params: OrcPbfParams, x, v, ids, dt, fp64=False, final_err=False):
+def pbf_step(params: OrcPbfParams, x, v, ids, dt, fp64=False, final_err=False, L=None):
     """One substep in place; returns max density constraint of the last lambda
     pass, and with final_err=True also (max(0, max C), mean max(0, C)) at the
-    final positions."""
+    final positions.  L: an oracle handle from load() (default: the built one)."""
     n = x.shape[0]
+    L = lib() if L is None else L
     if fp64:
         assert x.dtype == np.float64 and v.dtype == np.float64
         mc, fe = C.c_double(0), (C.c_double * 2)()
-        lib().orc_pbf_step_f64(C.byref(params), n, _dp(x), _dp(v), _ip(ids), dt, C.byref(mc), fe)
+        L.orc_pbf_step_f64(C.byref(params), n, _dp(x), _dp(v), _ip(ids), dt, C.byref(mc), fe)
     else:
         assert x.dtype == np.float32 and v.dtype == np.float32
         mc, fe = C.c_float(0), (C.c_float * 2)()
-        lib().orc_pbf_step_f32(C.byref(params), n, _fp(x), _fp(v), _ip(ids), dt, C.byref(mc), fe)
+        L.orc_pbf_step_f32(C.byref(params), n, _fp(x), _fp(v), _ip(ids), dt, C.byref(mc), fe)
     return (mc.value, fe[0], fe[1]) if final_err else mc.value
 
 

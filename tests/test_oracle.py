@@ -202,21 +202,52 @@ def test_sand_head_on_contact_closed_form(oracle):
     assert np.allclose(st.v[:2, 0], (p[:, 0] - pos[st.perm(), 0]) / dt, rtol=1e-6)
 
 
-def test_sand_static_friction_holds_on_solid(oracle):
-    # a sand particle pressed into a solid with a small tangential slide: the
-    # static branch (mu_s * d > |x_tan|) removes 0.7 of the slide per substep
+def _sand_on_solid(oracle, slide0, overlap0):
+    # one sand particle resting on a solid particle (old position at contact
+    # distance 1 above it, shifted slide0 to the -x side) and predicted exactly
+    # above it, overlap0 into it (x* = x_old + v dt with dyadic dt and v, so
+    # the first contact normal is exactly +y).  The tangential displacement
+    # friction acts on is then the slide (src/Simulate.cpp:276-286: x_delta is
+    # taken against the old position, x_j's delta is 0), and the normal part
+    # of x_delta is only what the contact has not yet pushed back.
     s = oracle.init_constants(16, 16, 16, 0.5, 1.0, 3.1)
     s.gravity[:] = [0.0, 0.0, 0.0]
-    solid = np.array([[8.0, 7.0, 8.0]], np.float32)
-    pos = np.array([[8.0, 7.5, 8.0]], np.float32)  # overlap 0.5
-    st = oracle.SandStream(s, pos, solid)
-    st.v[0] = [0.2, 0.0, 0.0]  # slides 0.002 in x during the predict
-    st.step(np.float32(0.01))
-    x = st.p[0]
-    # normal push: overlap shrinks by 0.2 per substep (full weight 0.8 vs solid)
-    assert abs((1.0 - (x[1] - 7.0)) - 0.5 * 0.2 ** 4) < 1e-5
-    # tangential: slide 0.002 is reduced (static friction), never reversed
-    assert 0.0 <= x[0] - 8.0 < 0.002
+    dt = np.float32(1.0 / 64.0)
+    solid = np.array([[1.0, 0.0, 1.0]], np.float32)
+    old = np.array([[np.float32(1.0) - np.float32(slide0), 1.0, 1.0]], np.float32)
+    st = oracle.SandStream(s, old, solid)
+    st.v[0] = [np.float32(slide0) / dt, -np.float32(overlap0) / dt, 0.0]
+    pred = old[0] + st.v[0] * dt
+    assert np.array_equal(pred, np.array([1.0, 1.0 - overlap0, 1.0], np.float32))
+    st.step(dt)
+    x = st.p[0].astype(np.float64)
+    return x[0] - float(old[0, 0]), 1.0 - float(np.linalg.norm(x - solid[0]))
+
+
+def test_sand_static_friction_closed_form(oracle):
+    # static branch (mu_s * d > |x_tan| + 1e-9, mu_s = 0.95): each of the 4
+    # substeps removes friction_coeff = 0.7 of the slide, so it ends at 0.3^4
+    # of its start; the overlap ends at 0.2^4 of its start (collision 0.8 at
+    # full weight against a solid).  mu_s * 0.8 * overlap_k stays above the
+    # slide in every substep; the contact normal tilts by at most slide0 rad
+    # after the first substep, which moves the closed forms by well under 1%.
+    # (Friction 0.5 would end at 0.5^4 of the slide, the kinetic branch at the
+    # slide less 0.7 * 0.9 * 0.8 * overlap per substep.)
+    slide0, overlap0 = 1.0 / 1024.0, 1.0 / 128.0
+    slide, overlap = _sand_on_solid(oracle, slide0, overlap0)
+    assert abs(slide / (slide0 * 0.3 ** 4) - 1.0) < 0.01, slide / slide0
+    assert abs(overlap / (overlap0 * 0.2 ** 4) - 1.0) < 0.01, overlap / overlap0
+
+
+def test_sand_kinetic_friction_closed_form(oracle):
+    # kinetic branch (mu_s * d <= |x_tan|): the slide shrinks by
+    # friction_coeff * mu_k * d = 0.7 * 0.9 * (0.8 * overlap_k) per substep,
+    # overlap_k = 0.2^k overlap0: in total 0.63 * overlap0 * (1 - 0.2^4)
+    slide0, overlap0 = 1.0 / 8.0, 1.0 / 128.0
+    slide, overlap = _sand_on_solid(oracle, slide0, overlap0)
+    removed = 0.7 * 0.9 * overlap0 * (1 - 0.2 ** 4)
+    assert abs((slide0 - slide) / removed - 1.0) < 0.01, (slide0 - slide) / removed
+    assert abs(overlap / (overlap0 * 0.2 ** 4) - 1.0) < 0.01
 
 
 # ------------------------------------------------------------------ fluid reference path
