@@ -95,6 +95,20 @@ int lus_cell_ids(const LusGrid* g, int n, const float* pos, int* out);
  * reference leaves behind (first slot of each key; counts[num_cells] = n). */
 int lus_counting_sort(int* counts, const int* keys, size_t n, size_t num_cells, int* sorted);
 
+/* The same counting sort on device buffers, allocating nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * keys[n], counts[num_cells + 1] and sorted[n] are device pointers on the
+ * current device; the work is enqueued on `stream` (a hipStream_t, NULL for
+ * the null stream) and the call returns without waiting.  Every intermediate
+ * lives in one static device array per GPU (about 41 MB), sized for at most
+ * LUS_NOALLOC_MAX_KEYS keys and LUS_NOALLOC_MAX_CELLS cells; larger sizes are
+ * rejected with LUS_EINVAL.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (other threads, other streams) race. */
+#define LUS_NOALLOC_MAX_KEYS (1 << 21)
+#define LUS_NOALLOC_MAX_CELLS (1 << 22)
+int lus_counting_sort_noalloc(const int* keys, int n, int num_cells, int* counts, int* sorted, void* stream);
+
 /* find_neighbors_uniform_grid (src/neighbors/Neighbors.cpp:366-450): neighbor
  * lists of the n_sand moving particles in the reference's order.  Indices in
  * idx are reference slots: sand k -> sand_start + k, solid k -> solid_index[k].

@@ -68,6 +68,20 @@ int set_error(int code, const char* fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+int lus::report_error(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err_msg, sizeof(g_err_msg), fmt, ap);
+    va_end(ap);
+    g_err_code = code;
+    fprintf(stderr, "lustrine_gpu: %s\n", g_err_msg);
+    return code;
+}
+
+namespace {
+
 #define LUS_HIP(call)                                                                      \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \

@@ -67,6 +67,9 @@ struct SandK {
     int sym_limit;            // slots below it receive symmetric pushes
 };
 
+// Records a failure for lus_gpu_last_error (lus_capi.cpp) and returns `code`.
+int report_error(int code, const char* fmt, ...);
+
 // ---- launchers (lus_kernels.hip); all enqueue on `s` and return hipError_t ----
 // prefix scan of m ints: out[k] = sum in[0..k); zero_in clears `in` after use.
 // tile_sums needs scan_tile_count(m) + 1 ints.

@@ -79,6 +79,7 @@ _SIGS = {
     "lus_gpu_device_count": (C.c_int, []),
     "lus_cell_ids": (C.c_int, [_P(LusGrid), C.c_int, _f, _i]),
     "lus_counting_sort": (C.c_int, [_i, _i, C.c_size_t, C.c_size_t, _i]),
+    "lus_counting_sort_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lus_neighbors_fluid": (C.c_int, [_P(LusGrid), C.c_float, C.c_int, _f, C.c_int, C.c_int, _f, _i,
                                       _l, _i, C.c_long, _l]),
     "lus_gpu_stream_copy": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, _f]),
@@ -191,6 +192,13 @@ def cell_ids(g: LusGrid, pos) -> np.ndarray:
     out = np.zeros(pos.shape[0], np.int32)
     _check(load().lus_cell_ids(C.byref(g), pos.shape[0], _ptr_f(pos), _ptr_i(out)), "lus_cell_ids")
     return out
+
+
+def counting_sort_noalloc(keys_dev, n, num_cells, counts_dev, sorted_dev, stream=None):
+    """lus_counting_sort_noalloc on device pointers (ints): enqueues the sort on
+    `stream` (None: the null stream) and returns without waiting."""
+    _check(load().lus_counting_sort_noalloc(keys_dev, n, num_cells, counts_dev, sorted_dev, stream),
+           "lus_counting_sort_noalloc")
 
 
 def counting_sort(keys, num_cells):
