@@ -305,6 +305,17 @@ int lus_sand_init(LusGpu* ctx, const LusSandParams* p, int n_sand, const float* 
  * the neighbour lists' total length sizes their buffer before they are
  * written (the lists have no fixed capacity, as the reference's vectors). */
 int lus_sand_step(LusGpu* ctx, float dt, int attract_flag, int blow_flag, const float* player);
+/* Which of the reference's sand Simulate_funs lus_sand_step computes (after
+ * lus_sand_init; default 0): 0 simulate_sand (src/Simulate.cpp:156-325), the
+ * default simulate_fun; 1 simulate_sand_credits (:327-510), installed by
+ * init_simulation_extra_parameters(..., with_credits) (src/Lustrine.cpp:312):
+ * attracted bit 2 marks a credit particle that falls at 1 until its first
+ * contact; 2 simulate_sand_v3 (:513-667), wrapper index 1
+ * (src/LustrineWrapper.cpp:666-677): dt clamped to [0.001, 0.016], no box
+ * clamp, half-space walls, one contact pass.  Sets the collision, friction,
+ * mu_s and mu_k constants to that function's locals.  A v3 prediction whose
+ * cell index leaves the grid makes lus_sand_step fail with LUS_EINVAL. */
+int lus_sand_set_variant(LusGpu* ctx, int variant);
 /* State in the reference's host order (sorted slots of the last step).
  * perm (n_sand): the last counting-sort permutation, new slot -> old slot. */
 int lus_sand_download(LusGpu* ctx, float* pos, float* pos_star, float* vel, int* attracted,
@@ -401,6 +412,46 @@ int lus_sand_sink_evict(LusGpu* ctx, const int* cells, int ncells, int* n_sand);
  * colliders).  Only those records cross to the host. */
 int lus_sand_player_box(LusGpu* ctx, const float* player, const float* half_extent, int max_out, int* slots,
                         float* pos, int* count);
+
+/* ------------------------------------------------------------------ */
+/* simulate_fluid (src/Simulate.cpp:27-115), the reference's fluid      */
+/* Simulate_fun, with its sequential pressure sweep                    */
+/* ------------------------------------------------------------------ */
+/* The Simulation fields simulate_fluid reads (src/Simulation.hpp:141-171). */
+typedef struct LusFluidParams {
+    float particleRadius, kernelRadius, kernelFactor;
+    float cubic_kernel_k, cubic_kernel_l;
+    float mass, rest_density, relaxation_epsilon;
+    float s_corr_dq, s_corr_k, s_corr_n;
+    float gravity[3];
+    float domain[3];   /* domainX, domainY, domainZ: resolve_collision uses (int) of each */
+    LusGrid grid;      /* gridX/Y/Z and cell_size of find_neighbors_uniform_grid */
+} LusFluidParams;
+
+/* The state simulate_fluid works on, in the reference's slot layout: arrays of
+ * `total` (Simulation::total_allocated) AoS float[3]; moving particles at
+ * [sand_start, sand_end), solids at [solid_start, solid_end) (their
+ * positions_star are what the lists test against; solids never move).
+ * velocities are read at the moving slots only.  lambdas: the reference's
+ * `lambdas` vector (n_lambdas entries; simulate_fluid reads lambdas[t] for the
+ * t-th entry of a list, src/Simulate.cpp:97, so lists longer than n_lambdas
+ * are reported by lus_fluid_step; it writes lambdas[i] for every moving slot,
+ * so n_lambdas must reach sand_end). */
+int lus_fluid_init(LusGpu* ctx, const LusFluidParams* p, int total, int sand_start, int sand_end,
+                   int solid_start, int solid_end, const float* positions, const float* positions_star,
+                   const float* velocities, int n_lambdas, const float* lambdas);
+/* One simulate_fluid call (src/Simulate.cpp:27-115): dt clamped to [0.001,
+ * 0.01], prediction, neighbour lists (Neighbors.cpp:366-450), lambdas, and the
+ * pressure sweep in the reference's sequential order (particle i sees the
+ * particles before it corrected), run level by level on the device.  Waits for
+ * the device.  A predicted position whose cell index lies outside the grid
+ * (the reference indexes outside its cell vector there) fails with LUS_EINVAL. */
+int lus_fluid_step(LusGpu* ctx, float dt);
+/* positions, positions_star (all `total` slots), velocities (moving slots),
+ * lambdas (n_lambdas); each nullable. */
+int lus_fluid_download(LusGpu* ctx, float* positions, float* positions_star, float* velocities, float* lambdas);
+/* Of the last step: dependency levels of the sweep, list entries. */
+int lus_fluid_stats(LusGpu* ctx, int* levels, long* list_entries);
 
 #ifdef __cplusplus
 }

@@ -8,15 +8,27 @@
  *   void simulate_sand_gpu(Lustrine::Simulation* s, float dt) {
  *       Lustrine::Bullet::simulate_bullet(&s->bullet_physics_simulation, dt,
  *                                         s->ptr_sand_start, s->ptr_sand_end);
- *       lus_adapter::simulate_sand(s, dt);
+ *       if (lus_adapter::simulate_sand(s, dt) != LUS_OK) {
+ *           lus_adapter::release(s);
+ *           Lustrine::simulate_sand(s, dt);                   // the reference, this step
+ *       }
  *   }
  *   sim.simulate_fun = simulate_sand_gpu;   // typedef at src/Simulation.hpp:22
  *
- * simulate_sand() replaces src/Simulate.cpp:156-325 (after its Bullet call):
- * it leaves positions, positions_star, velocities and attracted in the same
- * permuted slot order, with the same values, as the reference function.
- * simulate_pbf() is the position-based-fluids substep (no reference
- * counterpart with iterations and XSPH); particle order on the host is kept.
+ * simulate_sand() replaces src/Simulate.cpp:156-325 (after its Bullet call),
+ * simulate_sand_credits() :327-510 and simulate_sand_v3() :513-667: they leave
+ * positions, positions_star, velocities and attracted in the same permuted
+ * slot order, with the same values, as the reference functions.
+ * simulate_fluid() replaces :27-115 (same values up to s_coor's pow, see
+ * lus_fluid_step).  simulate_pbf() is the position-based-fluids substep (no
+ * reference counterpart with iterations and XSPH); particle order on the host
+ * is kept.
+ *
+ * Host arrays: by default every call leaves the results in the Simulation's
+ * arrays, as the reference does.  set_sync(sim, kSyncOnDemand) keeps them on
+ * the device instead; sync_positions() (what simulation_bind_positions_copy
+ * reads) or sync_host() (every array) bring them back, and must be called
+ * before host code reads or edits the arrays.
  *
  * query_cell_num_particles() and evict_sinks() serve the two host readers of
  * uniform_gird_cells (src/Lustrine.cpp:864-914 and :806-836) from the device
@@ -25,10 +37,10 @@
  * Device state persists per Simulation between calls.  It is uploaded again
  * when the sand range changes behind the adapter's back (sources move
  * ptr_sand_end, src/Lustrine.cpp:780; sinks run through evict_sinks() keep
- * it in step) or after mark_dirty().  Errors: the functions
- * return the LUS_E* code (0 = ok) and leave the host state untouched on
- * failure; the message is in lus_gpu_last_error().  Not thread-safe, like
- * the reference (one host thread per Simulation).
+ * it in step) or after mark_dirty().  Errors: the functions return the LUS_E*
+ * code (0 = ok) and leave the host arrays as the last synchronised state; the
+ * message is in lus_gpu_last_error().  Not thread-safe, like the reference
+ * (one host thread per Simulation).
  */
 #ifndef LUSTRINE_GPU_SIMULATE_HPP
 #define LUSTRINE_GPU_SIMULATE_HPP
@@ -41,12 +53,18 @@
 
 namespace lus_adapter {
 
+enum Sync { kSyncEveryStep = 0, kSyncOnDemand = 1 };
+
 struct State {
     LusGpu* ctx = nullptr;
-    int kind = 0;  // 1 sand, 2 pbf
+    int kind = 0;     // 1 sand, 2 pbf, 3 simulate_fluid
+    int variant = 0;  // sand: 0 simulate_sand, 1 simulate_sand_credits, 2 simulate_sand_v3
     int n_sand = -1;
     int ptr_solid_start = -1;
     bool dirty = true;
+    int sync = kSyncEveryStep;
+    bool pos_stale = false;   // the device holds newer positions than the host
+    bool rest_stale = false;  // ... newer velocities / attracted / lambdas
 };
 
 inline std::unordered_map<const void*, State>& states() {
@@ -81,15 +99,69 @@ inline int ensure_ctx(State& st, int device) {
     return lus_gpu_create(&st.ctx, device);
 }
 
-/* simulate_sand (src/Simulate.cpp:156-325) on the GPU. */
+/* Brings the host arrays up to the device state (all of them, or positions
+ * and positions_star only). */
 template <class Sim>
-int simulate_sand(Sim* s, float dt, int device = 0) {
+int sync_host(Sim* s, bool positions_only = false) {
+    auto it = states().find(s);
+    if (it == states().end()) return LUS_OK;
+    State& st = it->second;
+    if (!st.pos_stale && (positions_only || !st.rest_stale)) return LUS_OK;
+    int rc = LUS_OK;
+    const bool rest = !positions_only && st.rest_stale;
+    if (st.kind == 1) {
+        rc = lus_sand_download(st.ctx, fptr(s->positions), fptr(s->positions_star),
+                               rest ? fptr(s->velocities) : nullptr, rest ? s->attracted : nullptr, nullptr);
+    } else if (st.kind == 2) {
+        rc = lus_pbf_download(st.ctx, 1, fptr(s->positions), rest ? fptr(s->velocities) : nullptr, nullptr);
+        if (rc == LUS_OK)
+            std::memcpy(fptr(s->positions_star), fptr(s->positions), sizeof(float) * 3 * (size_t)st.n_sand);
+    } else if (st.kind == 3) {
+        rc = lus_fluid_download(st.ctx, fptr(s->positions), fptr(s->positions_star),
+                                rest ? fptr(s->velocities) : nullptr, rest ? s->lambdas.data() : nullptr);
+    }
+    if (rc) return rc;
+    st.pos_stale = false;
+    if (rest) st.rest_stale = false;
+    return LUS_OK;
+}
+
+/* What simulation_bind_positions_copy (src/LustrineWrapper.cpp:377-379) reads. */
+template <class Sim>
+int sync_positions(Sim* s) { return sync_host(s, true); }
+
+/* kSyncEveryStep (default): each call leaves its results in the host arrays.
+ * kSyncOnDemand: they stay on the device until sync_positions / sync_host. */
+template <class Sim>
+int set_sync(Sim* s, int mode) {
+    int rc = LUS_OK;
+    if (mode == kSyncEveryStep) rc = sync_host(s);
+    states()[s].sync = mode;
+    return rc;
+}
+
+inline void after_step(State& st) {
+    if (st.sync == kSyncEveryStep) return;
+    st.pos_stale = true;
+    st.rest_stale = true;
+}
+
+/* The sand Simulate_funs on the GPU: variant 0 simulate_sand
+ * (src/Simulate.cpp:156-325), 1 simulate_sand_credits (:327-510), 2
+ * simulate_sand_v3 (:513-667). */
+template <class Sim>
+int simulate_sand_variant(Sim* s, float dt, int variant, int device = 0) {
     State& st = states()[s];
     int rc = ensure_ctx(st, device);
     if (rc) return rc;
     const int n = s->ptr_sand_end - s->ptr_sand_start;
     if (s->ptr_sand_start != 0 || n <= 0) return LUS_EINVAL;
     if (st.kind != 1 || st.dirty || n != st.n_sand || s->ptr_solid_start != st.ptr_solid_start) {
+        // device results not yet on the host keep their slots below the old end
+        if (st.kind && (st.pos_stale || st.rest_stale) && n >= st.n_sand) {
+            rc = sync_host(s);
+            if (rc) return rc;
+        }
         LusSandParams p{};
         p.particleRadius = s->particleRadius;
         p.particleDiameter = s->particleDiameter;
@@ -105,7 +177,8 @@ int simulate_sand(Sim* s, float dt, int device = 0) {
         p.gravity[0] = s->gravity.x;
         p.gravity[1] = s->gravity.y;
         p.gravity[2] = s->gravity.z;
-        /* the locals of src/Simulate.cpp:159-163 and its 4 substeps (:226) */
+        /* the locals of src/Simulate.cpp:159-163 and its 4 substeps (:226);
+         * lus_sand_set_variant below sets those of the other functions */
         p.collision_coeff = 0.8f;
         p.friction_coeff = 0.7f;
         p.mu_s = 0.95f;
@@ -124,21 +197,103 @@ int simulate_sand(Sim* s, float dt, int device = 0) {
                            solid, ns ? solid_index.data() : nullptr);
         if (rc) return rc;
         st.kind = 1;
+        st.variant = 0;
         st.n_sand = n;
         st.ptr_solid_start = s->ptr_solid_start;
         st.dirty = false;
+        st.pos_stale = st.rest_stale = false;
+    }
+    if (st.variant != variant) {
+        rc = lus_sand_set_variant(st.ctx, variant);
+        if (rc) return rc;
+        st.variant = variant;
     }
     const float player[3] = {s->bullet_physics_simulation.player_position.x,
                              s->bullet_physics_simulation.player_position.y,
                              s->bullet_physics_simulation.player_position.z};
     rc = lus_sand_step(st.ctx, dt, s->attract_flag ? 1 : 0, s->blow_flag ? 1 : 0, player);
     if (rc) return rc;
-    rc = lus_sand_download(st.ctx, fptr(s->positions), fptr(s->positions_star), fptr(s->velocities),
-                           s->attracted, nullptr);
-    if (rc) return rc;
-    s->time_step = dt;
-    s->first_iteration = false;
+    if (st.sync == kSyncEveryStep) {
+        rc = lus_sand_download(st.ctx, fptr(s->positions), fptr(s->positions_star), fptr(s->velocities),
+                               s->attracted, nullptr);
+        if (rc) return rc;
+    }
+    after_step(st);
+    if (variant == 2) {  // src/Simulate.cpp:522-524
+        const float t = dt < 0.001f ? 0.001f : dt;
+        s->time_step = 0.016f < t ? 0.016f : t;
+    } else {
+        s->time_step = dt;
+        s->first_iteration = false;
+    }
     s->computed_static_particles = true;
+    return LUS_OK;
+}
+
+template <class Sim>
+int simulate_sand(Sim* s, float dt, int device = 0) { return simulate_sand_variant(s, dt, 0, device); }
+template <class Sim>
+int simulate_sand_credits(Sim* s, float dt, int device = 0) { return simulate_sand_variant(s, dt, 1, device); }
+template <class Sim>
+int simulate_sand_v3(Sim* s, float dt, int device = 0) { return simulate_sand_variant(s, dt, 2, device); }
+
+/* simulate_fluid (src/Simulate.cpp:27-115) on the GPU: every moving slot's
+ * positions, positions_star, velocities and the lambdas vector as the
+ * reference leaves them (slot order unchanged, as the reference keeps it). */
+template <class Sim>
+int simulate_fluid(Sim* s, float dt, int device = 0) {
+    State& st = states()[s];
+    int rc = ensure_ctx(st, device);
+    if (rc) return rc;
+    const int n = s->ptr_sand_end - s->ptr_sand_start;
+    if (n < 0) return LUS_EINVAL;
+    if (st.kind != 3 || st.dirty || n != st.n_sand || s->ptr_solid_start != st.ptr_solid_start) {
+        if (st.kind && (st.pos_stale || st.rest_stale) && n >= st.n_sand) {
+            rc = sync_host(s);
+            if (rc) return rc;
+        }
+        LusFluidParams p{};
+        p.particleRadius = s->particleRadius;
+        p.kernelRadius = s->kernelRadius;
+        p.kernelFactor = s->kernelFactor;
+        p.cubic_kernel_k = s->cubic_kernel_k;
+        p.cubic_kernel_l = s->cubic_kernel_l;
+        p.mass = s->mass;
+        p.rest_density = s->rest_density;
+        p.relaxation_epsilon = s->relaxation_epsilon;
+        p.s_corr_dq = s->s_corr_dq;
+        p.s_corr_k = s->s_corr_k;
+        p.s_corr_n = s->s_corr_n;
+        p.gravity[0] = s->gravity.x;
+        p.gravity[1] = s->gravity.y;
+        p.gravity[2] = s->gravity.z;
+        p.domain[0] = s->domainX;
+        p.domain[1] = s->domainY;
+        p.domain[2] = s->domainZ;
+        p.grid.cell_size = s->cell_size;
+        p.grid.gridX = s->gridX;
+        p.grid.gridY = s->gridY;
+        p.grid.gridZ = s->gridZ;
+        rc = lus_fluid_init(st.ctx, &p, (int)s->total_allocated, s->ptr_sand_start, s->ptr_sand_end,
+                            s->ptr_solid_start, s->ptr_solid_end, fptr(s->positions), fptr(s->positions_star),
+                            fptr(s->velocities), (int)s->lambdas.size(), s->lambdas.data());
+        if (rc) return rc;
+        st.kind = 3;
+        st.n_sand = n;
+        st.ptr_solid_start = s->ptr_solid_start;
+        st.dirty = false;
+        st.pos_stale = st.rest_stale = false;
+    }
+    rc = lus_fluid_step(st.ctx, dt);
+    if (rc) return rc;
+    if (st.sync == kSyncEveryStep) {
+        rc = lus_fluid_download(st.ctx, fptr(s->positions), fptr(s->positions_star), fptr(s->velocities),
+                                s->lambdas.data());
+        if (rc) return rc;
+    }
+    after_step(st);
+    const float t = dt < 0.001f ? 0.001f : dt;  // src/Simulate.cpp:31-32
+    s->time_step = 0.01f < t ? 0.01f : t;
     return LUS_OK;
 }
 
@@ -152,6 +307,10 @@ int simulate_pbf(Sim* s, float dt, int iterations = 4, int exact = 0, int device
     const int n = s->ptr_sand_end - s->ptr_sand_start;
     if (s->ptr_sand_start != 0 || n <= 0) return LUS_EINVAL;
     if (st.kind != 2 || st.dirty || n != st.n_sand) {
+        if (st.kind && (st.pos_stale || st.rest_stale) && n >= st.n_sand) {
+            rc = sync_host(s);
+            if (rc) return rc;
+        }
         LusPbfParams p{};
         p.h = s->kernelRadius;
         p.f = s->kernelFactor;
@@ -181,12 +340,16 @@ int simulate_pbf(Sim* s, float dt, int iterations = 4, int exact = 0, int device
         st.kind = 2;
         st.n_sand = n;
         st.dirty = false;
+        st.pos_stale = st.rest_stale = false;
     }
     rc = lus_pbf_step(st.ctx, dt, 1);
     if (rc) return rc;
-    rc = lus_pbf_download(st.ctx, 1, fptr(s->positions), fptr(s->velocities), nullptr);
-    if (rc) return rc;
-    std::memcpy(fptr(s->positions_star), fptr(s->positions), sizeof(float) * 3 * (size_t)n);
+    if (st.sync == kSyncEveryStep) {
+        rc = lus_pbf_download(st.ctx, 1, fptr(s->positions), fptr(s->velocities), nullptr);
+        if (rc) return rc;
+        std::memcpy(fptr(s->positions_star), fptr(s->positions), sizeof(float) * 3 * (size_t)n);
+    }
+    after_step(st);
     s->time_step = dt;
     return LUS_OK;
 }
@@ -226,7 +389,7 @@ int evict_sinks(Sim* s) {
     if (rc) return rc;
     const int removed = st.n_sand - n;
     if (removed == 0) return LUS_OK;
-    if (n > 0) {
+    if (n > 0 && st.sync == kSyncEveryStep) {
         rc = lus_sand_download(st.ctx, fptr(s->positions), fptr(s->positions_star), fptr(s->velocities),
                                s->attracted, nullptr);
         if (rc) return rc;

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblustrine_gpu.so")
-SOURCES = ["lus_kernels.hip", "lus_pbf.hip", "lus_mlp.hip", "lus_noalloc.hip", "lus_capi.cpp"]
+SOURCES = ["lus_kernels.hip", "lus_pbf.hip", "lus_mlp.hip", "lus_noalloc.hip", "lus_fluid.hip", "lus_capi.cpp"]
 ARCH = os.environ.get("LUS_OFFLOAD_ARCH", "gfx950")
 
 

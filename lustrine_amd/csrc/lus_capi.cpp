@@ -161,6 +161,7 @@ struct PbfState {
     lus::PbfK k{};
     int n = 0, C = 0;
     Arena mem;
+    float* stage = nullptr;  // 6 n floats: packed positions and velocities for downloads
     float4 *x = nullptr, *x2 = nullptr, *v = nullptr, *v2 = nullptr, *xa = nullptr, *xl = nullptr,
            *xp = nullptr;
     int *id = nullptr, *id2 = nullptr, *key = nullptr, *slot = nullptr, *tmp = nullptr,
@@ -239,6 +240,8 @@ struct SandState {
     Arena nbr_mem;
     float4* final_iter = nullptr;
     int* qbuf = nullptr;  // one int: cell-box count
+    int* err = nullptr;   // one int: a simulate_sand_v3 prediction left the grid
+    float* stage = nullptr;  // 6 n floats: packed positions and velocities for downloads
     bool stepped = false;
 };
 
@@ -251,8 +254,13 @@ struct LusGpu {
     PbfState pbf;
     NetState net;
     SandState sand;
+    lus::FluidState* fluid = nullptr;      // simulate_fluid state (lus_fluid.hip)
     std::vector<LusMlpTrainer*> trainers;  // live trainers, detached on destroy
 };
+
+int lus::ctx_device(const LusGpu* c) { return c->device; }
+hipStream_t lus::ctx_stream(const LusGpu* c) { return c->stream; }
+lus::FluidState*& lus::ctx_fluid(LusGpu* c) { return c->fluid; }
 
 // A trainer belongs to one context.  Destroying the context first frees the
 // trainer's device memory and detaches it (ctx = nullptr): later calls on it
@@ -596,6 +604,8 @@ void lus_gpu_destroy(LusGpu* c) {
     c->net.mem.release();
     c->sand.mem.release();
     c->sand.nbr_mem.release();
+    lus::fluid_free(c->fluid);
+    c->fluid = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -910,6 +920,7 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
     LUS_HIP(hipStreamSynchronize(c->stream));
     pbf_drop_graph(P);
     P.mem.release();
+    P.stage = nullptr;
     P.ready = false;
     P.stepped = false;
     P.parity = 0;
@@ -1116,27 +1127,25 @@ int lus_pbf_download(LusGpu* c, int order, float* pos, float* vel, int* ids) {
     PbfState& P = c->pbf;
     LUS_HIP(hipSetDevice(c->device));
     LUS_HIP(hipStreamSynchronize(c->stream));
+    if (order != 0 && order != 1) return set_error(LUS_EINVAL, "lus_pbf_download: order must be 0 or 1");
     const int n = P.n;
-    std::vector<float4> hx((size_t)n), hv((size_t)n);
-    std::vector<int> hid((size_t)n);
-    LUS_HIP(hipMemcpy(hx.data(), P.x, sizeof(float4) * n, hipMemcpyDeviceToHost));
-    LUS_HIP(hipMemcpy(hv.data(), P.v, sizeof(float4) * n, hipMemcpyDeviceToHost));
-    LUS_HIP(hipMemcpy(hid.data(), P.id, sizeof(int) * n, hipMemcpyDeviceToHost));
-    if (order == 1) {
-        std::vector<float4> ox((size_t)n), ov((size_t)n);
-        for (int r = 0; r < n; ++r) {
-            ox[hid[r]] = hx[r];
-            ov[hid[r]] = hv[r];
-        }
-        hx.swap(ox);
-        hv.swap(ov);
-        for (int r = 0; r < n; ++r) hid[r] = r;
-    } else if (order != 0) {
-        return set_error(LUS_EINVAL, "lus_pbf_download: order must be 0 or 1");
+    hipStream_t s = c->stream;
+    // vec3 rows packed (and for order 1 put back in input order by id) on
+    // the device: each array is one copy into the caller's memory
+    if ((pos || vel) && !P.stage) LUS_HIP(P.mem.alloc(&P.stage, 6 * (size_t)n));
+    const int* by = order == 1 ? P.id : nullptr;
+    if (pos) {
+        LUS_HIP(lus::launch_pack3(P.x, by, n, P.stage, s));
+        LUS_HIP(hipMemcpyAsync(pos, P.stage, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
     }
-    if (pos) from_f4(hx, pos);
-    if (vel) from_f4(hv, vel);
-    if (ids) memcpy(ids, hid.data(), sizeof(int) * n);
+    if (vel) {
+        LUS_HIP(lus::launch_pack3(P.v, by, n, P.stage + 3 * (size_t)n, s));
+        LUS_HIP(hipMemcpyAsync(vel, P.stage + 3 * (size_t)n, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    }
+    if (ids && order == 0) LUS_HIP(hipMemcpyAsync(ids, P.id, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    LUS_HIP(hipStreamSynchronize(s));
+    if (ids && order == 1)
+        for (int r = 0; r < n; ++r) ids[r] = r;
     return LUS_OK;
 }
 
@@ -1726,6 +1735,7 @@ int lus_sand_init(LusGpu* c, const LusSandParams* p, int n_sand, const float* po
     LUS_HIP(hipStreamSynchronize(c->stream));
     S.mem.release();
     S.nbr_mem.release();
+    S.stage = nullptr;
     S.ready = false;
     S.stepped = false;
     S.p = *p;
@@ -1760,6 +1770,9 @@ int lus_sand_init(LusGpu* c, const LusSandParams* p, int n_sand, const float* po
     k.gZ = g.gridZ;
     k.gXZ = g.gridX * g.gridZ;
     k.sym_limit = p->ptr_solid_start;
+    k.variant = 0;
+    k.bcoef = 0.9f;
+    for (int a = 0; a < 3; ++a) k.dom[a] = p->domain[a];
     Arena& m = S.mem;
     const int n = n_sand;
     LUS_HIP(m.alloc(&S.x, n));
@@ -1784,6 +1797,7 @@ int lus_sand_init(LusGpu* c, const LusSandParams* p, int n_sand, const float* po
     LUS_HIP(m.alloc(&S.nbr_cnt1, n));
     LUS_HIP(m.alloc(&S.ntiles, lus::scan_tile_count(n + 1) + 1));
     LUS_HIP(m.alloc(&S.qbuf, 1));
+    LUS_HIP(m.alloc(&S.err, 1));
     std::vector<float4> hx = to_f4(pos, n), hv = to_f4(vel, n);
     std::vector<int> ha((size_t)n, 0);
     if (attracted) memcpy(ha.data(), attracted, sizeof(int) * n);
@@ -1835,7 +1849,13 @@ int lus_sand_step(LusGpu* c, float dt, int attract_flag, int blow_flag, const fl
     auto mark = [&](int q) -> hipError_t { return c->profiling ? hipEventRecord(pe.e[q], s) : hipSuccess; };
     if (c->profiling) LUS_HIP(pe.create(5));
     LUS_HIP(mark(0));
-    LUS_HIP(lus::launch_sand_predict_key(S.x, S.v, S.att, n, k, dt, S.xs, S.key, S.slot, S.counts, s));
+    if (k.variant == 2) {  // simulate_sand_v3: dt = clamp(dt, 0.001, 0.016) (src/Simulate.cpp:522)
+        const float t = dt < 0.001f ? 0.001f : dt;
+        dt = 0.016f < t ? 0.016f : t;
+        LUS_HIP(hipMemsetAsync(S.err, 0, sizeof(int), s));
+    }
+    LUS_HIP(lus::launch_sand_predict_key(S.x, S.v, S.att, n, k, dt, S.err, S.xs, S.key, S.slot, S.counts,
+                                         s));
     LUS_HIP(mark(1));
     LUS_HIP(lus::launch_exclusive_scan(S.counts, S.cs, S.C + 1, S.tiles, 1, s));
     LUS_HIP(lus::launch_scatter(S.key, S.slot, S.cs, n, S.tmp, s));
@@ -1845,9 +1865,14 @@ int lus_sand_step(LusGpu* c, float dt, int attract_flag, int blow_flag, const fl
     LUS_HIP(lus::launch_sand_nbr(S.A, S.key2, S.cs, n, S.solid, S.ocs, k, nullptr, S.nbr_cnt,
                                  S.nbr_cnt1, nullptr, 0, s));
     LUS_HIP(lus::launch_exclusive_scan(S.nbr_cnt, S.nbr_off, n + 1, S.ntiles, 0, s));
-    int total = 0;
+    int total = 0, bad_key = 0;
     LUS_HIP(hipMemcpyAsync(&total, S.nbr_off + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (k.variant == 2) LUS_HIP(hipMemcpyAsync(&bad_key, S.err, sizeof(int), hipMemcpyDeviceToHost, s));
     LUS_HIP(hipStreamSynchronize(s));
+    if (bad_key)
+        return set_error(LUS_EINVAL,
+                         "lus_sand_step: a predicted position's cell lies outside the grid (simulate_sand_v3 "
+                         "does not clamp; the reference indexes outside its cell vector there)");
     if (total < 0) return set_error(LUS_EHIP, "lus_sand_step: neighbor count overflow");
     if ((long)total > S.nbr_cap || !S.nbr_idx) {
         S.nbr_mem.release();
@@ -1860,8 +1885,10 @@ int lus_sand_step(LusGpu* c, float dt, int attract_flag, int blow_flag, const fl
     LUS_HIP(mark(2));
     float4* A = S.A;
     float4* B = S.B;
-    for (int it = 0; it < S.p.substeps; ++it) {
-        LUS_HIP(lus::launch_sand_substep(A, S.x2, S.solid, S.nbr_off, S.nbr_idx, n, k, B, s));
+    // simulate_sand_v3's substeps all read the same positions_tmp: one pass
+    const int passes = k.variant == 2 ? (S.p.substeps > 0 ? 1 : 0) : S.p.substeps;
+    for (int it = 0; it < passes; ++it) {
+        LUS_HIP(lus::launch_sand_substep(A, S.x2, S.solid, S.nbr_off, S.nbr_idx, n, k, B, S.att2, s));
         std::swap(A, B);
     }
     LUS_HIP(mark(3));
@@ -1883,36 +1910,55 @@ int lus_sand_step(LusGpu* c, float dt, int attract_flag, int blow_flag, const fl
     std::swap(S.x, S.x2);
     std::swap(S.v, S.v2);
     std::swap(S.att, S.att2);
-    S.prev_attract = k.attract;
+    if (k.variant != 2) S.prev_attract = k.attract;  // v3 keeps no previous flag
     S.stepped = true;
+    return LUS_OK;
+}
+
+int lus_sand_set_variant(LusGpu* c, int variant) {
+    if (!c || !c->sand.ready) return set_error(LUS_ESTATE, "lus_sand_set_variant: no sand scene");
+    if (variant < 0 || variant > 2) return set_error(LUS_EINVAL, "lus_sand_set_variant: variant %d", variant);
+    SandState& S = c->sand;
+    lus::SandK& k = S.k;
+    // the locals of each function (src/Simulate.cpp:159-163, 330-334, 516-520)
+    static const float coeff[3][4] = {{0.8f, 0.7f, 0.95f, 0.9f}, {0.8f, 0.7f, 0.8f, 0.7f}, {0.8f, 0.5f, 0.8f, 0.8f}};
+    k.variant = variant;
+    k.collision = coeff[variant][0];
+    k.friction = coeff[variant][1];
+    k.mu_s = coeff[variant][2];
+    k.mu_k = coeff[variant][3];
+    k.bcoef = 0.9f;
+    S.p.collision_coeff = k.collision;
+    S.p.friction_coeff = k.friction;
+    S.p.mu_s = k.mu_s;
+    S.p.mu_k = k.mu_k;
     return LUS_OK;
 }
 
 int lus_sand_download(LusGpu* c, float* pos, float* pos_star, float* vel, int* attracted, int* perm) {
     if (!c || !c->sand.ready) return set_error(LUS_ESTATE, "lus_sand_download: no sand scene");
     SandState& S = c->sand;
+    if (perm && !S.stepped) return set_error(LUS_ESTATE, "lus_sand_download: no step yet");
     LUS_HIP(hipSetDevice(c->device));
-    LUS_HIP(hipStreamSynchronize(c->stream));
+    hipStream_t s = c->stream;
     const int n = S.n;
-    std::vector<float4> h((size_t)n);
-    if (pos) {
-        LUS_HIP(hipMemcpy(h.data(), S.x, sizeof(float4) * n, hipMemcpyDeviceToHost));
-        from_f4(h, pos);
-    }
-    if (pos_star) {
-        LUS_HIP(hipMemcpy(h.data(), S.stepped ? S.final_iter : S.x, sizeof(float4) * n,
-                          hipMemcpyDeviceToHost));
-        from_f4(h, pos_star);
+    // vec3 rows are packed on the device and each array is one copy into the
+    // caller's memory; the stream is waited for once
+    if ((pos || pos_star || vel) && !S.stage) LUS_HIP(S.mem.alloc(&S.stage, 6 * (size_t)n));
+    if (pos || pos_star) {
+        LUS_HIP(lus::launch_pack3(S.x, nullptr, n, S.stage, s));
+        LUS_HIP(hipMemcpyAsync(pos ? pos : pos_star, S.stage, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
     }
     if (vel) {
-        LUS_HIP(hipMemcpy(h.data(), S.v, sizeof(float4) * n, hipMemcpyDeviceToHost));
-        from_f4(h, vel);
+        LUS_HIP(lus::launch_pack3(S.v, nullptr, n, S.stage + 3 * (size_t)n, s));
+        LUS_HIP(hipMemcpyAsync(vel, S.stage + 3 * (size_t)n, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
     }
-    if (attracted) LUS_HIP(hipMemcpy(attracted, S.att, sizeof(int) * n, hipMemcpyDeviceToHost));
-    if (perm) {
-        if (!S.stepped) return set_error(LUS_ESTATE, "lus_sand_download: no step yet");
-        LUS_HIP(hipMemcpy(perm, S.perm, sizeof(int) * n, hipMemcpyDeviceToHost));
-    }
+    if (attracted) LUS_HIP(hipMemcpyAsync(attracted, S.att, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    if (perm) LUS_HIP(hipMemcpyAsync(perm, S.perm, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    LUS_HIP(hipStreamSynchronize(s));
+    // positions_star equals positions after a step (src/Simulate.cpp:316-319)
+    // and before the first one (both are the uploaded positions)
+    if (pos && pos_star) memcpy(pos_star, pos, sizeof(float) * 3 * (size_t)n);
     return LUS_OK;
 }
 

@@ -65,10 +65,23 @@ struct SandK {
     int gX, gY, gZ, gXZ;
     int attract, blow, prev_attract;
     int sym_limit;            // slots below it receive symmetric pushes
+    // 0 simulate_sand, 1 simulate_sand_credits, 2 simulate_sand_v3
+    // (src/Simulate.cpp:156-325, 327-510, 513-667)
+    int variant;
+    float bcoef;              // boundary_collision_coeff of simulate_sand_v3
+    float dom[3];             // domainX/Y/Z: simulate_sand_v3's walls
 };
 
 // Records a failure for lus_gpu_last_error (lus_capi.cpp) and returns `code`.
 int report_error(int code, const char* fmt, ...);
+
+// The simulate_fluid state of a context (lus_fluid.hip) and the context's
+// device, stream and fluid slot for the translation units outside lus_capi.cpp.
+struct FluidState;
+void fluid_free(FluidState* f);
+int ctx_device(const LusGpu* c);
+hipStream_t ctx_stream(const LusGpu* c);
+FluidState*& ctx_fluid(LusGpu* c);
 
 // ---- launchers (lus_kernels.hip); all enqueue on `s` and return hipError_t ----
 // prefix scan of m ints: out[k] = sum in[0..k); zero_in clears `in` after use.
@@ -85,6 +98,8 @@ hipError_t launch_scatter(const int* key, const int* slot, const int* cell_start
 hipError_t launch_stable_rank(const int* tmp, const int* key, const int* cell_start, int n,
                               int* perm, int* key_sorted, hipStream_t s);
 hipError_t launch_gather4(const float4* src, const int* perm, int n, float4* dst, hipStream_t s);
+// dst[3 q .. 3 q + 2] = src[r].xyz, q = ids ? ids[r] : r (ids a permutation of 0..n-1)
+hipError_t launch_pack3(const float4* src, const int* ids, int n, float* dst, hipStream_t s);
 hipError_t launch_gather1(const int* src, const int* perm, int n, int* dst, hipStream_t s);
 hipError_t launch_stream_copy(const float4* src, float4* dst, size_t n4, hipStream_t s);
 
@@ -234,7 +249,7 @@ hipError_t launch_adam(float* p, const float* g, float* m, float* v, int np, flo
                        float b2, float eps, float c1, float c2, hipStream_t s);
 
 // sand (simulate_sand)
-hipError_t launch_sand_predict_key(float4* x, float4* v, int* att, int n, SandK k, float dt,
+hipError_t launch_sand_predict_key(float4* x, float4* v, int* att, int n, SandK k, float dt, int* err,
                                    float4* xs, int* key, int* slot, int* counts, hipStream_t s);
 hipError_t launch_sand_permute(const int* tmp, const int* key, const int* cell_start,
                                const float4* x, const float4* xs, const float4* v, const int* att,
@@ -244,7 +259,7 @@ hipError_t launch_sand_nbr(const float4* xs, const int* key2, const int* sand_cs
                            const float4* solid, const int* solid_cs, SandK k, const int* off,
                            int* cnt, int* cnt1, int* idx, int fill, hipStream_t s);
 hipError_t launch_sand_substep(const float4* A, const float4* xold, const float4* solid,
-                               const int* off, const int* idx, int n, SandK k, float4* B,
+                               const int* off, const int* idx, int n, SandK k, float4* B, int* att,
                                hipStream_t s);
 hipError_t launch_sand_vupdate(const float4* A, float4* xold_to_new, float4* vout, int n, float dt,
                                hipStream_t s);

@@ -267,6 +267,24 @@ __global__ __launch_bounds__(kBlock) void k_gather4(const float4* __restrict__ s
     dst[i] = src[perm[i]];
 }
 
+// AoS float[3] rows from float4: row q = ids ? ids[r] : r receives src[r]
+// (the host's vec3 layout, written on the device so a download is one copy)
+__global__ __launch_bounds__(kBlock) void k_pack3(const float4* __restrict__ src, const int* __restrict__ ids,
+                                                   int n, float* __restrict__ dst) {
+    const int r = xcd_index();
+    if (r >= n) return;
+    const float4 a = src[r];
+    const int q = ids ? ids[r] : r;
+    dst[3 * (size_t)q] = a.x;
+    dst[3 * (size_t)q + 1] = a.y;
+    dst[3 * (size_t)q + 2] = a.z;
+}
+
+hipError_t launch_pack3(const float4* src, const int* ids, int n, float* dst, hipStream_t s) {
+    launch_kernel(k_pack3, dim3(grid_for(n)), dim3(kBlock), 0, s, src, ids, n, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather4(const float4* src, const int* perm, int n, float4* dst, hipStream_t s) {
     launch_kernel(k_gather4, dim3(grid_for(n)), dim3(kBlock), 0, s, src, perm, n, dst);
     return hipGetLastError();
@@ -385,21 +403,90 @@ hipError_t launch_fluid_nbr(const float4* sand_sorted, const int* sand_key, cons
 // Predicted position of sand slot i with gravity and the attract / blow
 // forces (src/Simulate.cpp:185-212); writes v, attracted, x*, key; returns
 // the cell.
-__device__ __forceinline__ int sand_predict(int i, float4* __restrict__ x, float4* __restrict__ v,
-                                            int* __restrict__ att, const SandK& k, float dt,
-                                            float4* __restrict__ xs, int* __restrict__ key) {
+// simulate_sand_v3's integration (src/Simulate.cpp:541-556): gravity, the
+// unbounded attract / blow pushes, no box clamp; the cell index is reported
+// (*err) when it falls outside the grid, where the reference indexes outside
+// its cell vector, and used as the reference uses it otherwise
+__device__ __forceinline__ int sand_predict_v3(int i, float4* __restrict__ x, float4* __restrict__ v,
+                                               const SandK& k, float dt, float4* __restrict__ xs,
+                                               int* __restrict__ key, int* __restrict__ err) {
     const float4 p = x[i];
     float4 vi = v[i];
     const float w = k.w;
     vi.x = vi.x + k.g[0] * dt;
     vi.y = vi.y + k.g[1] * dt;
     vi.z = vi.z + k.g[2] * dt;
+    if (k.attract || k.blow) {
+        const float d[3] = {k.player[0] - p.x, k.player[1] - p.y, k.player[2] - p.z};
+        const float inv = 1.0f / sqrtf(dot3(d[0], d[1], d[2], d[0], d[1], d[2]));
+        float add[3];
+        if (k.attract) {
+            for (int c = 0; c < 3; ++c) {
+                float t = d[c] * inv;
+                t = t * k.r;
+                t = t * 1000.0f;
+                t = t * dt;
+                t = t * w;
+                add[c] = t;
+            }
+            vi.x = vi.x + add[0];
+            vi.y = vi.y + add[1];
+            vi.z = vi.z + add[2];
+        }
+        if (k.blow) {
+            for (int c = 0; c < 3; ++c) {
+                float t = -(d[c] * inv);
+                t = t * k.r;
+                t = t * 200.0f;
+                t = t * w;
+                add[c] = t;
+            }
+            vi.x = vi.x + add[0];
+            vi.y = vi.y + add[1];
+            vi.z = vi.z + add[2];
+        }
+    }
+    float4 q;
+    q.x = p.x + vi.x * dt;
+    q.y = p.y + vi.y * dt;
+    q.z = p.z + vi.z * dt;
+    q.w = 0.0f;
+    v[i] = vi;
+    xs[i] = q;
+    const float px = q.x / k.cs, py = q.y / k.cs, pz = q.z / k.cs;
+    bool bad = !(fabsf(px) < 1.0e9f && fabsf(py) < 1.0e9f && fabsf(pz) < 1.0e9f);
+    int c = 0;
+    if (!bad) {
+        const long long cl = (long long)(int)py * k.gXZ + (long long)(int)px * k.gZ + (long long)(int)pz;
+        bad = cl < 0 || cl >= (long long)k.gXZ * k.gY;
+        c = (int)cl;
+    }
+    if (bad) {
+        atomicOr(err, 1);
+        c = 0;
+    }
+    key[i] = c;
+    return c;
+}
+
+__device__ __forceinline__ int sand_predict(int i, float4* __restrict__ x, float4* __restrict__ v,
+                                            int* __restrict__ att, const SandK& k, float dt,
+                                            float4* __restrict__ xs, int* __restrict__ key) {
+    const float4 p = x[i];
+    float4 vi = v[i];
+    const float w = k.w;
+    const bool credits = k.variant == 1;
     int a = att[i];
-    if (k.prev_attract && !k.attract) a = 0;
+    // simulate_sand_credits: a particle flagged 2 falls at 1 (:361-364)
+    if (credits && (a & 2)) vi = make_float4(0.0f, -1.0f, 0.0f, vi.w);
+    vi.x = vi.x + k.g[0] * dt;
+    vi.y = vi.y + k.g[1] * dt;
+    vi.z = vi.z + k.g[2] * dt;
+    if (k.prev_attract && !k.attract) a = credits ? (a & ~1) : 0;
     if (k.attract) {
         const float d0x = k.player[0] - p.x, d0y = k.player[1] - p.y, d0z = k.player[2] - p.z;
-        if (sqrtf(dot3(d0x, d0y, d0z, d0x, d0y, d0z)) < k.attract_radius) a = 1;
-        if (a) {
+        if (sqrtf(dot3(d0x, d0y, d0z, d0x, d0y, d0z)) < k.attract_radius) a = credits ? ((a | 1) & ~2) : 1;
+        if (credits ? (a & 1) : a) {
             const float ox = k.player[0] + 0.0f, oy = k.player[1] + 1.5f, oz = k.player[2] + 0.0f;
             const float d[3] = {ox - p.x, oy - p.y, oz - p.z};
             const float inv = 1.0f / sqrtf(dot3(d[0], d[1], d[2], d[0], d[1], d[2]));
@@ -436,6 +523,7 @@ __device__ __forceinline__ int sand_predict(int i, float4* __restrict__ x, float
             vi.x = vi.x + add[0];
             vi.y = vi.y + add[1];
             vi.z = vi.z + add[2];
+            if (credits) a &= ~2;
         }
     }
     float4 q;
@@ -454,22 +542,25 @@ __device__ __forceinline__ int sand_predict(int i, float4* __restrict__ x, float
 __global__ __launch_bounds__(kBlock) void k_sand_predict_key(float4* __restrict__ x,
                                                               float4* __restrict__ v,
                                                               int* __restrict__ att, int n, SandK k,
-                                                              float dt, float4* __restrict__ xs,
+                                                              float dt, int* __restrict__ err,
+                                                              float4* __restrict__ xs,
                                                               int* __restrict__ key,
                                                               int* __restrict__ slot,
                                                               int* __restrict__ counts) {
     const int i = xcd_index();
     const bool active = i < n;
-    const int c = active ? sand_predict(i, x, v, att, k, dt, xs, key) : 0;
+    int c = 0;
+    if (active)
+        c = k.variant == 2 ? sand_predict_v3(i, x, v, k, dt, xs, key, err) : sand_predict(i, x, v, att, k, dt, xs, key);
     // one atomic per run of equal keys in the wave (every lane calls it)
     const int sl = wave_atomic_slot(counts, c, active);
     if (active) slot[i] = sl;
 }
 
-hipError_t launch_sand_predict_key(float4* x, float4* v, int* att, int n, SandK k, float dt,
+hipError_t launch_sand_predict_key(float4* x, float4* v, int* att, int n, SandK k, float dt, int* err,
                                    float4* xs, int* key, int* slot, int* counts, hipStream_t s) {
     launch_kernel(k_sand_predict_key, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, att, n, k,
-                       dt, xs, key, slot, counts);
+                       dt, err, xs, key, slot, counts);
     return hipGetLastError();
 }
 
@@ -618,19 +709,20 @@ hipError_t launch_sand_nbr(const float4* xs, const int* key2, const int* sand_cs
 // The contact + friction projection of particle (pi, xoi) against neighbour jj
 // at pj (its old position xold[jj] is read for a sand contact only) into dp
 // (src/Simulate.cpp:240-300).
-__device__ __forceinline__ void sand_pair(const float4 pi, const float4 xoi, const float4 pj, const float4* __restrict__ xold, int jj,
+// Returns whether the pair is in contact (within the diameter).
+__device__ __forceinline__ bool sand_pair(const float4 pi, const float4 xoi, const float4 pj, const float4* __restrict__ xold, int jj,
                                           bool sand, const SandK& k, float& dpx, float& dpy, float& dpz) {
     float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
-    // |ij| == 0 -> ij = (0, 1e-5, 0) (src/Simulate.cpp:236-239); the length
-    // is computed again only then
+    // |ij| == 0 -> ij = (0, 1e-5, 0) (src/Simulate.cpp:236-239; not in
+    // simulate_sand_v3); the length is computed again only then
     float len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
-    if (len == 0.0f) {
+    if (len == 0.0f && k.variant != 2) {
         ijx = 0.0f;
         ijy = 0.00001f;
         ijz = 0.0f;
         len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
     }
-    if (len > k.d) return;
+    if (len > k.d) return false;
     float sc;
     if (sand) {
         sc = k.collision * k.mass_n;
@@ -685,6 +777,36 @@ __device__ __forceinline__ void sand_pair(const float4 pi, const float4 xoi, con
         dpy = dpy - (k.friction * xty) * ratio;
         dpz = dpz - (k.friction * xtz) * ratio;
     }
+    return true;
+}
+
+// simulate_sand_v3's six half-space projections at positions_tmp
+// (src/Simulate.cpp:601-616 with solve_boundary_collision_constraint,
+// :119-131), summed in its wall order
+__device__ __forceinline__ void sand_walls_v3(const float4 q, const SandK& k, float& ox, float& oy, float& oz) {
+    const float nrm[6][3] = {{1.0f, 0.0f, 0.0f}, {-1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f},
+                             {0.0f, -1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}, {0.0f, 0.0f, -1.0f}};
+    const float org[6][3] = {{0.0f, 0.0f, 0.0f}, {k.dom[0], 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f},
+                             {0.0f, k.dom[1], 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, k.dom[2]}};
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+#pragma unroll
+    for (int w = 0; w < 6; ++w) {
+        const float rx = q.x - org[w][0], ry = q.y - org[w][1], rz = q.z - org[w][2];
+        const float C = dot3(nrm[w][0], nrm[w][1], nrm[w][2], rx, ry, rz) - k.r;
+        float bx = 0.0f, by = 0.0f, bz = 0.0f;
+        if (!(C >= 0.0f)) {
+            const float sv = C / dot3(nrm[w][0], nrm[w][1], nrm[w][2], nrm[w][0], nrm[w][1], nrm[w][2]);
+            bx = (-sv) * nrm[w][0];
+            by = (-sv) * nrm[w][1];
+            bz = (-sv) * nrm[w][2];
+        }
+        dx = dx + k.bcoef * bx;
+        dy = dy + k.bcoef * by;
+        dz = dz + k.bcoef * bz;
+    }
+    ox = dx;
+    oy = dy;
+    oz = dz;
 }
 
 // One Jacobi substep of contact + friction (src/Simulate.cpp:228-308).  The
@@ -692,17 +814,25 @@ __device__ __forceinline__ void sand_pair(const float4 pi, const float4 xoi, con
 // requested before the four are projected in list order (a sand contact's old
 // position is loaded inside the projection, ~a quarter of the entries).
 // Places past the list end read slot i, in bounds and unused.
+//
+// simulate_sand_credits: a particle flagged 2 loses the flag at its first
+// contact (:463-469; the three rand() draws there are not used).
+// simulate_sand_v3: its four substeps all start from the same positions_tmp
+// and give the same bits, so one pass computes them; then the half-space
+// projections of positions_tmp are added and nothing is clamped (:566-616).
 __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restrict__ A,
                                                           const float4* __restrict__ xold,
                                                           const float4* __restrict__ solid,
                                                           const int* __restrict__ off,
                                                           const int* __restrict__ idx, int n,
-                                                          SandK k, float4* __restrict__ B) {
+                                                          SandK k, float4* __restrict__ B,
+                                                          int* __restrict__ att) {
     const int i = xcd_index();
     if (i >= n) return;
     const float4 pi = A[i];
     const float4 xoi = xold[i];
     float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
+    bool contact = false;
     const int t1 = off[i + 1];
     for (int t = off[i]; t < t1; t += 4) {
         int j[4];
@@ -716,21 +846,34 @@ __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restric
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (t + u < t1 && j[u] != i) sand_pair(pi, xoi, pj[u], xold, j[u], j[u] < n, k, dpx, dpy, dpz);
+            if (t + u < t1 && j[u] != i)
+                contact |= sand_pair(pi, xoi, pj[u], xold, j[u], j[u] < n, k, dpx, dpy, dpz);
     }
     float4 r;
-    r.x = rclamp(pi.x + dpx, k.r, k.hi[0]);
-    r.y = rclamp(pi.y + dpy, k.r, k.hi[1]);
-    r.z = rclamp(pi.z + dpz, k.r, k.hi[2]);
+    if (k.variant == 2) {
+        float wx, wy, wz;
+        sand_walls_v3(pi, k, wx, wy, wz);
+        r.x = (pi.x + dpx) + wx;
+        r.y = (pi.y + dpy) + wy;
+        r.z = (pi.z + dpz) + wz;
+    } else {
+        r.x = rclamp(pi.x + dpx, k.r, k.hi[0]);
+        r.y = rclamp(pi.y + dpy, k.r, k.hi[1]);
+        r.z = rclamp(pi.z + dpz, k.r, k.hi[2]);
+    }
     r.w = 0.0f;
     B[i] = r;
+    if (k.variant == 1 && contact) {
+        const int a = att[i];
+        if (a & 2) att[i] = a & ~2;
+    }
 }
 
 hipError_t launch_sand_substep(const float4* A, const float4* xold, const float4* solid,
-                               const int* off, const int* idx, int n, SandK k, float4* B,
+                               const int* off, const int* idx, int n, SandK k, float4* B, int* att,
                                hipStream_t s) {
     launch_kernel(k_sand_substep, dim3(grid_for(n)), dim3(kBlock), 0, s, A, xold, solid, off,
-                       idx, n, k, B);
+                       idx, n, k, B, att);
     return hipGetLastError();
 }
 

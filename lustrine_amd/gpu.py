@@ -59,6 +59,16 @@ class LusSandParams(C.Structure):
     ]
 
 
+class LusFluidParams(C.Structure):
+    _fields_ = [
+        ("particleRadius", C.c_float), ("kernelRadius", C.c_float), ("kernelFactor", C.c_float),
+        ("cubic_kernel_k", C.c_float), ("cubic_kernel_l", C.c_float),
+        ("mass", C.c_float), ("rest_density", C.c_float), ("relaxation_epsilon", C.c_float),
+        ("s_corr_dq", C.c_float), ("s_corr_k", C.c_float), ("s_corr_n", C.c_float),
+        ("gravity", C.c_float * 3), ("domain", C.c_float * 3), ("grid", LusGrid),
+    ]
+
+
 _P = C.POINTER
 _f = _P(C.c_float)
 _i = _P(C.c_int)
@@ -114,6 +124,7 @@ _SIGS = {
     "lus_sand_init": (C.c_int, [C.c_void_p, _P(LusSandParams), C.c_int, _f, _f, _i, C.c_int, _f, _i]),
     "lus_sand_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, _f]),
     "lus_sand_download": (C.c_int, [C.c_void_p, _f, _f, _f, _i, _i]),
+    "lus_sand_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_sand_neighbors": (C.c_int, [C.c_void_p, _l, _i, C.c_long, _l]),
     "lus_sand_query_cells": (C.c_int, [C.c_void_p, _f, _f, C.c_int, _i]),
     "lus_sand_sink_evict": (C.c_int, [C.c_void_p, _i, C.c_int, _i]),
@@ -123,6 +134,11 @@ _SIGS = {
                                       C.c_int, _f, _f, _f, _i]),
     "lus_grid_from_voxels": (C.c_int, [C.c_void_p, _P(C.c_ubyte), C.c_int, C.c_int, C.c_int, _P(C.c_ubyte), _i, _f,
                                        _i]),
+    "lus_fluid_init": (C.c_int, [C.c_void_p, _P(LusFluidParams), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 _f, _f, _f, C.c_int, _f]),
+    "lus_fluid_step": (C.c_int, [C.c_void_p, C.c_float]),
+    "lus_fluid_download": (C.c_int, [C.c_void_p, _f, _f, _f, _f]),
+    "lus_fluid_stats": (C.c_int, [C.c_void_p, _i, _l]),
     "lus_get_num_observation": (C.c_long, []),
     "lus_get_cycles": (C.c_longlong, [C.c_int]),
     "lus_get_duration": (C.c_double, [C.c_int]),
@@ -192,6 +208,22 @@ def cell_ids(g: LusGrid, pos) -> np.ndarray:
     out = np.zeros(pos.shape[0], np.int32)
     _check(load().lus_cell_ids(C.byref(g), pos.shape[0], _ptr_f(pos), _ptr_i(out)), "lus_cell_ids")
     return out
+
+
+def fluid_params(sim) -> LusFluidParams:
+    """LusFluidParams from any object with the reference Simulation's field
+    names (particleRadius, kernelRadius, kernelFactor, cubic_kernel_k/l, mass,
+    rest_density, relaxation_epsilon, s_corr_dq/k/n, gravity, domainX/Y/Z,
+    cell_size, gridX/Y/Z)."""
+    p = LusFluidParams()
+    for f in ("particleRadius", "kernelRadius", "kernelFactor", "cubic_kernel_k", "cubic_kernel_l", "mass",
+              "rest_density", "relaxation_epsilon", "s_corr_dq", "s_corr_k", "s_corr_n"):
+        setattr(p, f, getattr(sim, f))
+    for a in range(3):
+        p.gravity[a] = sim.gravity[a]
+    p.domain[0], p.domain[1], p.domain[2] = sim.domainX, sim.domainY, sim.domainZ
+    p.grid = grid(sim.cell_size, sim.gridX, sim.gridY, sim.gridZ)
+    return p
 
 
 def counting_sort_noalloc(keys_dev, n, num_cells, counts_dev, sorted_dev, stream=None):
@@ -461,6 +493,10 @@ class Gpu:
         pl = None if player is None else _f32(player, (3,))
         _check(self._lib.lus_sand_step(self._h, dt, int(attract), int(blow), _ptr_f(pl)), "lus_sand_step")
 
+    def sand_set_variant(self, variant):
+        """0 simulate_sand, 1 simulate_sand_credits, 2 simulate_sand_v3."""
+        _check(self._lib.lus_sand_set_variant(self._h, int(variant)), "lus_sand_set_variant")
+
     def sand_download(self):
         n = self.n_sand
         pos = np.zeros((n, 3), np.float32)
@@ -471,6 +507,40 @@ class Gpu:
         _check(self._lib.lus_sand_download(self._h, _ptr_f(pos), _ptr_f(ps), _ptr_f(vel), _ptr_i(att),
                                            _ptr_i(perm)), "lus_sand_download")
         return pos, ps, vel, att, perm
+
+    # --- simulate_fluid (src/Simulate.cpp:27-115)
+    def fluid_init(self, params: LusFluidParams, positions, positions_star, velocities, sand, solid,
+                   lambdas=None):
+        """The reference's slot layout: arrays of `total` rows; sand = (start,
+        end) of the moving slots, solid = (start, end) of the solid slots."""
+        p = _f32(positions, (-1, 3))
+        ps = _f32(positions_star, (-1, 3))
+        v = _f32(velocities, (-1, 3))
+        total = p.shape[0]
+        assert ps.shape[0] == total and v.shape[0] >= sand[1]
+        lam = np.zeros(total, np.float32) if lambdas is None else np.ascontiguousarray(lambdas, np.float32)
+        self.fluid_total, self.fluid_nl = total, lam.shape[0]
+        _check(self._lib.lus_fluid_init(self._h, C.byref(params), total, sand[0], sand[1], solid[0], solid[1],
+                                        _ptr_f(p), _ptr_f(ps), _ptr_f(v), lam.shape[0], _ptr_f(lam)),
+               "lus_fluid_init")
+
+    def fluid_step(self, dt):
+        _check(self._lib.lus_fluid_step(self._h, float(dt)), "lus_fluid_step")
+
+    def fluid_download(self):
+        """(positions, positions_star, velocities, lambdas) in the slot layout;
+        velocities hold only the moving slots' values."""
+        n = self.fluid_total
+        p, ps, v = (np.zeros((n, 3), np.float32) for _ in range(3))
+        lam = np.zeros(self.fluid_nl, np.float32)
+        _check(self._lib.lus_fluid_download(self._h, _ptr_f(p), _ptr_f(ps), _ptr_f(v), _ptr_f(lam)),
+               "lus_fluid_download")
+        return p, ps, v, lam
+
+    def fluid_stats(self):
+        lv, ent = C.c_int(0), C.c_long(0)
+        _check(self._lib.lus_fluid_stats(self._h, C.byref(lv), C.byref(ent)), "lus_fluid_stats")
+        return lv.value, ent.value
 
     def sand_neighbors(self):
         n = self.n_sand

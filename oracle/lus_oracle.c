@@ -405,11 +405,30 @@ int orc_sink_evict(const OrcSim* s, OrcSandState* st, const int* cells, int ncel
     return removed;
 }
 
-int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
-    const float collision_coeff = 0.8f;   /* Simulate.cpp:159-163 */
-    const float friction_coeff = 0.7f;
-    const float mu_s = 0.95f;
-    const float mu_k = 0.9f;
+/* solve_boundary_collision_constraint (src/Simulate.cpp:119-131) */
+static void boundary_constraint(const float n[3], const float p0[3], const float* p, float d, float out[3]) {
+    float rel[3] = {p[0] - p0[0], p[1] - p0[1], p[2] - p0[2]};
+    float C = v3dot(n, rel) - d;
+    out[0] = out[1] = out[2] = 0.0f;
+    if (C >= 0) return;
+    float sv = C / v3dot(n, n);
+    for (int c = 0; c < 3; c++) out[c] = (-sv) * n[c];
+}
+
+/* variant 0: simulate_sand (src/Simulate.cpp:156-325); 1: simulate_sand_credits
+ * (:327-510); 2: simulate_sand_v3 (:513-667).  The three share the neighbour
+ * search and the contact + friction projection; they differ in the locals
+ * (:159-163, :330-334, :516-520), the integration loop, and what follows the
+ * contacts (box clamp per substep for 0 and 1; for 2, four identical passes
+ * from the same positions_tmp, then the six half-space boundary projections
+ * from positions_tmp, :601-616). */
+int orc_simulate_sand_variant(const OrcSim* s, OrcSandState* st, float dt, int variant) {
+    const float collision_coeff = 0.8f;
+    const float boundary_collision_coeff = 0.9f;
+    const float friction_coeff = variant == 2 ? 0.5f : 0.7f;
+    const float mu_s = variant == 0 ? 0.95f : 0.8f;
+    const float mu_k = variant == 0 ? 0.9f : (variant == 1 ? 0.7f : 0.8f);
+    if (variant == 2) dt = gclamp(dt, 0.001f, 0.016f); /* :522 */
     const float time_step = dt;
     const float X = s->domainX, Y = s->domainY, Z = s->domainZ;
     const float r = s->particleRadius;
@@ -422,13 +441,59 @@ int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
         float w = 1.0f / s->mass;
         float* vi = v + 3 * (size_t)i;
         float* pi = p + 3 * (size_t)i;
+        if (variant == 2) { /* :541-556 */
+            for (int c = 0; c < 3; c++) vi[c] = vi[c] + s->gravity[c] * dt;
+            if (st->attract_flag) {
+                float d[3] = {st->player_position[0] - pi[0], st->player_position[1] - pi[1],
+                              st->player_position[2] - pi[2]};
+                float inv = 1.0f / sqrtf(v3dot(d, d));
+                for (int c = 0; c < 3; c++) {
+                    float t = d[c] * inv;
+                    t = t * r;
+                    t = t * 1000.0f;
+                    t = t * dt;
+                    t = t * w;
+                    vi[c] = vi[c] + t;
+                }
+            }
+            if (st->blow_flag) {
+                float d[3] = {st->player_position[0] - pi[0], st->player_position[1] - pi[1],
+                              st->player_position[2] - pi[2]};
+                float inv = 1.0f / sqrtf(v3dot(d, d));
+                for (int c = 0; c < 3; c++) {
+                    float t = -(d[c] * inv);
+                    t = t * r;
+                    t = t * 200.0f;
+                    t = t * w;
+                    vi[c] = vi[c] + t;
+                }
+            }
+            float* psi = ps + 3 * (size_t)i;
+            for (int c = 0; c < 3; c++) psi[c] = pi[c] + vi[c] * dt; /* no clamp */
+            continue;
+        }
+        if (variant == 1 && (st->attracted[i] & 2)) { /* :361-364 credits fall at 1 */
+            vi[0] = 0.0f;
+            vi[1] = -1.0f;
+            vi[2] = 0.0f;
+        }
         for (int c = 0; c < 3; c++) vi[c] = vi[c] + s->gravity[c] * dt;
-        if (st->prev_attract_flag && !st->attract_flag) st->attracted[i] = 0;
+        if (st->prev_attract_flag && !st->attract_flag) {
+            if (variant == 1) st->attracted[i] &= ~1;
+            else st->attracted[i] = 0;
+        }
         if (st->attract_flag) {
             float d0[3] = {st->player_position[0] - pi[0], st->player_position[1] - pi[1],
                            st->player_position[2] - pi[2]};
-            if (v3len(d0) < st->attract_radius) st->attracted[i] = 1;
-            if (st->attracted[i]) {
+            if (v3len(d0) < st->attract_radius) {
+                if (variant == 1) {
+                    st->attracted[i] |= 1;
+                    st->attracted[i] &= ~2;
+                } else {
+                    st->attracted[i] = 1;
+                }
+            }
+            if (variant == 1 ? (st->attracted[i] & 1) : st->attracted[i]) {
                 float o[3] = {st->player_position[0] + 0.0f, st->player_position[1] + 1.5f,
                               st->player_position[2] + 0.0f};
                 float d[3] = {o[0] - pi[0], o[1] - pi[1], o[2] - pi[2]};
@@ -459,6 +524,7 @@ int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
                     t = t * w;
                     vi[c] = vi[c] + t;
                 }
+                if (variant == 1) st->attracted[i] &= ~2;
             }
         }
         float* psi = ps + 3 * (size_t)i;
@@ -472,8 +538,10 @@ int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
     IList* nbrs = (IList*)calloc((size_t)nslots, sizeof(IList));
     sand_neighbors_v1(s, st, nbrs);
     p = st->positions; ps = st->positions_star; v = st->velocities;
-    /* positions_tmp :217-223 */
-    if (st->first_iteration) {
+    /* positions_tmp :217-223 (variant 2: every slot on every call, :562) */
+    if (variant == 2) {
+        memcpy(st->positions_tmp, ps, sizeof(float) * 3 * st->total_allocated);
+    } else if (st->first_iteration) {
         memcpy(st->positions_tmp, ps, sizeof(float) * 3 * st->total_allocated);
         st->first_iteration = 0;
     } else {
@@ -481,8 +549,13 @@ int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
     }
     float* pt = st->positions_tmp;
     const float dd = s->particleDiameter;
-    for (int substep = 0; substep < 4; ++substep) { /* :226-311 */
+    /* variants 0 and 1: 4 Jacobi substeps (:226-311); variant 2: the 4
+     * substeps sit inside the particle loop and all read the same
+     * positions_tmp (:566-633) */
+    const int outer = variant == 2 ? 1 : 4, inner = variant == 2 ? 4 : 1;
+    for (int substep = 0; substep < outer; ++substep) {
         for (int i = st->ptr_sand_start; i < st->ptr_sand_end; i++) {
+          for (int rep = 0; rep < inner; ++rep) {
             float dp[3] = {0.0f, 0.0f, 0.0f};
             for (int t = 0; t < nbrs[i].n; t++) {
                 int j = nbrs[i].a[t];
@@ -490,7 +563,8 @@ int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
                 const float* pi = pt + 3 * (size_t)i;
                 const float* pj = pt + 3 * (size_t)j;
                 float ij[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
-                if (v3len(ij) == 0.0f) { ij[0] = 0.0f; ij[1] = 0.00001f; ij[2] = 0.0f; }
+                /* zero-length fix of :236-239 (absent from variant 2) */
+                if (variant != 2 && v3len(ij) == 0.0f) { ij[0] = 0.0f; ij[1] = 0.00001f; ij[2] = 0.0f; }
                 float len = v3len(ij);
                 if (len > dd) continue;
                 float tmp[3], xid[3], xjd[3], nx[3], rel[3], xt[3];
@@ -545,20 +619,43 @@ int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
                         for (int c = 0; c < 3; c++) dp[c] = dp[c] - (friction_coeff * xt[c]) * ratio;
                     }
                 }
+                if (variant == 1 && (st->attracted[i] & 2)) { /* :463-469 */
+                    /* the three draws are made and not used */
+                    (void)rand();
+                    (void)rand();
+                    (void)rand();
+                    st->attracted[i] &= ~2;
+                }
             }
             for (int c = 0; c < 3; c++) ps[3 * (size_t)i + c] = pt[3 * (size_t)i + c] + dp[c];
+          }
         }
-        for (int i = st->ptr_sand_start; i < st->ptr_sand_end; i++) /* :296-308 */
-            for (int c = 0; c < 3; c++) ps[3 * (size_t)i + c] = gclamp(ps[3 * (size_t)i + c], r, hiv[c]);
+        if (variant == 2) { /* :601-616: six half-spaces, evaluated at positions_tmp */
+            const float wn[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+            const float w0[6][3] = {{0, 0, 0}, {X, 0, 0}, {0, 0, 0}, {0, Y, 0}, {0, 0, 0}, {0, 0, Z}};
+            for (int i = st->ptr_sand_start; i < st->ptr_sand_end; i++) {
+                const float* q = pt + 3 * (size_t)i;
+                float dp[3] = {0.0f, 0.0f, 0.0f};
+                for (int wall = 0; wall < 6; wall++) {
+                    float bc[3];
+                    boundary_constraint(wn[wall], w0[wall], q, r, bc);
+                    for (int c = 0; c < 3; c++) dp[c] = dp[c] + boundary_collision_coeff * bc[c];
+                }
+                for (int c = 0; c < 3; c++) ps[3 * (size_t)i + c] = ps[3 * (size_t)i + c] + dp[c];
+            }
+        } else {
+            for (int i = st->ptr_sand_start; i < st->ptr_sand_end; i++) /* :296-308 */
+                for (int c = 0; c < 3; c++) ps[3 * (size_t)i + c] = gclamp(ps[3 * (size_t)i + c], r, hiv[c]);
+        }
         memcpy(pt + 3 * (size_t)st->ptr_sand_start, ps + 3 * (size_t)st->ptr_sand_start,
-               sizeof(float) * 3 * (size_t)st->num_sand_particles); /* :310 */
+               sizeof(float) * 3 * (size_t)st->num_sand_particles); /* :310, :619 */
     }
     for (int i = st->ptr_sand_start; i < st->ptr_sand_end; i++) /* :316-319 */
         for (int c = 0; c < 3; c++) {
             v[3 * (size_t)i + c] = (ps[3 * (size_t)i + c] - p[3 * (size_t)i + c]) / time_step;
             p[3 * (size_t)i + c] = ps[3 * (size_t)i + c];
         }
-    st->prev_attract_flag = st->attract_flag; /* :323 */
+    if (variant != 2) st->prev_attract_flag = st->attract_flag; /* :323, :508 */
     /* flatten the neighbor lists for inspection */
     long total = 0;
     for (int i = st->ptr_sand_start; i < st->ptr_sand_end; i++) total += nbrs[i].n;
@@ -580,6 +677,10 @@ int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
     for (int i = st->ptr_sand_end; i < nslots; i++) free(nbrs[i].a);
     free(nbrs);
     return 0;
+}
+
+int orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt) {
+    return orc_simulate_sand_variant(s, st, dt, 0);
 }
 
 void orc_sand_state_free(OrcSandState* st) {

@@ -112,6 +112,9 @@ int orc_sink_evict(const OrcSim* s, OrcSandState* st, const int* cells, int ncel
 
 /* One call of simulate_sand.  Returns 0 on success. */
 int  orc_simulate_sand(const OrcSim* s, OrcSandState* st, float dt);
+/* One call of the sand function `variant`: 0 simulate_sand (src/Simulate.cpp:
+ * 156-325), 1 simulate_sand_credits (:327-510), 2 simulate_sand_v3 (:513-667). */
+int  orc_simulate_sand_variant(const OrcSim* s, OrcSandState* st, float dt, int variant);
 void orc_sand_state_free(OrcSandState* st);
 
 /* One call of simulate_fluid (src/Simulate.cpp:27-115), sequential in-place

@@ -122,6 +122,7 @@ def load(path: str):
                                       C.c_int, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_int),
                                       C.c_long]
     L.orc_simulate_sand.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState), C.c_float]
+    L.orc_simulate_sand_variant.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState), C.c_float, C.c_int]
     L.orc_sand_state_free.argtypes = [C.POINTER(OrcSandState)]
     L.orc_query_cell_num_particles.restype = C.c_int
     L.orc_query_cell_num_particles.argtypes = [C.POINTER(OrcSim), C.POINTER(OrcSandState),
@@ -245,8 +246,9 @@ class SandStream:
         self.st = st
         self.n = n
 
-    def step(self, dt):
-        lib().orc_simulate_sand(C.byref(self.s), C.byref(self.st), dt)
+    def step(self, dt, variant=0):
+        """variant 0 simulate_sand, 1 simulate_sand_credits, 2 simulate_sand_v3."""
+        lib().orc_simulate_sand_variant(C.byref(self.s), C.byref(self.st), dt, variant)
 
     def perm(self):
         return np.ctypeslib.as_array(self.st.sorted_indices, shape=(self.n,)).copy()

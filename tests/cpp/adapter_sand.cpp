@@ -1,8 +1,14 @@
-// Drives lus_adapter::simulate_sand / simulate_pbf through a stand-in with the
+// Drives the lus_adapter Simulate_fun bodies through a stand-in with the
 // member names and layout of Lustrine::Simulation (src/Simulation.hpp:135-251)
 // that the adapter reads.  Input/output: raw little-endian files.
 //   adapter_sand.exe sand     <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
+//   adapter_sand.exe credits  ... (simulate_sand_credits; attracted = 2 on every third slot)
+//   adapter_sand.exe v3       ... (simulate_sand_v3)
+//   adapter_sand.exe sandlazy ... (simulate_sand with the host arrays synchronised on demand)
+//   adapter_sand.exe fluid    ... (simulate_fluid; out also float32[total] lambdas)
 //   adapter_sand.exe pbf      <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
+//   adapter_sand.exe bench    <in> <out> <steps> ...: times simulate_sand calls, every-step
+//                             sync then on-demand sync, and writes "ms_every ms_lazy ms_sync"
 //   adapter_sand.exe sandsink <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
 //   adapter_sand.exe sandbox  <in> <out> <steps> <X> <Y> <Z> <kernel_radius_scale>
 // in:  int32 n_sand, int32 n_solid, int32 total, float32[3*n_sand] sand,
@@ -15,6 +21,7 @@
 //      num_remaining_sand_particles;
 //      sandbox: then float32 player_position[3], player_box_scale[3]; out
 //      first int32 count and float32[3*count] collider positions
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -52,6 +59,9 @@ struct Simulation {  // members read by the adapter, with their reference types
     float time_step = 0.01f, relaxation_epsilon = 10.0f;
     float s_corr_dq = 0.5f, s_corr_k = 1.0f, s_corr_n = 4, c_xsph = 0.1f;
     vec3* velocities = nullptr;
+    std::vector<float> lambdas;
+    float cubic_kernel_k = 0.0f, cubic_kernel_l = 0.0f;
+    size_t total_allocated = 0;
     int gridX, gridY, gridZ;
     float cell_size;
     bool computed_static_particles = false;
@@ -118,6 +128,16 @@ int main(int argc, char** argv) {
     s.gridX = (int)(s.domainX / s.cell_size) + 1;
     s.gridY = (int)(s.domainY / s.cell_size) + 1;
     s.gridZ = (int)(s.domainZ / s.cell_size) + 1;
+    {   // src/Lustrine.cpp:253-259: k and l in double from h^3 as a float
+        const double pi = 3.14159265358979323846;
+        const float h3 = (float)std::pow((double)s.kernelRadius, 3.0);
+        s.cubic_kernel_k = (float)(8.0f / (pi * (double)h3));
+        s.cubic_kernel_l = (float)(48.0f / (pi * (double)h3));
+    }
+    s.total_allocated = (size_t)total;
+    s.lambdas.assign((size_t)total, 0.0f);
+    if (mode == "credits")
+        for (int i = 0; i < n; i += 3) att[i] = 2;
     s.positions = pos.data();
     s.positions_star = ps.data();
     s.velocities = vel.data();
@@ -129,9 +149,38 @@ int main(int argc, char** argv) {
     s.ptr_solid_end = ns ? total : total - 1;
     s.sink = &sink;
     const int steps = atoi(argv[4]);
+    if (mode == "bench") {
+        // one warm call, then `steps` timed calls with each sync mode
+        auto now = [] { return std::chrono::steady_clock::now(); };
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        if (lus_adapter::simulate_sand(&s, 0.016f)) return 1;
+        auto t0 = now();
+        for (int t = 0; t < steps; ++t)
+            if (lus_adapter::simulate_sand(&s, 0.016f)) return 1;
+        auto t1 = now();
+        lus_adapter::set_sync(&s, lus_adapter::kSyncOnDemand);
+        auto t2 = now();
+        for (int t = 0; t < steps; ++t)
+            if (lus_adapter::simulate_sand(&s, 0.016f)) return 1;
+        lus_gpu_sync(lus_adapter::states()[&s].ctx);
+        auto t3 = now();
+        if (lus_adapter::sync_host(&s)) return 1;
+        auto t4 = now();
+        lus_adapter::release(&s);
+        FILE* o = fopen(argv[3], "w");
+        if (!o) return 2;
+        fprintf(o, "%.4f %.4f %.4f\n", ms(t0, t1) / steps, ms(t2, t3) / steps, ms(t3, t4));
+        fclose(o);
+        return 0;
+    }
+    if (mode == "sandlazy") lus_adapter::set_sync(&s, lus_adapter::kSyncOnDemand);
     for (int t = 0; t < steps; ++t) {
-        int rc = mode == "pbf" ? lus_adapter::simulate_pbf(&s, 1.0f / 60.0f, 4, 1)
-                               : lus_adapter::simulate_sand(&s, 0.016f);
+        int rc;
+        if (mode == "pbf") rc = lus_adapter::simulate_pbf(&s, 1.0f / 60.0f, 4, 1);
+        else if (mode == "credits") rc = lus_adapter::simulate_sand_credits(&s, 0.016f);
+        else if (mode == "v3") rc = lus_adapter::simulate_sand_v3(&s, 0.016f);
+        else if (mode == "fluid") rc = lus_adapter::simulate_fluid(&s, 1.0f / 60.0f);
+        else rc = lus_adapter::simulate_sand(&s, 0.016f);
         if (!rc && mode == "sandsink") rc = lus_adapter::evict_sinks(&s);
         if (rc) {
             char msg[512];
@@ -139,6 +188,13 @@ int main(int argc, char** argv) {
             fprintf(stderr, "step %d failed: %s\n", t, msg);
             return 1;
         }
+    }
+    if (mode == "sandlazy") {
+        // positions only first (what simulation_bind_positions_copy reads),
+        // then the rest
+        if (lus_adapter::sync_positions(&s)) return 1;
+        if (lus_adapter::states()[&s].pos_stale || !lus_adapter::states()[&s].rest_stale) return 3;
+        if (lus_adapter::sync_host(&s)) return 1;
     }
     int q[2] = {0, 0};
     if (mode == "sandsink") {
@@ -169,6 +225,10 @@ int main(int argc, char** argv) {
     fwrite(pos.data(), sizeof(float), 3 * (size_t)nw, o);
     fwrite(vel.data(), sizeof(float), 3 * (size_t)nw, o);
     fwrite(att.data(), sizeof(int), (size_t)nw, o);
+    if (mode == "fluid") {
+        fwrite(ps.data(), sizeof(float), 3 * (size_t)total, o);
+        fwrite(s.lambdas.data(), sizeof(float), (size_t)total, o);
+    }
     fclose(o);
     return 0;
 }

@@ -137,3 +137,63 @@ def test_adapter_player_box_matches_reference(gpu_mod, oracle, tmp_path):
     got = raw[4: 4 + 12 * k].view(np.float32).reshape(k, 3)
     assert k == len(want) and k > 20
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("mode,variant", [("credits", 1), ("v3", 2), ("sandlazy", 0)])
+def test_adapter_sand_variants_and_lazy_sync_match_reference(gpu_mod, oracle, tmp_path, mode, variant):
+    # simulate_sand_credits (every third slot flagged 2 by the stand-in),
+    # simulate_sand_v3, and simulate_sand with the host arrays brought back on
+    # demand only (sync_positions, then sync_host): the same host state
+    box = (16, 16, 16)
+    sand = scenes.jittered_lattice(8, 8, 8, origin=(3.0, 2.0, 3.0), amplitude=0.2, seed=13)
+    solids = scenes.lattice(16, 1, 16)
+    s = oracle.init_constants(*box, 0.5, 1.0, 3.1)
+    st = oracle.SandStream(s, sand, solids)
+    n = sand.shape[0]
+    if mode == "credits":
+        st.att[0:n:3] = 2
+    for _ in range(3):
+        st.step(np.float32(0.016), variant=variant)
+    pos, vel, att = _run(mode, tmp_path, sand, solids, st.total, box, 3.1, 3)
+    assert np.array_equal(pos, st.p[:n])
+    assert np.array_equal(vel, st.v[:n])
+    assert np.array_equal(att, st.att[:n])
+
+
+def test_adapter_simulate_fluid_matches_reference(gpu_mod, oracle, tmp_path):
+    # the GPU simulate_fluid as a Simulate_fun on the stand-in, against the
+    # oracle's simulate_fluid: 2 steps of a block over a floor of solids
+    box = (16, 16, 16)
+    sand = scenes.jittered_lattice(8, 6, 8, origin=(3.0, 2.0, 3.0), amplitude=0.005, seed=14)
+    solids = scenes.lattice(16, 1, 16)
+    s = oracle.init_constants(*box, 0.5, 1.0, 4.0)
+    n, ns = sand.shape[0], solids.shape[0]
+    total = max(n + ns, 16 ** 3)
+    p = np.zeros((total, 3), np.float32)
+    p[:n] = sand
+    for k in range(ns):
+        p[total - 1 - k] = solids[k]
+    po, pso, vo, lo = p.copy(), p.copy(), np.zeros_like(p), np.zeros(total, np.float32)
+    for _ in range(2):
+        oracle.simulate_fluid(s, po, pso, vo, lo, (0, n), (total - ns, total), 1.0 / 60.0)
+    inp = tmp_path / "fluid_in.bin"
+    out = tmp_path / "fluid_out.bin"
+    with open(inp, "wb") as f:
+        np.array([n, ns, total], np.int32).tofile(f)
+        np.ascontiguousarray(sand, np.float32).tofile(f)
+        np.ascontiguousarray(solids, np.float32).tofile(f)
+    r = subprocess.run([EXE, "fluid", str(inp), str(out), "2", *map(str, box), "4.0"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.uint8)
+    pos = raw[: 12 * n].view(np.float32).reshape(n, 3)
+    vel = raw[12 * n: 24 * n].view(np.float32).reshape(n, 3)
+    off = 28 * n
+    ps = raw[off: off + 12 * total].view(np.float32).reshape(total, 3)
+    lam = raw[off + 12 * total: off + 16 * total].view(np.float32)
+    # positions within 1e-5 of the largest coordinate (s_coor's pow may move
+    # an ulp), positions_star equal to positions, solids untouched
+    assert np.abs(pos - po[:n]).max() <= 1e-5 * np.abs(po[:n]).max()
+    assert np.abs(vel - vo[:n]).max() <= 1e-5 * max(np.abs(vo[:n]).max(), 1.0) + 4 * 2.0e-6 / 0.01
+    assert np.array_equal(ps[:n], pos) and np.array_equal(ps[n:], p[n:])
+    assert np.abs(lam[:n] - lo[:n]).max() <= 1e-5 * np.abs(lo[:n]).max()

@@ -202,7 +202,7 @@ def test_sand_head_on_contact_closed_form(oracle):
     assert np.allclose(st.v[:2, 0], (p[:, 0] - pos[st.perm(), 0]) / dt, rtol=1e-6)
 
 
-def _sand_on_solid(oracle, slide0, overlap0):
+def _sand_on_solid(oracle, slide0, overlap0, variant=0):
     # one sand particle resting on a solid particle (old position at contact
     # distance 1 above it, shifted slide0 to the -x side) and predicted exactly
     # above it, overlap0 into it (x* = x_old + v dt with dyadic dt and v, so
@@ -219,7 +219,7 @@ def _sand_on_solid(oracle, slide0, overlap0):
     st.v[0] = [np.float32(slide0) / dt, -np.float32(overlap0) / dt, 0.0]
     pred = old[0] + st.v[0] * dt
     assert np.array_equal(pred, np.array([1.0, 1.0 - overlap0, 1.0], np.float32))
-    st.step(dt)
+    st.step(dt, variant)
     x = st.p[0].astype(np.float64)
     return x[0] - float(old[0, 0]), 1.0 - float(np.linalg.norm(x - solid[0]))
 
@@ -538,3 +538,104 @@ def test_grid_from_voxels_closed_form(oracle):
     assert colors[c1].tolist() == [1.0, 0.0, np.float32(51) / np.float32(255), 1.0]
     assert np.array_equal(colors[c2], np.array([10, 20, 30, 40], np.float32) / np.float32(255))
     assert np.all(colors[[i for i in range(24) if i not in (c1, c2)]] == 0)
+
+
+# ------------------------------------------------------------------ sand variants
+def test_sand_v3_head_on_contact_is_one_projection(oracle):
+    # simulate_sand_v3 (src/Simulate.cpp:513-667) runs its 4 substeps inside
+    # the particle loop, all from the same positions_tmp: the overlap shrinks
+    # by 0.8 once (0.2 remains), not 0.2^4 as in simulate_sand
+    s = oracle.init_constants(16, 16, 16, 0.5, 1.0, 3.1)
+    s.gravity[:] = [0.0, 0.0, 0.0]
+    gap0 = 0.3
+    pos = np.array([[8.0, 8.0, 8.0], [8.0 + 1.0 - gap0, 8.0, 8.0]], np.float32)
+    st = oracle.SandStream(s, pos, None)
+    st.step(np.float32(1.0 / 64.0), variant=2)
+    p = st.p[:2]
+    assert abs((1.0 - abs(p[1, 0] - p[0, 0])) - gap0 * 0.2) < 1e-6
+    assert abs((p[0, 0] + p[1, 0]) - (16.0 + 1.0 - gap0)) < 1e-6
+
+
+def test_sand_v3_boundary_projection_and_dt_clamp(oracle):
+    # no box clamp in v3: a prediction past a wall is pushed back by 0.9 of
+    # its depth (solve_boundary_collision_constraint, :119-131, evaluated at
+    # positions_tmp, :601-616), and dt is clamped to [0.001, 0.016] (:522)
+    s = oracle.init_constants(16, 16, 16, 0.5, 1.0, 3.1)
+    s.gravity[:] = [0.0, 0.0, 0.0]
+    pos = np.array([[1.0, 8.0, 8.0]], np.float32)
+    st = oracle.SandStream(s, pos, None)
+    st.v[0] = [-50.0, 0.0, 0.0]
+    st.step(np.float32(0.1), variant=2)  # dt -> 0.016: x* = 1 - 0.8 = 0.2
+    xs = np.float32(1.0) + np.float32(-50.0) * np.float32(0.016)
+    want = xs + np.float32(0.9) * (np.float32(0.5) - xs)
+    assert abs(st.p[0, 0] - want) < 1e-6 and st.p[0, 0] < 0.5
+    assert abs(st.v[0, 0] - (want - 1.0) / 0.016) < 1e-3
+
+
+def test_sand_v3_static_friction_half(oracle):
+    # v3's friction 0.5, mu_s 0.8: one static projection halves the slide;
+    # the overlap keeps 0.2 of itself
+    slide0, overlap0 = 1.0 / 1024.0, 1.0 / 128.0
+    slide, overlap = _sand_on_solid(oracle, slide0, overlap0, variant=2)
+    assert abs(slide / (slide0 * 0.5) - 1.0) < 0.01
+    assert abs(overlap / (overlap0 * 0.2) - 1.0) < 0.01
+
+
+def test_sand_credits_fall_and_release(oracle):
+    # simulate_sand_credits (:327-510): a particle flagged 2 has its velocity
+    # set to (0, -1, 0) before gravity; the flag clears at its first contact
+    s = oracle.init_constants(16, 16, 16, 0.5, 1.0, 3.1)
+    s.gravity[:] = [0.0, 0.0, 0.0]
+    pos = np.array([[4.0, 8.0, 4.0], [10.0, 8.0, 10.0], [10.0, 8.9, 10.0]], np.float32)
+    st = oracle.SandStream(s, pos, None)
+    st.att[:3] = [2, 2, 0]
+    st.v[0] = [5.0, 5.0, 5.0]
+    dt = np.float32(1.0 / 64.0)
+    st.step(dt, variant=1)
+    perm = st.perm()
+    back = {int(o): k for k, o in enumerate(perm)}
+    k0 = back[0]
+    assert np.array_equal(st.p[k0], pos[0] + np.array([0, -1, 0], np.float32) * dt)
+    assert st.att[k0] == 2  # no contact: still a credit particle
+    assert st.att[back[1]] == 0  # touched particle 2: released
+    assert st.att[back[2]] == 0
+
+
+def test_sand_credits_friction_constants(oracle):
+    # credits' mu_s 0.8, mu_k 0.7 (friction 0.7): static leaves 0.3^4 of the
+    # slide; kinetic removes 0.7 * 0.7 * 0.8 * overlap_k per substep
+    slide, overlap = _sand_on_solid(oracle, 1.0 / 1024.0, 1.0 / 128.0, variant=1)
+    assert abs(slide / (1.0 / 1024.0 * 0.3 ** 4) - 1.0) < 0.01
+    slide0, overlap0 = 1.0 / 8.0, 1.0 / 128.0
+    slide, overlap = _sand_on_solid(oracle, slide0, overlap0, variant=1)
+    removed = 0.7 * 0.7 * overlap0 * (1 - 0.2 ** 4)
+    assert abs((slide0 - slide) / removed - 1.0) < 0.01
+
+
+def counting_sort2(keys):
+    """Sorting::counting_sort2 (src/neighbors/Sorting.cpp:35-68): counts in a
+    hash map, the used cells sorted, running sums, then the backward scatter
+    of counting_sort."""
+    counts, used = {}, set()
+    for k in keys:
+        counts[int(k)] = counts.get(int(k), 0) + 1
+        used.add(int(k))
+    run = 0
+    for c in sorted(used):
+        run += counts[c]
+        counts[c] = run
+    out = [0] * len(keys)
+    for i in range(len(keys) - 1, -1, -1):
+        c = int(keys[i])
+        out[counts[c] - 1] = i
+        counts[c] -= 1
+    return np.array(out, np.int32)
+
+
+@pytest.mark.parametrize("n,c", [(1, 1), (50, 3), (20000, 1987), (5000, 1 << 20)])
+def test_counting_sort2_gives_counting_sort_permutation(oracle, n, c):
+    # the hash-map variant used by the v1/v2 sand paths yields the same
+    # stable permutation as counting_sort, which the GPU sort reproduces
+    keys = glibc_rand_keys(n, c) if c == 1987 else np.random.default_rng(n).integers(0, c, n).astype(np.int32)
+    perm, _ = oracle.counting_sort(keys, c)
+    assert np.array_equal(counting_sort2(keys), perm)
