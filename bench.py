@@ -315,6 +315,7 @@ def bench_sand(args, G, scenes) -> int:
         out["phases_ms"] = {"step": slots[0] * 1e3, "neighbour_search": slots[3] * 1e3,
                             "substeps": slots[4] * 1e3, "velocity_update": slots[6] * 1e3,
                             "window": f"step {args.warmup + args.steps + 1}"}
+    out["adapter_call"] = adapter_call_ms(pos, cfg.box, steps=10)
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         O.build()
@@ -329,6 +330,82 @@ def bench_sand(args, G, scenes) -> int:
         out["cpu_baseline"] = {"value": n * done / csec, "unit": "particle-steps/s", "cores": 1, "kind": "port",
                                "sample": f"steps 2-{done + 1} of the sand scene ({csec:.1f} s), FP32 oracle "
                                          f"simulate_sand, 1 thread of {cpu_model()}"}
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+    return 0
+
+
+def adapter_call_ms(pos, box, steps=10) -> dict:
+    """One lus_adapter::simulate_sand call on a Simulation-shaped host state
+    (tests/cpp/adapter_sand.exe, the header-only drop-in): ms per call when
+    every call leaves positions, positions_star, velocities and attracted in
+    the host arrays, ms per call when they stay on the device, and the ms of
+    the one synchronisation that brings them back."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "tests", "cpp", "adapter_sand.exe")
+    if not os.path.exists(exe):
+        return {"skipped": "tests/cpp/adapter_sand.exe is not built"}
+    n = pos.shape[0]
+    total = max(n, int(box[0]) * int(box[1]) * int(box[2]))
+    with tempfile.TemporaryDirectory() as td:
+        inp, outp = os.path.join(td, "in.bin"), os.path.join(td, "out.txt")
+        with open(inp, "wb") as f:
+            np.array([n, 0, total], np.int32).tofile(f)
+            np.ascontiguousarray(pos, np.float32).tofile(f)
+        r = subprocess.run([exe, "bench", inp, outp, str(steps), *map(str, box), "3.1"], capture_output=True,
+                           text=True, timeout=300)
+        if r.returncode != 0:
+            return {"error": r.stderr[-400:]}
+        every, lazy, sync = (float(t) for t in open(outp).read().split())
+    return {"ms_per_call_host_synced": every, "ms_per_call_on_demand": lazy, "ms_sync_host": sync,
+            "particles": n, "calls": steps,
+            "note": "lus_adapter::simulate_sand on a Simulation-shaped state; host synced: positions, "
+                    "positions_star, velocities, attracted back in the host arrays after every call"}
+
+
+def bench_fluid(args, G, scenes) -> int:
+    """The reference's simulate_fluid (src/Simulate.cpp:27-115) on the GPU: the
+    1,048,576-particle lattice in 256 x 128 x 256 with the reference's
+    Simulation defaults (rest density 24, epsilon 10, cubic kernel, h = 4 r),
+    its sequential pressure sweep run as dependency levels.  Host clock
+    around synchronised steps (the sweep synchronises every 32 levels)."""
+    from oracle import oracle as O
+    O.build()
+    cfg = scenes.cfg2()
+    s = O.init_constants(*cfg.box, 0.5, 1.0, 4.0)
+    sand = cfg.positions()
+    n = sand.shape[0]
+    total = max(n, int(cfg.box[0]) * int(cfg.box[1]) * int(cfg.box[2]))
+    p = np.zeros((total, 3), np.float32)
+    p[:n] = sand
+    dt = 1.0 / 60.0
+    with G.Gpu(0) as g:
+        g.fluid_init(G.fluid_params(s), p, p, np.zeros_like(p), (0, n), (total - 1, total - 1))
+        for _ in range(args.warmup):
+            g.fluid_step(dt)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.fluid_step(dt)
+        sec = time.perf_counter() - t0
+        levels, entries = g.fluid_stats()
+    out = {"metric": "simulate_fluid particle-steps/s at 1M particles", "value": n * args.steps / sec,
+           "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": sec * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "fluid_dam_break_1m", "particles": n, "sweep_levels": levels,
+                      "list_entries": entries, "parallelism": "single-gpu"}}
+    if not args.no_cpu_baseline:
+        po, pso, vo, lo = p.copy(), p.copy(), np.zeros_like(p), np.zeros(total, np.float32)
+        O.simulate_fluid(s, po, pso, vo, lo, (0, n), (total - 1, total - 1), dt)
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            O.simulate_fluid(s, po, pso, vo, lo, (0, n), (total - 1, total - 1), dt)
+            done += 1
+        csec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * done / csec, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"steps 2-{done + 1} ({csec:.1f} s), FP32 oracle simulate_fluid, "
+                                         f"1 thread of {cpu_model()}"}
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
     return 0
@@ -397,7 +474,8 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg1", "cfg2", "cfg2_tail", "cfg3", "cfg4", "sand_1m"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg0", "cfg1", "cfg2", "cfg2_tail", "cfg3", "cfg4", "sand_1m",
+                                                         "fluid_1m"])
     ap.add_argument("--network", default=None, help="warm-start network file (cfg3)")
     ap.add_argument("--iterations", type=int, default=None)
     ap.add_argument("--exact", action="store_true",
@@ -430,6 +508,8 @@ def main(argv=None):
         return bench_mixed(args, G, scenes)
     if args.workload == "sand_1m":
         return bench_sand(args, G, scenes)
+    if args.workload == "fluid_1m":
+        return bench_fluid(args, G, scenes)
     cfg = {"cfg0": scenes.cfg0, "cfg2": scenes.cfg2, "cfg2_tail": scenes.cfg2_tail}[args.workload]()
     if args.iterations is not None:
         cfg.iterations = args.iterations
@@ -492,16 +572,26 @@ def main(argv=None):
         dom = max((p for p in phases if "alg_bytes_per_launch" in phases[p]), key=lambda k: phases[k]["total_ms"])
         d = phases[dom]
         achieved = d["alg_bytes_per_launch"] / (d["avg_launch_us"] * 1e-6) / 1e9
-        traffic = None
+        # HBM traffic per launch is not measurable inside this process (it
+        # takes the FETCH_SIZE / WRITE_SIZE counters of two profiled runs):
+        # it is read from the committed summary of those runs, whose window
+        # and command are reported with it
+        traffic = traffic_src = None
         tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{cfg.name}.json")
         if os.path.exists(tfile):
             with open(tfile) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+                tj = json.load(f)
+            traffic = tj.get("hbm_bytes_per_launch")
+            traffic_src = {"file": os.path.relpath(tfile, ROOT), "window": tj.get("window"),
+                           "kernel": tj.get("kernel"), "avg_launch_us_profiled": tj.get("avg_launch_us")}
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
                            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                            "avg_launch_us": d["avg_launch_us"],
                            "alg_bytes_per_launch": d["alg_bytes_per_launch"],
-                           "measured_copy_GBps": round(copy_gbps, 1)}
+                           "measured_copy_GBps": round(copy_gbps, 1),
+                           "window": f"steps {args.warmup + 1}-{args.warmup + args.steps} after "
+                                     f"{args.warmup} warmup, kernels timed by their own dispatch events",
+                           "traffic_source": traffic_src}
         out["phases"] = phases
     if not args.no_learned and args.workload == "cfg2" and cfg.iterations >= 1:
         from lustrine_amd import learned as L

@@ -257,6 +257,12 @@ int lus_pbf_set_mode(LusGpu* ctx, int learned, int learned_iterations, float bou
  * learned solution's (before a rerun). */
 int lus_pbf_learned_stats(LusGpu* ctx, long long* steps, long long* reruns, float* errs,
                           float* errs_mean, int cap, int* nerrs);
+/* Particles of the last step whose neighbour list could not be stored (more
+ * than 128 entries, or a neighbour more than 65,535 slots past its stencil
+ * layer's first) and that therefore re-tested their 27-cell candidates in
+ * every pass of that step (0 before a step and for the tiled passes).  No
+ * reference counterpart (the reference's lists are std::vectors). */
+int lus_pbf_retest_count(LusGpu* ctx, long long* count);
 
 /* Training of the network on the device.  A trainer holds a dataset of
  * (raw features, target) rows; the target of a recorded particle is the plain

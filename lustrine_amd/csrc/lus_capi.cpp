@@ -185,6 +185,7 @@ struct PbfState {
     int* maxd = nullptr;   // per-block max of C, then per-block sum of max(0, C)
     int* flag = nullptr;   // 1: rerun the plain step
     int* counters = nullptr;  // [0] measured steps, [1] reruns
+    int* retest = nullptr;    // one int: re-test count of the last step (lus_pbf_retest_count)
     float* hist = nullptr;    // rings of e (max) and of mean max(0, C)
     bool learned = false, check = false;
     int learned_iters = 1;
@@ -966,6 +967,7 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
     LUS_HIP(m.alloc(&P.maxd, 2 * (size_t)P.nblk));
     LUS_HIP(m.alloc(&P.flag, 1));
     LUS_HIP(m.alloc(&P.counters, 2));
+    LUS_HIP(m.alloc(&P.retest, 1));
     LUS_HIP(m.alloc(&P.hist, 2 * LUS_DENSITY_HISTORY));
     LUS_HIP(hipMemset(P.counters, 0, 2 * sizeof(int)));
     LUS_HIP(hipMemset(P.flag, 0, sizeof(int)));
@@ -1435,6 +1437,24 @@ int lus_pbf_learned_stats(LusGpu* c, long long* steps, long long* reruns, float*
         }
     }
     if (nerrs) *nerrs = (errs || errs_mean) ? nh : 0;
+    return LUS_OK;
+}
+
+int lus_pbf_retest_count(LusGpu* c, long long* count) {
+    if (!c || !c->pbf.ready || !count) return set_error(LUS_ESTATE, "lus_pbf_retest_count: no PBF scene");
+    PbfState& P = c->pbf;
+    if (!P.stepped || P.tiled) {
+        *count = 0;  // no lists yet, or the tiled passes (row masks, no lists)
+        return LUS_OK;
+    }
+    LUS_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    LUS_HIP(hipMemsetAsync(P.retest, 0, sizeof(int), s));
+    LUS_HIP(lus::launch_pbf_count_retest(P.masks, P.n, P.retest, s));
+    int v = 0;
+    LUS_HIP(hipMemcpyAsync(&v, P.retest, sizeof(int), hipMemcpyDeviceToHost, s));
+    LUS_HIP(hipStreamSynchronize(s));
+    *count = v;
     return LUS_OK;
 }
 

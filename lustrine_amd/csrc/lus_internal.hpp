@@ -100,6 +100,9 @@ hipError_t launch_stable_rank(const int* tmp, const int* key, const int* cell_st
 hipError_t launch_gather4(const float4* src, const int* perm, int n, float4* dst, hipStream_t s);
 // dst[3 q .. 3 q + 2] = src[r].xyz, q = ids ? ids[r] : r (ids a permutation of 0..n-1)
 hipError_t launch_pack3(const float4* src, const int* ids, int n, float* dst, hipStream_t s);
+// *out += particles of the neighbour-list buffer nb whose header marks the
+// re-test path (*out must be cleared by the caller)
+hipError_t launch_pbf_count_retest(const unsigned long long* nb, int n, int* out, hipStream_t s);
 hipError_t launch_gather1(const int* src, const int* perm, int n, int* dst, hipStream_t s);
 hipError_t launch_stream_copy(const float4* src, float4* dst, size_t n4, hipStream_t s);
 

@@ -1691,4 +1691,20 @@ hipError_t launch_pbf_decide(const int* maxd, int nblk, int n, float bound, int 
     return hipGetLastError();
 }
 
+// Particles of the last list-building pass whose list could not be stored
+// (header count kNbrRetest): they re-test their candidates in every pass.
+__global__ __launch_bounds__(kBlock) void k_pbf_count_retest(const int4* __restrict__ hdr, int n,
+                                                              int* __restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    int c = (i < n && (hdr[i].w & 0xFFFF) == kNbrRetest) ? 1 : 0;
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
+hipError_t launch_pbf_count_retest(const unsigned long long* nb, int n, int* out, hipStream_t s) {
+    launch_kernel(k_pbf_count_retest, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                  reinterpret_cast<const int4*>(nb), n, out);
+    return hipGetLastError();
+}
+
 }  // namespace lus

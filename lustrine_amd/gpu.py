@@ -111,6 +111,7 @@ _SIGS = {
     "lus_pbf_set_network": (C.c_int, [C.c_void_p, _f, C.c_int]),
     "lus_pbf_set_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int]),
     "lus_pbf_learned_stats": (C.c_int, [C.c_void_p, _P(C.c_longlong), _P(C.c_longlong), _f, _f, C.c_int, _i]),
+    "lus_pbf_retest_count": (C.c_int, [C.c_void_p, _P(C.c_longlong)]),
     "lus_mlp_trainer_create": (C.c_int, [C.c_void_p, C.c_long, _P(C.c_void_p)]),
     "lus_mlp_trainer_destroy": (None, [C.c_void_p]),
     "lus_mlp_trainer_add": (C.c_int, [C.c_void_p, _f, _f, C.c_long]),
@@ -441,6 +442,13 @@ class Gpu:
     def pbf_set_mode(self, learned=False, learned_iterations=1, bound=0.01, check=False):
         _check(self._lib.lus_pbf_set_mode(self._h, int(bool(learned)), learned_iterations, bound,
                                           int(bool(check))), "lus_pbf_set_mode")
+
+    def pbf_retest_count(self):
+        """Particles of the last step that took the re-test path (a list past
+        128 entries or an entry beyond 16 bits)."""
+        c = C.c_longlong(0)
+        _check(self._lib.lus_pbf_retest_count(self._h, C.byref(c)), "lus_pbf_retest_count")
+        return c.value
 
     def pbf_learned_stats(self, with_mean=False):
         """(measured steps, plain reruns, per-step max density error e, oldest

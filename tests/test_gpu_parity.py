@@ -317,10 +317,24 @@ def test_pbf_list_entries_beyond_16_bits(gpu_mod, oracle):
         oracle.pbf_step(p, x, v, ids, np.float32(cfg.dt))
     with gpu_mod.Gpu(0) as g:
         g.pbf_init(gpu_mod.pbf_params(cfg), x0, None)
-        g.pbf_step(np.float32(cfg.dt), 2)
+        g.pbf_step(np.float32(cfg.dt), 1)
+        retest = g.pbf_retest_count()
+        g.pbf_step(np.float32(cfg.dt), 1)
         gx, gv, gid = g.pbf_download(order=0)
     assert np.array_equal(gid, ids)
     assert np.array_equal(gx, x) and np.array_equal(gv, v), (max_ulp(gx, x), max_ulp(gv, v))
+    # first step: the middle column's particles whose neighbours span more
+    # than 65,535 slots (about half of them, 16,819 measured) are counted
+    mid = int(pop[1])
+    assert 0.25 * mid <= retest <= mid, (retest, mid)
+
+
+def test_pbf_retest_count_zero_on_dam_break(gpu_mod):
+    cfg = scenes.cfg0()
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_init(gpu_mod.pbf_params(cfg, exact=False), cfg.positions(), None)
+        g.pbf_step(np.float32(cfg.dt), 5)
+        assert g.pbf_retest_count() == 0
 
 
 def test_pbf_original_order_download(gpu_mod):
@@ -417,3 +431,32 @@ def test_sand_no_solids(gpu_mod, oracle):
     assert np.array_equal(got[4], want[4]) and np.array_equal(gidx, oidx)
     for a, b in zip(got[:3], want[:3]):
         assert rel_err(a, b) <= REL
+
+
+def test_pbf_tail_one_step_4m(gpu_mod, oracle):
+    # the 4,194,304-particle tail (256 x 64 x 256 in 512 x 128 x 512, 4.29 M
+    # cells): one exact-mode step bit for bit against the oracle, then a fused
+    # step's size-independent properties (ids a permutation, cell order,
+    # every position in the clamp box and finite)
+    cfg = scenes.cfg2_tail()
+    (gx, gv, gid, _), (x, v, ids, _) = _pbf_compare(gpu_mod, oracle, cfg)
+    assert np.array_equal(gid, ids)
+    print(f"4m ulp(x)={max_ulp(gx, x)} ulp(v)={max_ulp(gv, v)}")
+    assert np.array_equal(gx, x) and np.array_equal(gv, v)
+    x0 = cfg.positions()
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_init(gpu_mod.pbf_params(cfg, exact=False), x0, None)
+        g.pbf_step(np.float32(cfg.dt), 3)
+        fx, fv, fid = g.pbf_download(order=0)
+    assert np.array_equal(np.sort(fid), np.arange(cfg.n))
+    lo, hi = np.float32(0.5), np.array(cfg.box, np.float32) - np.float32(0.5)
+    assert np.isfinite(fx).all() and np.isfinite(fv).all()
+    assert (fx >= lo).all() and (fx <= hi).all()
+    gxs, gys, gzs = cfg.grid
+    cs = np.float32(cfg.cell_size)
+    key = ((fx[:, 1] / cs).astype(np.int64) * gxs * gzs + (fx[:, 0] / cs).astype(np.int64) * gzs
+           + (fx[:, 2] / cs).astype(np.int64))
+    # slots were sorted by the cells of the step's predicted positions; the
+    # final positions stay within a cell or so of them: the cell keys of the
+    # committed positions are nearly sorted
+    assert np.mean(np.diff(key) >= 0) > 0.95
