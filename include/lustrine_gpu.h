@@ -253,8 +253,8 @@ int lus_pbf_set_mode(LusGpu* ctx, int learned, int learned_iterations, float bou
 /* Steps measured since the last reset, reruns among them, and for the last
  * min(steps, LUS_DENSITY_HISTORY, cap) steps (oldest first) e in errs and the
  * mean over particles of max(0, rho_i/rho0 - 1) in errs_mean (either may be
- * NULL); *nerrs receives their count.  In learned mode the values are the
- * learned solution's (before a rerun). */
+ * NULL); *nerrs receives their count.  In learned mode a step that reran
+ * records the rerun's (plain) solution, any other the learned one. */
 int lus_pbf_learned_stats(LusGpu* ctx, long long* steps, long long* reruns, float* errs,
                           float* errs_mean, int cap, int* nerrs);
 /* Particles of the last step whose neighbour list could not be stored (more

@@ -467,9 +467,12 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         LUS_HIP(lus::launch_pbf_restore_old(P.tmp, P.key, P.cs, P.x, P.n, P.x2, P.flag, s));
         LUS_HIP(mark(3));
         if (int r = plain_iterations(true, P.flag)) return r;
-        LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp, nullptr,
+        LUS_HIP(lus::launch_pbf_xsph(P.xa, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp, P.maxd,
                                      P.flag, tx, s));
         LUS_HIP(mark(6));
+        LUS_HIP(lus::launch_pbf_decide(P.maxd, nred, P.n, P.bound, 2, P.flag, P.counters, P.hist,
+                                       LUS_DENSITY_HISTORY, s));
+        LUS_HIP(mark(10));
     } else {
         if (rec) {
             LUS_HIP(lus::launch_pbf_features(P.key2, P.cs, P.n, k, P.masks, P.xp, P.x2, nullptr, nullptr,

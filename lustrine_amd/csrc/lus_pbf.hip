@@ -1639,7 +1639,9 @@ hipError_t launch_pbf_record_target(const float4* xa, const float4* xp, int n, i
 
 // One block: reduce the XSPH pass's block maxima and block sums, record
 // e = max(0, max C) in hist[step % cap] and mean max(0, C) in
-// hist[cap + step % cap], decide the rerun.
+// hist[cap + step % cap], decide the rerun.  learned: 0 a plain step (record
+// only), 1 a learned attempt (record, set *flag when e > bound), 2 after the
+// gated rerun (when *flag is set, the rerun's e replaces the step's entry).
 __global__ __launch_bounds__(kBlock) void k_pbf_decide(const int* __restrict__ maxd, int nblk, int n,
                                                        float bound, int learned, int* __restrict__ flag,
                                                        int* __restrict__ counters, float* __restrict__ hist,
@@ -1664,6 +1666,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_decide(const int* __restrict__ m
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (learned == 2 && *flag == 0) return;  // no rerun: the attempt's entry stands
         int b = wmax[0];
         float tot = wsum[0];
 #pragma unroll
@@ -1674,6 +1677,13 @@ __global__ __launch_bounds__(kBlock) void k_pbf_decide(const int* __restrict__ m
         const int bits = b >= 0 ? b : (b ^ 0x7FFFFFFF);
         float c = __int_as_float(bits);
         const float e = c > 0.0f ? c : (c == c ? 0.0f : c);  // max(0, C), NaN kept
+        if (learned == 2) {
+            // the rerun's final state replaces the entry of this step
+            const unsigned last = (unsigned)counters[0] - 1u;
+            hist[last % (unsigned)cap] = e;
+            hist[cap + last % (unsigned)cap] = tot / (float)n;
+            return;
+        }
         const int fb = (learned && !(e <= bound)) ? 1 : 0;
         const unsigned step = (unsigned)counters[0];
         hist[step % (unsigned)cap] = e;

@@ -55,7 +55,10 @@ def test_adapter_simulate_pbf_matches_oracle(gpu_mod, oracle, tmp_path):
     cfg = scenes.small(8, 8, 8, box=box, gravity=(0.0, -10.0, 0.0), c_xsph=0.1)
     sand = cfg.positions()
     p = oracle.pbf_params(cfg)
-    p.rest_density = 24.0  # the stand-in keeps the Simulation default (src/Simulation.hpp:157)
+    # the stand-in keeps the Simulation defaults (src/Simulation.hpp:157-168)
+    p.rest_density = 24.0
+    p.epsilon = 10.0
+    p.scorr_k = 1.0
     x, v = sand.copy(), np.zeros_like(sand)
     ids = np.arange(sand.shape[0], dtype=np.int32)
     for _ in range(2):

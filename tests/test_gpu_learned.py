@@ -162,14 +162,20 @@ def test_rerun_gives_plain_bits(gpu_mod, graph):
     net = _net(M.random_params(np.random.default_rng(9)), 0.05)
     with gpu_mod.Gpu(0) as g:
         g.pbf_init(gpu_mod.pbf_params(cfg, exact=True), x0)
+        g.pbf_set_mode(learned=False, check=True)
+        g.pbf_step(np.float32(cfg.dt), steps)
+        _, _, perrs, perrm = g.pbf_learned_stats(with_mean=True)
+        g.pbf_init(gpu_mod.pbf_params(cfg, exact=True), x0)
         g.pbf_set_network(net)
         g.pbf_set_mode(learned=True, learned_iterations=1, bound=-1.0)
         g.pbf_set_graph(graph)
         g.pbf_step(np.float32(cfg.dt), steps)
         lx, lv, _ = g.pbf_download(order=1)
-        n, reruns, errs = g.pbf_learned_stats()
+        n, reruns, errs, errm = g.pbf_learned_stats(with_mean=True)
     assert n == steps and reruns == steps and errs.shape == (steps,)
     assert np.array_equal(lx, px) and np.array_equal(lv, pv)
+    # every step reran: the recorded errors are the plain step's
+    assert np.array_equal(errs, perrs) and np.array_equal(errm, perrm)
 
 
 def test_nan_network_falls_back(gpu_mod):

@@ -114,5 +114,6 @@ def test_solids_never_move_and_fp64_agrees(oracle):
     vs[i32] = v32
     assert not vs[solid].any()
     assert np.array_equal(p32, ph[i32])                          # phases ride along
-    assert np.abs(a32 - a64).max() < 1e-4
+    # FP32 vs FP64 over four steps: within 1e-5 of the largest coordinate
+    assert np.abs(a32 - a64).max() < 1e-5 * np.abs(a64).max()
     assert (a32[~solid] >= 0.5).all() and (a32[~solid] <= 23.5).all()

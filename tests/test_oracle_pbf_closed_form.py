@@ -101,11 +101,12 @@ def closed_form_step(x, v, dt, cfg, rho0):
 
 # ------------------------------------------------------------------ scenes
 def _cfg(box=(12, 12, 12)):
-    # the dam-break constants (h = 2, f = 1/2, m = 5, eps = 10, s_corr
-    # k = 1, n = 4, dq = 0.5), no gravity, one iteration, a large XSPH factor so
-    # that its term is well above the rounding of v = (x* - x) / dt
+    # h = 2, f = 1/2, m = 5 and the reference's Simulation defaults eps = 10,
+    # s_corr k = 1, n = 4, dq = 0.5 (large terms, so each is far above the
+    # tolerance); no gravity, one iteration, a large XSPH factor so that its
+    # term is well above the rounding of v = (x* - x) / dt
     return scenes.small(2, 2, 2, box=box, gravity=(0.0, 0.0, 0.0), c_xsph=0.1, iterations=1,
-                        dt=1.0 / 64.0)
+                        dt=1.0 / 64.0, epsilon=10.0, scorr_k=1.0)
 
 
 def _pair():
