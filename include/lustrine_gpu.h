@@ -174,6 +174,12 @@ int lus_pbf_set_graph(LusGpu* ctx, int on);
  * loads.  Same results bit for bit.  No reference counterpart (launch
  * configuration). */
 int lus_pbf_set_tiling(LusGpu* ctx, int tiled, int zt);
+/* List build of the flat passes (the first lambda pass of a step): staged = 1
+ * (default) loads each wave's candidate ranges of a stencil row once into LDS
+ * and tests them there; 0 gathers every candidate per lane from global memory.
+ * Same results bit for bit.  No reference counterpart (launch configuration;
+ * the scan replaced is src/neighbors/Neighbors.cpp:306-361). */
+int lus_pbf_set_build(LusGpu* ctx, int staged);
 /* Threads per block (64, 128 or 256; default 256) of the per-slot neighbour
  * passes.  Same results.  No reference counterpart (launch configuration). */
 int lus_pbf_set_launch(LusGpu* ctx, int block);

@@ -204,6 +204,7 @@ struct PbfState {
     int lcap = 0, lcap_x = 0;  // staged float4 per array (XSPH stages two arrays)
     int nred = 0;          // block-reduction entries the last step wrote
     int fblock = lus::kBlock;  // threads per block of the flat neighbour passes
+    bool wsbuild = true;       // flat list build with wave-staged candidates
 };
 
 // the network of the learned step (device copies), kept across scene inits
@@ -385,6 +386,7 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         tx.lcap = P.lcap_x;
     }
     ta.block = tx.block = P.fblock;
+    ta.stage = P.wsbuild ? 1 : 0;
     // block-reduction entries the passes write
     const int nred = ta.tiles ? P.tgrid : lus::grid_for(P.n, P.fblock);
     const int K = P.p.iterations;
@@ -1112,6 +1114,15 @@ int lus_pbf_set_launch(LusGpu* c, int block) {
     return LUS_OK;
 }
 
+int lus_pbf_set_build(LusGpu* c, int staged) {
+    if (!c) return set_error(LUS_EINVAL, "lus_pbf_set_build: NULL context");
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    pbf_drop_graph(c->pbf);
+    c->pbf.wsbuild = staged != 0;
+    return LUS_OK;
+}
+
 int lus_pbf_set_tiling(LusGpu* c, int tiled, int zt) {
     if (!c) return set_error(LUS_EINVAL, "lus_pbf_set_tiling: NULL context");
     if (tiled && (zt < 1 || zt > lus::kTileZMax))
@@ -1253,6 +1264,7 @@ int lus_pbf_time_kernels(LusGpu* c, int reps, float* us) {
     // the last step's tiles still describe the current cell order
     lus::TileArgs ta, tx;
     ta.block = tx.block = P.fblock;
+    ta.stage = P.wsbuild ? 1 : 0;
     if (P.tiled) {
         ta.tiles = tx.tiles = P.tl;
         ta.ntiles = tx.ntiles = P.toff + lus::tile_chunks(P.k, P.zt);

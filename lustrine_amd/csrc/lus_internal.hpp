@@ -147,6 +147,7 @@ struct TileArgs {
     int grid = 0;                 // blocks of kTileT threads, a multiple of 8
     int lcap = 0;                 // staged float4 per staged array
     int block = kBlock;           // threads per block of a flat launch (64, 128 or 256)
+    int stage = 1;                // flat list build: wave-staged candidates (0: per-lane gathers)
 };
 int tile_chunks(const PbfK& k, int zt);
 // tcount: tile_chunks + 1 ints (last one 0); toff: the same; scan_tiles:

@@ -159,13 +159,68 @@ __device__ __forceinline__ int4* nbr_list(unsigned long long* nb, int n, int i) 
     return lists + (size_t)(i >> 6) * (kNbrGroups * 64) + (i & 63);
 }
 
+// Appends the neighbours of slot i to its list while a build visits them in
+// ascending slot order: entries relative to the first neighbour of their
+// layer (stencil rows 0-2, 3-5, 6-8), eight 16-bit entries per int4 group,
+// one 16-byte store per completed group (lst: group 0 of slot i's list, group
+// g 64 int4 further); entries past kNbrCap are counted, not stored.
+struct ListWriter {
+    int c = 0;
+    int lb = -1;  // first neighbour of the current layer (-1: none yet)
+    int b0, b1;
+    // min(n0, 255) | min(n01, 255) << 8, bit 16: an entry does not fit 16 bits
+    int meta = 0;
+    unsigned int q0 = 0u, q1 = 0u, q2 = 0u, q3 = 0u;  // the group being filled, entry 0 lowest
+
+    __device__ __forceinline__ explicit ListWriter(int i) : b0(i), b1(i) {}
+    __device__ __forceinline__ void push(unsigned int e) {
+        q0 = (q0 >> 16) | (q1 << 16);
+        q1 = (q1 >> 16) | (q2 << 16);
+        q2 = (q2 >> 16) | (q3 << 16);
+        q3 = (q3 >> 16) | (e << 16);
+    }
+    // at the top of stencil row r: close layer 0 (r = 3) or layer 1 (r = 6)
+    // (selects, not branches: branches on r let the compiler keep b0 and b1
+    // in a stack slot indexed by the layer)
+    __device__ __forceinline__ void row(int r, int i) {
+        const int first = lb < 0 ? i : lb;
+        const int cc = c < 255 ? c : 255;
+        meta |= r == 3 ? cc : (r == 6 ? cc << 8 : 0);
+        b0 = r == 3 ? first : b0;
+        b1 = r == 6 ? first : b1;
+        lb = (r == 3 || r == 6) ? -1 : lb;
+    }
+    __device__ __forceinline__ void emit(int j, int4* __restrict__ lst) {
+        lb = lb < 0 ? j : lb;
+        const unsigned int e = (unsigned int)(j - lb);
+        meta |= e > 0xFFFFu ? 0x10000 : 0;
+        push(e & 0xFFFFu);
+        ++c;
+        if ((c & 7) == 0 && c <= kNbrCap)
+            lst[(size_t)((c >> 3) - 1) * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
+    }
+    // the last, partial group (zero entries in the unused places: they decode
+    // to b2, a particle's slot, loaded by a walk but not visited) and the
+    // header hdr[i]
+    __device__ __forceinline__ void finish(int i, int4* __restrict__ hdr, int4* __restrict__ lst) {
+        const int b2 = lb < 0 ? i : lb;
+        const int rem = c & 7;
+        if (rem != 0 && c < kNbrCap) {
+            for (int u = rem; u < 8; ++u) push(0u);
+            lst[(size_t)(c >> 3) * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
+        }
+        // c <= kNbrCap: n0 and n01 are below 255
+        const int w = (c > kNbrCap || (meta & 0x10000)) ? kNbrRetest : (c | (meta & 0xFFFF) << 16);
+        hdr[i] = make_int4(b0, b1, b2, w);
+    }
+};
+
 // Test the 27-cell candidates of slot i (positions xp) row by row, in
 // ascending slot order -- a row of at most 64 slots through a 64-bit hit mask
 // (32-bit mask arithmetic when no lane of the wave has a hit above bit 31), a
 // longer row slot by slot -- and call f(j, cur[j], aux[j]) on every hit while
-// appending it to i's list, eight 16-bit entries at a time (one 16-byte store
-// per group, so the lanes of a wave write whole segments); entries past
-// kNbrCap are counted, not stored.  Writes the header hdr[i].
+// appending it to i's list (ListWriter; the lanes of a wave write whole
+// segments of a group).  Writes the header hdr[i].
 template <class F>
 __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, const int* __restrict__ cs,
                                            const float4* __restrict__ xp, const PbfK& k, int4* __restrict__ hdr,
@@ -179,39 +234,14 @@ __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, 
     const int z0 = cz > 0 ? cz - 1 : 0;
     const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
     const float4 pi = xp[i];
-    int c = 0;
-    int lb = -1;                     // first neighbour of the current layer (-1: none yet)
-    int b0 = i, b1 = i;
-    // min(n0, 255) | min(n01, 255) << 8, bit 16: an entry does not fit 16 bits
-    int meta = 0;
-    unsigned int q0 = 0u, q1 = 0u, q2 = 0u, q3 = 0u;  // the group being filled, entry 0 lowest
-    auto push = [&](unsigned int e) {
-        q0 = (q0 >> 16) | (q1 << 16);
-        q1 = (q1 >> 16) | (q2 << 16);
-        q2 = (q2 >> 16) | (q3 << 16);
-        q3 = (q3 >> 16) | (e << 16);
-    };
+    ListWriter lw(i);
     auto emit = [&](int j, const float4 pj, const float4 aj) {
-        lb = lb < 0 ? j : lb;
-        const unsigned int e = (unsigned int)(j - lb);
-        meta |= e > 0xFFFFu ? 0x10000 : 0;
-        push(e & 0xFFFFu);
-        ++c;
-        if ((c & 7) == 0 && c <= kNbrCap)
-            lst[(size_t)((c >> 3) - 1) * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
+        lw.emit(j, lst);
         f(j, pj, aj);
     };
 #pragma unroll 1
     for (int r = 0; r < 9; ++r) {
-        if (r == 3) {
-            meta |= c < 255 ? c : 255;
-            b0 = lb < 0 ? i : lb;
-            lb = -1;
-        } else if (r == 6) {
-            meta |= (c < 255 ? c : 255) << 8;
-            b1 = lb < 0 ? i : lb;
-            lb = -1;
-        }
+        lw.row(r, i);
         const int y = cy + r / 3 - 1;
         const int x = cx + r % 3 - 1;
         if (y < 0 || y >= k.gY || x < 0 || x >= k.gX) continue;
@@ -241,17 +271,7 @@ __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, 
         else
             visit_row<unsigned int>((unsigned int)mm, j0, 0, cur, aux, emit);
     }
-    const int b2 = lb < 0 ? i : lb;
-    // the last, partial group: zero entries in the unused places (they decode
-    // to b2, a particle's slot, loaded by a walk but not visited)
-    const int rem = c & 7;
-    if (rem != 0 && c < kNbrCap) {
-        for (int u = rem; u < 8; ++u) push(0u);
-        lst[(size_t)(c >> 3) * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
-    }
-    // c <= kNbrCap: n0 and n01 are below 255
-    const int w = (c > kNbrCap || (meta & 0x10000)) ? kNbrRetest : (c | (meta & 0xFFFF) << 16);
-    hdr[i] = make_int4(b0, b1, b2, w);
+    lw.finish(i, hdr, lst);
 }
 
 // Walk forms of visit_list: kWalkWide, one group of eight entries per step
@@ -780,19 +800,27 @@ __device__ __forceinline__ float spiky_unit(float r2, float mf, float hf) {
     return g * (g * rinv);
 }
 
-// lambda pass: density constraint and its gradient norm (PBF eq. 1, 8, 11)
-template <bool kFast, class Visit>
-__device__ __forceinline__ float lambda_body(int i, const float4* __restrict__ xa, float4* __restrict__ xl,
-                                             const PbfK& k, Visit&& visit) {
-    const float4 pi = xa[i];
+// lambda pass: density constraint and its gradient norm (PBF eq. 1, 8, 11),
+// summed over the neighbours pj of pi (add), then C_i and lambda_i (finish)
+template <bool kFast>
+struct LambdaAcc {
     float rho = 0.0f, a = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    if constexpr (kFast) {
-        // SPf leaves the sums: a and |grad|^2 scale by SPf^2 (f_mr0sq_spf2)
-        const float mf = vreg(-k.f), hf = vreg(k.hf), mf2 = vreg(-k.f2), hf2 = vreg(k.hf2);
-        visit([&](int, const float4 pj, const float4) {
-            const float dx = pi.x - pj.x;
-            const float dy = pi.y - pj.y;
-            const float dz = pi.z - pj.z;
+    float mf, hf, mf2, hf2;  // fast mode: kernel constants in vector registers
+
+    __device__ __forceinline__ explicit LambdaAcc(const PbfK& k) {
+        if constexpr (kFast) {
+            mf = vreg(-k.f);
+            hf = vreg(k.hf);
+            mf2 = vreg(-k.f2);
+            hf2 = vreg(k.hf2);
+        }
+    }
+    __device__ __forceinline__ void add(const float4 pi, const float4 pj, const PbfK& k) {
+        const float dx = pi.x - pj.x;
+        const float dy = pi.y - pj.y;
+        const float dz = pi.z - pj.z;
+        if constexpr (kFast) {
+            // SPf leaves the sums: a and |grad|^2 scale by SPf^2 (f_mr0sq_spf2)
             const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
             if (r2 <= k.h2) {
                 const float t = __builtin_fmaf(mf2, r2, hf2);
@@ -805,35 +833,38 @@ __device__ __forceinline__ float lambda_body(int i, const float4* __restrict__ x
                     gz = __builtin_fmaf(s, dz, gz);
                 }
             }
-        });
-        const float Ci = rho * k.c_rho - 1.0f;
-        const float gg = gx * gx + gy * gy + gz * gz;
-        const float den = k.f_mr0sq_spf2 * (a + gg) + k.eps;
-        xl[i] = make_float4(pi.x, pi.y, pi.z, -Ci / den);
-        return Ci;
-    }
-    visit([&](int, const float4 pj, const float4) {
-        const float dx = pi.x - pj.x;
-        const float dy = pi.y - pj.y;
-        const float dz = pi.z - pj.z;
-        const float r2 = dist2<kFast>(dx, dy, dz);
-        if (r2 <= k.h2) {
-            const float t = poly_t<kFast>(r2, k);
-            rho = acc<kFast>(rho, t * t, t);
-            if (r2 > 0.0f) {
-                const float s = spiky_s<kFast>(r2, k);
-                a = acc<kFast>(a, s * s, r2);
-                gx = acc<kFast>(gx, s, dx);
-                gy = acc<kFast>(gy, s, dy);
-                gz = acc<kFast>(gz, s, dz);
+        } else {
+            const float r2 = dist2<kFast>(dx, dy, dz);
+            if (r2 <= k.h2) {
+                const float t = poly_t<kFast>(r2, k);
+                rho = acc<kFast>(rho, t * t, t);
+                if (r2 > 0.0f) {
+                    const float s = spiky_s<kFast>(r2, k);
+                    a = acc<kFast>(a, s * s, r2);
+                    gx = acc<kFast>(gx, s, dx);
+                    gy = acc<kFast>(gy, s, dy);
+                    gz = acc<kFast>(gz, s, dz);
+                }
             }
         }
-    });
-    const float Ci = rho * k.c_rho - 1.0f;
-    const float gg = gx * gx + gy * gy + gz * gz;
-    const float den = k.mr0sq * (a + gg) + k.eps;
-    xl[i] = make_float4(pi.x, pi.y, pi.z, -Ci / den);
-    return Ci;
+    }
+    // C_i; writes (x_i, lambda_i) to *out
+    __device__ __forceinline__ float finish(const float4 pi, float4* __restrict__ out, const PbfK& k) const {
+        const float Ci = rho * k.c_rho - 1.0f;
+        const float gg = gx * gx + gy * gy + gz * gz;
+        const float den = (kFast ? k.f_mr0sq_spf2 : k.mr0sq) * (a + gg) + k.eps;
+        *out = make_float4(pi.x, pi.y, pi.z, -Ci / den);
+        return Ci;
+    }
+};
+
+template <bool kFast, class Visit>
+__device__ __forceinline__ float lambda_body(int i, const float4* __restrict__ xa, float4* __restrict__ xl,
+                                             const PbfK& k, Visit&& visit) {
+    const float4 pi = xa[i];
+    LambdaAcc<kFast> s(k);
+    visit([&](int, const float4 pj, const float4) { s.add(pi, pj, k); });
+    return s.finish(pi, xl + i, k);
 }
 
 template <bool kBuild, bool kFast>
@@ -852,6 +883,227 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict_
         Ci = lambda_body<kFast>(i, xa, xl, k, [&](auto&& f) {
             for_each_neighbor<kBuild, kWalkWide>(i, n, key2, cs, masks, xp, xa, nullptr, k, f);
         });
+    block_max_store(Ci, maxc);
+}
+
+// Wave-staged list build: the first lambda pass of a step, testing the
+// candidates at x* (the pass's own input).  The 64 slots of a wave lie in a
+// few (y, x) cell columns; within a column the slots ascend with cz and so do
+// their stencil windows, so the union of the wave's windows in stencil row r
+// is one contiguous slot range per column.  The wave loads those ranges once,
+// coalesced, into LDS and its lanes test their candidates and read
+// their hits there instead of gathering them one lane at a time through the
+// texture units.  The staged copy holds x, y, z of a slot in three consecutive
+// floats, so one address reaches four candidates through the instructions'
+// offset fields.  A row whose ranges exceed kStageCap slots or kStageSegs
+// columns is tested from global memory as build_list does.  Same candidates,
+// same order, same arithmetic as build_list + lambda_body: the same bits.
+constexpr int kStageCap = 256;
+constexpr int kStageSegs = 4;
+
+// LDS float, addressed with 32-bit offsets (a generic pointer into LDS would
+// be indexed with 64-bit arithmetic)
+typedef __attribute__((address_space(3))) float lds_float;
+
+// x, y, z of staged slot q
+__device__ __forceinline__ float4 staged_at(const lds_float* __restrict__ S, int q) {
+    const lds_float* __restrict__ p = S + __umul24((unsigned)q, 3u);  // q < kStageCap
+    return make_float4(p[0], p[1], p[2], 0.0f);
+}
+
+// Visit the set bits of a row mask in ascending order from the staged copy:
+// f(j0 + t, S[lo + t]); the next hit is read before f runs.
+template <typename M, class F>
+__device__ __forceinline__ void visit_row_staged(M mm, int j0, int lo, const lds_float* __restrict__ S, F& f) {
+    if (!mm) return;
+    int t = low_bit(mm);
+    mm &= mm - (M)1;
+    float4 pa = staged_at(S, lo + t);
+    while (mm) {
+        const int tn = low_bit(mm);
+        mm &= mm - (M)1;
+        const float4 pb = staged_at(S, lo + tn);
+        f(j0 + t, pa);
+        t = tn;
+        pa = pb;
+    }
+    f(j0 + t, pa);
+}
+
+// Hit mask of the L (<= bits of M) staged candidates S[lo + t] of pi: four
+// tests make a nibble of inline-constant bits, shifted into the mask once
+template <typename M>
+__device__ __forceinline__ M test_row_staged(const float4 pi, int lo, int L, const lds_float* __restrict__ S,
+                                             const PbfK& k) {
+    M mm = (M)0;
+    int t = 0;
+    const lds_float* __restrict__ b = S + 3 * lo;
+    for (; t + 4 <= L; t += 4, b += 12) {
+        const float4 c0 = make_float4(b[0], b[1], b[2], 0.0f);
+        const float4 c1 = make_float4(b[3], b[4], b[5], 0.0f);
+        const float4 c2 = make_float4(b[6], b[7], b[8], 0.0f);
+        const float4 c3 = make_float4(b[9], b[10], b[11], 0.0f);
+        const unsigned int nib = (within_h(pi, c0, k) ? 1u : 0u) | (within_h(pi, c1, k) ? 2u : 0u) |
+                                 (within_h(pi, c2, k) ? 4u : 0u) | (within_h(pi, c3, k) ? 8u : 0u);
+        mm |= (M)nib << t;
+    }
+    for (; t < L; ++t, b += 3) {
+        const float4 c = make_float4(b[0], b[1], b[2], 0.0f);
+        mm |= (M)(within_h(pi, c, k) ? 1u : 0u) << t;
+    }
+    return mm;
+}
+
+// LDS accesses of one wave complete, and are seen by all its lanes, before
+// the wave goes on (staging -> tests, tests -> next row's staging)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool kFast>
+__global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __restrict__ xp,
+                                                                 const int* __restrict__ key2,
+                                                                 const int* __restrict__ cs, int n, PbfK k,
+                                                                 float4* __restrict__ xl,
+                                                                 int* __restrict__ maxc,
+                                                                 unsigned long long* __restrict__ nb,
+                                                                 const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;  // uniform: before any barrier
+    __shared__ float stage[kBlock / 64][3 * kStageCap];
+    const int i = xcd_index();
+    const int lane = (int)(threadIdx.x & 63);
+    lds_float* __restrict__ S = (lds_float*)stage[threadIdx.x >> 6];
+    // every lane of the wave takes part in the staging; a lane past n
+    // (i >= n, the last wave only) has empty windows and stores nothing
+    const bool act = i < n;
+    const int ii = act ? i : n - 1;
+    const int key = key2[ii] & ~kOvf;
+    const int cy = key / k.gXZ;
+    const int rm = key - cy * k.gXZ;
+    const int cx = rm / k.gZ;
+    const int cz = rm - cx * k.gZ;
+    const int z0 = cz > 0 ? cz - 1 : 0;
+    const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
+    const float4 pi = xp[ii];
+    int4* __restrict__ lst = nbr_list(nb, n, ii);
+    LambdaAcc<kFast> acc(k);
+    ListWriter lw(ii);
+    auto hit = [&](int j, const float4 pj) {
+        lw.emit(j, lst);
+        acc.add(pi, pj, k);
+    };
+    const unsigned long long below = (1ull << lane) - 1ull;  // lanes before this one
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        lw.row(r, ii);
+        const int y = cy + r / 3 - 1;
+        const int x = cx + r % 3 - 1;
+        const bool valid = act && y >= 0 && y < k.gY && x >= 0 && x < k.gX;
+        int j0 = 0, j1 = 0;
+        if (valid) {
+            const int row = y * k.gXZ + x * k.gZ;
+            j0 = cs[row + z0];
+            j1 = cs[row + z1 + 1];
+        }
+        const int L = j1 - j0;
+        // runs of lanes whose row r lies in one column (-1: no row)
+        const int col = valid ? y * k.gX + x : -1;
+        const int pcol = __shfl_up(col, 1, 64);
+        const int ncol = __shfl_down(col, 1, 64);
+        const unsigned long long fm = __ballot(valid && (lane == 0 || pcol != col));
+        const unsigned long long lm = __ballot(valid && (lane == 63 || ncol != col));
+        const int nseg = __popcll(fm);
+        if (nseg == 0) continue;  // uniform: no lane has row r
+        // the runs' slot ranges [start_s, end_s), packed at LDS offsets off_s
+        int s_start[kStageSegs], s_off[kStageSegs + 1];
+        int tot = 0;
+        {
+            unsigned long long f = fm, l = lm;
+#pragma unroll
+            for (int s = 0; s < kStageSegs; ++s) {
+                s_off[s] = tot;
+                s_start[s] = 0;
+                if (s < nseg) {
+                    const int lf = __ffsll((long long)f) - 1, ll = __ffsll((long long)l) - 1;
+                    f &= f - 1ull;
+                    l &= l - 1ull;
+                    const int a = __builtin_amdgcn_readlane(j0, lf);
+                    const int b = __builtin_amdgcn_readlane(j1, ll);
+                    s_start[s] = a;
+                    tot += b - a;
+                }
+            }
+            s_off[kStageSegs] = tot;
+        }
+        const bool staged = nseg <= kStageSegs && tot <= kStageCap;  // uniform
+        if (staged) {
+            for (int q = lane; q < tot; q += 64) {
+                int j = q - s_off[0] + s_start[0];
+#pragma unroll
+                for (int s = 1; s < kStageSegs; ++s)
+                    if (q >= s_off[s]) j = q - s_off[s] + s_start[s];
+                const float4 p = at4(xp, j);
+                S[3 * q] = p.x;
+                S[3 * q + 1] = p.y;
+                S[3 * q + 2] = p.z;
+            }
+            wave_lds_sync();
+            // this lane's run and the LDS offset of its window
+            const int seg = __popcll(fm & (below | (1ull << lane))) - 1;
+            int lo = j0 - s_start[0] + s_off[0];
+#pragma unroll
+            for (int s = 1; s < kStageSegs; ++s)
+                if (seg == s) lo = j0 - s_start[s] + s_off[s];
+            if (L > 64) {
+                for (int t = 0; t < L; ++t) {
+                    const float4 c = staged_at(S, lo + t);
+                    if (within_h(pi, c, k)) hit(j0 + t, c);
+                }
+            } else if (__all(L <= 32)) {
+                const unsigned int mm = test_row_staged<unsigned int>(pi, lo, L, S, k);
+                visit_row_staged<unsigned int>(mm, j0, lo, S, hit);
+            } else {
+                const unsigned long long mm = test_row_staged<unsigned long long>(pi, lo, L, S, k);
+                if (__any((int)((mm >> 32) != 0ull)))
+                    visit_row_staged<unsigned long long>(mm, j0, lo, S, hit);
+                else
+                    visit_row_staged<unsigned int>((unsigned int)mm, j0, lo, S, hit);
+            }
+            wave_lds_sync();  // the next row's staging overwrites this row's
+        } else if (valid) {
+            if (L > 64) {
+                for (int j = j0; j < j1; ++j) {
+                    const float4 c = at4(xp, j);
+                    if (within_h(pi, c, k)) hit(j, c);
+                }
+                continue;
+            }
+            unsigned long long mm = 0ull;
+            const float4* __restrict__ base = xp + j0;
+            int t = 0;
+            for (; t + 4 <= L; t += 4) {
+                const float3 c0 = xyz_at(base, t), c1 = xyz_at(base, t + 1), c2 = xyz_at(base, t + 2),
+                             c3 = xyz_at(base, t + 3);
+                mm |= within_h(pi, c0, k) ? (1ull << t) : 0ull;
+                mm |= within_h(pi, c1, k) ? (2ull << t) : 0ull;
+                mm |= within_h(pi, c2, k) ? (4ull << t) : 0ull;
+                mm |= within_h(pi, c3, k) ? (8ull << t) : 0ull;
+            }
+            for (; t < L; ++t) mm |= within_h(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
+            auto hit4 = [&](int j, const float4 pj, const float4) { hit(j, pj); };
+            if (__any((int)((mm >> 32) != 0ull)))
+                visit_row<unsigned long long>(mm, j0, 0, xp, nullptr, hit4);
+            else
+                visit_row<unsigned int>((unsigned int)mm, j0, 0, xp, nullptr, hit4);
+        }
+    }
+    float Ci = -1e30f;
+    if (act) {
+        Ci = acc.finish(pi, xl + i, k);
+        lw.finish(i, nbr_hdr(nb), lst);
+    }
     block_max_store(Ci, maxc);
 }
 
@@ -907,6 +1159,13 @@ hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, 
         return hipGetLastError();
     }
     const dim3 g(grid_for(n, ta.block)), b(ta.block);
+    if (build && ta.stage && xa == xp && n > 0) {
+        if (k.exact)
+            launch_kernel((k_pbf_lambda_build_ws<false>), g, b, 0, s, xp, key2, cs, n, k, xl, maxc, masks, gate);
+        else
+            launch_kernel((k_pbf_lambda_build_ws<true>), g, b, 0, s, xp, key2, cs, n, k, xl, maxc, masks, gate);
+        return hipGetLastError();
+    }
     if (build && k.exact)
         launch_kernel((k_pbf_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, maxc, masks, xp, gate);
     else if (build)

@@ -102,6 +102,7 @@ _SIGS = {
     "lus_pbf_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
     "lus_pbf_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "lus_pbf_set_build": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_set_launch": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_step_latency": (C.c_int, [C.c_void_p, C.c_float, C.c_int, _f]),
     "lus_pbf_download": (C.c_int, [C.c_void_p, C.c_int, _f, _f, _i]),
@@ -398,6 +399,11 @@ class Gpu:
 
     def pbf_set_tiling(self, tiled=True, zt=16):
         _check(self._lib.lus_pbf_set_tiling(self._h, int(bool(tiled)), zt), "lus_pbf_set_tiling")
+
+    def pbf_set_build(self, staged=True):
+        """staged: the list build tests wave-staged candidates in LDS; else
+        per-lane gathers (same results)."""
+        _check(self._lib.lus_pbf_set_build(self._h, int(bool(staged))), "lus_pbf_set_build")
 
     def pbf_step_latency(self, dt, steps):
         ms = np.zeros(steps, np.float32)

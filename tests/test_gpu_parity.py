@@ -460,3 +460,29 @@ def test_pbf_tail_one_step_4m(gpu_mod, oracle):
     # final positions stay within a cell or so of them: the cell keys of the
     # committed positions are nearly sorted
     assert np.mean(np.diff(key) >= 0) > 0.95
+
+
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("scene", ["small", "sheet", "dense", "cfg0", "cfg2"])
+def test_pbf_staged_build_matches_gathers(gpu_mod, scene, exact):
+    # the wave-staged list build (lus_pbf_set_build 1, the default) against
+    # the per-lane gather build: the same bits after three steps.  "sheet" (a
+    # 2-particle-thick layer: a wave spans dozens of cell columns) and
+    # "dense" (8x rest density: rows over 64 slots, ranges over the staging
+    # capacity) take the gather fallback for some rows
+    cfg = {"small": lambda: scenes.small(12, 12, 12, box=(24, 24, 24)),
+           "sheet": lambda: scenes.small(40, 2, 40, box=(48, 8, 48), seed=4),
+           "dense": lambda: scenes.small(8, 8, 8, box=(24, 24, 24), origin=(8.0, 8.0, 8.0)),
+           "cfg0": scenes.cfg0, "cfg2": scenes.cfg2}[scene]()
+    x0 = cfg.positions()
+    if scene == "dense":
+        x0 = scenes.jittered_lattice(8, 8, 8, cell=0.5, origin=(8.0, 8.0, 8.0), amplitude=0.01, seed=3)
+    out = []
+    with gpu_mod.Gpu(0) as g:
+        for staged in (False, True):
+            g.pbf_init(gpu_mod.pbf_params(cfg, exact=exact), x0)
+            g.pbf_set_build(staged)
+            g.pbf_step(np.float32(cfg.dt), 3)
+            out.append(g.pbf_download(order=0) + (g.pbf_last_max_constraint(), g.pbf_retest_count()))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
