@@ -167,6 +167,7 @@ __device__ __forceinline__ int4* nbr_list(unsigned long long* nb, int n, int i) 
 struct ListWriter {
     int c = 0;
     int lb = -1;  // first neighbour of the current layer (-1: none yet)
+    int jl = -1;  // last neighbour announced by row_hits in the current layer
     int b0, b1;
     // min(n0, 255) | min(n01, 255) << 8, bit 16: an entry does not fit 16 bits
     int meta = 0;
@@ -185,25 +186,47 @@ struct ListWriter {
     __device__ __forceinline__ void row(int r, int i) {
         const int first = lb < 0 ? i : lb;
         const int cc = c < 255 ? c : 255;
+        const bool close = r == 3 || r == 6;
         meta |= r == 3 ? cc : (r == 6 ? cc << 8 : 0);
+        meta |= (close && jl - first > 0xFFFF) ? 0x10000 : 0;
         b0 = r == 3 ? first : b0;
         b1 = r == 6 ? first : b1;
-        lb = (r == 3 || r == 6) ? -1 : lb;
+        lb = close ? -1 : lb;
+        jl = close ? -1 : jl;
     }
-    __device__ __forceinline__ void emit(int j, int4* __restrict__ lst) {
+    // A row whose hits run from jfirst to jlast, appended next with
+    // append_row: the layer's first neighbour is set here, and its entries
+    // (ascending within the layer) are checked against 16 bits once, through
+    // the layer's last one, when the layer closes.
+    __device__ __forceinline__ void row_hits(int jfirst, int jlast) {
+        lb = lb < 0 ? jfirst : lb;
+        jl = jlast;
+    }
+    __device__ __forceinline__ bool append_row(int j) {
+        push((unsigned int)(j - lb) & 0xFFFFu);
+        ++c;
+        return (c & 7) == 0 && c <= kNbrCap;
+    }
+    // append j; true when that completes a group to store (group(), number
+    // (c >> 3) - 1)
+    __device__ __forceinline__ bool append(int j) {
         lb = lb < 0 ? j : lb;
         const unsigned int e = (unsigned int)(j - lb);
         meta |= e > 0xFFFFu ? 0x10000 : 0;
         push(e & 0xFFFFu);
         ++c;
-        if ((c & 7) == 0 && c <= kNbrCap)
-            lst[(size_t)((c >> 3) - 1) * 64] = make_int4((int)q0, (int)q1, (int)q2, (int)q3);
+        return (c & 7) == 0 && c <= kNbrCap;
+    }
+    __device__ __forceinline__ int4 group() const { return make_int4((int)q0, (int)q1, (int)q2, (int)q3); }
+    __device__ __forceinline__ void emit(int j, int4* __restrict__ lst) {
+        if (append(j)) lst[(size_t)((c >> 3) - 1) * 64] = group();
     }
     // the last, partial group (zero entries in the unused places: they decode
     // to b2, a particle's slot, loaded by a walk but not visited) and the
     // header hdr[i]
     __device__ __forceinline__ void finish(int i, int4* __restrict__ hdr, int4* __restrict__ lst) {
         const int b2 = lb < 0 ? i : lb;
+        meta |= jl - b2 > 0xFFFF ? 0x10000 : 0;
         const int rem = c & 7;
         if (rem != 0 && c < kNbrCap) {
             for (int u = rem; u < 8; ++u) push(0u);
@@ -802,17 +825,22 @@ __device__ __forceinline__ float spiky_unit(float r2, float mf, float hf) {
 
 // lambda pass: density constraint and its gradient norm (PBF eq. 1, 8, 11),
 // summed over the neighbours pj of pi (add), then C_i and lambda_i (finish)
-template <bool kFast>
+template <bool kFast, bool kVreg = true>
 struct LambdaAcc {
     float rho = 0.0f, a = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    float mf, hf, mf2, hf2;  // fast mode: kernel constants in vector registers
+    float mf, hf, mf2, hf2;  // fast mode: kernel constants (kVreg: in vector registers)
 
     __device__ __forceinline__ explicit LambdaAcc(const PbfK& k) {
-        if constexpr (kFast) {
+        if constexpr (kFast && kVreg) {
             mf = vreg(-k.f);
             hf = vreg(k.hf);
             mf2 = vreg(-k.f2);
             hf2 = vreg(k.hf2);
+        } else if constexpr (kFast) {
+            mf = -k.f;
+            hf = k.hf;
+            mf2 = -k.f2;
+            hf2 = k.hf2;
         }
     }
     __device__ __forceinline__ void add(const float4 pi, const float4 pj, const PbfK& k) {
@@ -912,23 +940,18 @@ __device__ __forceinline__ float4 staged_at(const lds_float* __restrict__ S, int
 }
 
 // Visit the set bits of a row mask in ascending order from the staged copy:
-// f(j0 + t, S[lo + t]); the next hit is read before f runs.
+// f(j0 + t, S[lo + t]) (LDS latency is short: no read ahead)
 template <typename M, class F>
 __device__ __forceinline__ void visit_row_staged(M mm, int j0, int lo, const lds_float* __restrict__ S, F& f) {
-    if (!mm) return;
-    int t = low_bit(mm);
-    mm &= mm - (M)1;
-    float4 pa = staged_at(S, lo + t);
     while (mm) {
-        const int tn = low_bit(mm);
+        const int t = low_bit(mm);
         mm &= mm - (M)1;
-        const float4 pb = staged_at(S, lo + tn);
-        f(j0 + t, pa);
-        t = tn;
-        pa = pb;
+        f(j0 + t, staged_at(S, lo + t));
     }
-    f(j0 + t, pa);
 }
+
+__device__ __forceinline__ int high_bit(unsigned int m) { return 31 - __builtin_clz(m); }
+__device__ __forceinline__ int high_bit(unsigned long long m) { return 63 - __builtin_clzll(m); }
 
 // Hit mask of the L (<= bits of M) staged candidates S[lo + t] of pi: four
 // tests make a nibble of inline-constant bits, shifted into the mask once
@@ -988,10 +1011,15 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
     const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
     const float4 pi = xp[ii];
     int4* __restrict__ lst = nbr_list(nb, n, ii);
-    LambdaAcc<kFast> acc(k);
+    LambdaAcc<kFast, false> acc(k);
     ListWriter lw(ii);
     auto hit = [&](int j, const float4 pj) {
         lw.emit(j, lst);
+        acc.add(pi, pj, k);
+    };
+    // a hit of a row announced with row_hits
+    auto hit_row = [&](int j, const float4 pj) {
+        if (lw.append_row(j)) lst[(size_t)((lw.c >> 3) - 1) * 64] = lw.group();
         acc.add(pi, pj, k);
     };
     const unsigned long long below = (1ull << lane) - 1ull;  // lanes before this one
@@ -1063,13 +1091,15 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
                 }
             } else if (__all(L <= 32)) {
                 const unsigned int mm = test_row_staged<unsigned int>(pi, lo, L, S, k);
-                visit_row_staged<unsigned int>(mm, j0, lo, S, hit);
+                if (mm) lw.row_hits(j0 + low_bit(mm), j0 + high_bit(mm));
+                visit_row_staged<unsigned int>(mm, j0, lo, S, hit_row);
             } else {
                 const unsigned long long mm = test_row_staged<unsigned long long>(pi, lo, L, S, k);
+                if (mm) lw.row_hits(j0 + low_bit(mm), j0 + high_bit(mm));
                 if (__any((int)((mm >> 32) != 0ull)))
-                    visit_row_staged<unsigned long long>(mm, j0, lo, S, hit);
+                    visit_row_staged<unsigned long long>(mm, j0, lo, S, hit_row);
                 else
-                    visit_row_staged<unsigned int>((unsigned int)mm, j0, lo, S, hit);
+                    visit_row_staged<unsigned int>((unsigned int)mm, j0, lo, S, hit_row);
             }
             wave_lds_sync();  // the next row's staging overwrites this row's
         } else if (valid) {
@@ -1086,10 +1116,9 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
             for (; t + 4 <= L; t += 4) {
                 const float3 c0 = xyz_at(base, t), c1 = xyz_at(base, t + 1), c2 = xyz_at(base, t + 2),
                              c3 = xyz_at(base, t + 3);
-                mm |= within_h(pi, c0, k) ? (1ull << t) : 0ull;
-                mm |= within_h(pi, c1, k) ? (2ull << t) : 0ull;
-                mm |= within_h(pi, c2, k) ? (4ull << t) : 0ull;
-                mm |= within_h(pi, c3, k) ? (8ull << t) : 0ull;
+                const unsigned int nib = (within_h(pi, c0, k) ? 1u : 0u) | (within_h(pi, c1, k) ? 2u : 0u) |
+                                         (within_h(pi, c2, k) ? 4u : 0u) | (within_h(pi, c3, k) ? 8u : 0u);
+                mm |= (unsigned long long)nib << t;
             }
             for (; t < L; ++t) mm |= within_h(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
             auto hit4 = [&](int j, const float4 pj, const float4) { hit(j, pj); };
