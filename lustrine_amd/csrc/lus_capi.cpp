@@ -311,6 +311,7 @@ lus::PbfK pbf_constants(const LusPbfParams& p) {
     k.f_spf_neg_inv_r0 = (float)((double)k.SPf * (double)k.neg_inv_r0);
     k.f_mr0sq_spf2 = (float)((double)k.mr0sq * (double)k.SPf * (double)k.SPf);
     k.n_scorr = p.scorr_n;
+    k.nph = 1;
     k.cW = p.c_xsph * P6;
     k.eps = p.epsilon;
     for (int a = 0; a < 3; ++a) {
@@ -395,14 +396,14 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         // mixed sand + fluid: lambda (fluid), correction (fluid / sand), XSPH (fluid)
         for (int it = 0; it < K; ++it) {
             LUS_HIP(lus::launch_mix_lambda(it == 0 ? P.xp : P.xa, P.key2, P.cs, P.n, k, P.xl, P.masks, P.xp,
-                                           it == 0 ? 1 : 0, s));
+                                           it == 0 ? 1 : 0, ta.stage, s));
             LUS_HIP(mark(it == 0 ? 7 : 4));
             LUS_HIP(lus::launch_mix_correct(P.xl, P.key2, P.cs, P.n, k, P.xa, P.x2, it == K - 1 ? P.v2 : nullptr,
                                             dt, P.masks, P.xp, s));
             LUS_HIP(mark(5));
         }
         if (K == 0) {
-            LUS_HIP(lus::launch_mix_lambda(P.xp, P.key2, P.cs, P.n, k, P.xl, P.masks, P.xp, 1, s));
+            LUS_HIP(lus::launch_mix_lambda(P.xp, P.key2, P.cs, P.n, k, P.xl, P.masks, P.xp, 1, ta.stage, s));
             LUS_HIP(lus::launch_pbf_velocity(P.xp, P.x2, P.n, dt, P.v2, s));
         }
         LUS_HIP(lus::launch_mix_xsph(K > 0 ? P.xa : P.xp, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp,
@@ -899,11 +900,26 @@ int lus_mix_init(LusGpu* c, const LusPbfParams* p, const LusMixParams* mp, int n
     int r = lus_pbf_init(c, p, n, pos, vel);
     if (r) return r;
     PbfState& P = c->pbf;
+    // cell columns split into fluid, sand and solid runs: three times the keys
+    const long long C3 = 3ll * p->gridX * p->gridY * p->gridZ;
+    if (C3 >= (1ll << 30)) return set_error(LUS_EINVAL, "lus_mix_init: grid too large (3 x %lld cells)", C3 / 3);
+    P.C = (int)C3;
+    P.k.nph = 3;
+    LUS_HIP(P.mem.alloc(&P.counts, P.C + 1));
+    LUS_HIP(P.mem.alloc(&P.cs, P.C + 1));
+    LUS_HIP(P.mem.alloc(&P.tiles, lus::scan_tile_count(P.C + 1) + 1));
+    LUS_HIP(hipMemset(P.counts, 0, sizeof(int) * (P.C + 1)));
     LUS_HIP(P.mem.alloc(&P.ph, n));
     LUS_HIP(P.mem.alloc(&P.ph2, n));
     LUS_HIP(hipMemcpy(P.ph, phase, sizeof(int) * n, hipMemcpyHostToDevice));
     LUS_HIP(hipDeviceSynchronize());
     P.k.mix_d = mp->diameter;
+    {
+        const double far = (double)mp->diameter * (1.0 + 0x1p-19);
+        float f2 = (float)(far * far);
+        if ((double)f2 < far * far) f2 = std::nextafter(f2, INFINITY);
+        P.k.mix_d2_far = f2;
+    }
     P.k.mix_collision = mp->collision;
     P.k.mix_friction = mp->friction;
     P.k.mix_mu_s = mp->mu_s;

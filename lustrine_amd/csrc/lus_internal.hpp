@@ -42,6 +42,10 @@ struct PbfK {
     float h2, hf, hf2, f, f2, SPf, c_rho, mr0sq, eps, inv_tq3, scorr_k, neg_inv_r0, cW, cs;
     float lo[3], hi[3], g[3];
     int gX, gY, gZ, gXZ, n_scorr;
+    // cell keys are ((y gX + x) nph + c) gZ + z: nph = 1 is get_cell_id's
+    // order; the mixed scene splits each cell column into phase runs (nph = 3,
+    // c = 0 fluid, 1 sand, 2 solid), so a stencil row is nph slot ranges
+    int nph;
     int exact;                  // 1: oracle operation order; 0: fused fast path
     const SceneBox* scenes;     // batch table, nullptr for a single scene
     const int* sid;             // scene of each particle (input order), batch only
@@ -50,6 +54,8 @@ struct PbfK {
     const int* ph;              // phase (0 fluid, 1 sand, 2 solid), input order
     int* ph_out;                // phase in cell order
     float mix_d, mix_collision, mix_friction, mix_mu_s, mix_mu_k;
+    // a float |ij|^2 above it has sqrtf(|ij|^2) > mix_d: (d (1 + 2^-19))^2 or more
+    float mix_d2_far;
     // fused mode: spiky's SPf and s_corr's 1 / t(dq)^3 moved out of the
     // neighbour loops: -scorr_k / t(dq)^12, SPf * neg_inv_r0, mr0sq * SPf^2
     float f_scorr12, f_spf_neg_inv_r0, f_mr0sq_spf2;
@@ -187,7 +193,7 @@ hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const 
 
 // ---- mixed sand + fluid passes (k.ph, k.ph_out set) ----
 hipError_t launch_mix_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
-                             unsigned long long* masks, const float4* xp, int build, hipStream_t s);
+                             unsigned long long* masks, const float4* xp, int build, int stage, hipStream_t s);
 hipError_t launch_mix_correct(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
                               const float4* xold, float4* vout, float dt, unsigned long long* masks,
                               const float4* xp, hipStream_t s);

@@ -84,12 +84,26 @@ __device__ __forceinline__ float4 pbf_predict(float4 x, float4 v, const PbfK& k,
 }
 
 // get_cell_id (src/neighbors/Utils.hpp:24-33) in the (batch) grid: the y cell
-// index is shifted by the scene's offset
-__device__ __forceinline__ int pbf_cell(float4 p, const PbfK& k, int yoff) {
+// index is shifted by the scene's offset; c: the phase run of a mixed scene
+__device__ __forceinline__ int pbf_cell(float4 p, const PbfK& k, int yoff, int c) {
     const float px = p.x / k.cs;
     const float py = p.y / k.cs;
     const float pz = p.z / k.cs;
-    return ((int)py + yoff) * k.gX * k.gZ + ((int)px) * k.gZ + ((int)pz);
+    return ((((int)py + yoff) * k.gX + (int)px) * k.nph + c) * k.gZ + (int)pz;
+}
+
+// cell column (cy, cx) and cz of a key ((cy gX + cx) nph + c) gZ + cz
+__device__ __forceinline__ void key_cell(int key, const PbfK& k, int& cy, int& cx, int& cz) {
+    const int q = key / k.gZ;
+    cz = key - q * k.gZ;
+    const int col = k.nph == 1 ? q : q / k.nph;
+    cy = col / k.gX;
+    cx = col - cy * k.gX;
+}
+
+// first key of phase run c of cell column (y, x) (z = 0)
+__device__ __forceinline__ int row_key(int y, int x, int c, const PbfK& k) {
+    return ((y * k.gX + x) * k.nph + c) * k.gZ;
 }
 
 // the exact neighbour test (no FMA, oracle order)
@@ -249,11 +263,8 @@ __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, 
                                            const float4* __restrict__ xp, const PbfK& k, int4* __restrict__ hdr,
                                            int4* __restrict__ lst, const float4* __restrict__ cur,
                                            const float4* __restrict__ aux, F& f) {
-    const int key = key2[i] & ~kOvf;
-    const int cy = key / k.gXZ;
-    const int rm = key - cy * k.gXZ;
-    const int cx = rm / k.gZ;
-    const int cz = rm - cx * k.gZ;
+    int cy, cx, cz;
+    key_cell(key2[i] & ~kOvf, k, cy, cx, cz);
     const int z0 = cz > 0 ? cz - 1 : 0;
     const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
     const float4 pi = xp[i];
@@ -268,31 +279,34 @@ __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, 
         const int y = cy + r / 3 - 1;
         const int x = cx + r % 3 - 1;
         if (y < 0 || y >= k.gY || x < 0 || x >= k.gX) continue;
-        const int row = y * k.gXZ + x * k.gZ;
-        const int j0 = cs[row + z0];
-        const int j1 = cs[row + z1 + 1];
-        const int L = j1 - j0;
-        if (L > 64) {
-            for (int j = j0; j < j1; ++j)
-                if (within_h(pi, xp[j], k)) emit(j, at4(cur, j), aux_at(aux, j));
-            continue;
+#pragma unroll 1
+        for (int c = 0; c < k.nph; ++c) {
+            const int row = row_key(y, x, c, k);
+            const int j0 = cs[row + z0];
+            const int j1 = cs[row + z1 + 1];
+            const int L = j1 - j0;
+            if (L > 64) {
+                for (int j = j0; j < j1; ++j)
+                    if (within_h(pi, xp[j], k)) emit(j, at4(cur, j), aux_at(aux, j));
+                continue;
+            }
+            unsigned long long mm = 0ull;
+            const float4* __restrict__ base = xp + j0;
+            int t = 0;
+            for (; t + 4 <= L; t += 4) {
+                const float3 c0 = xyz_at(base, t), c1 = xyz_at(base, t + 1), c2 = xyz_at(base, t + 2),
+                             c3 = xyz_at(base, t + 3);
+                mm |= within_h(pi, c0, k) ? (1ull << t) : 0ull;
+                mm |= within_h(pi, c1, k) ? (2ull << t) : 0ull;
+                mm |= within_h(pi, c2, k) ? (4ull << t) : 0ull;
+                mm |= within_h(pi, c3, k) ? (8ull << t) : 0ull;
+            }
+            for (; t < L; ++t) mm |= within_h(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
+            if (__any((int)((mm >> 32) != 0ull)))
+                visit_row<unsigned long long>(mm, j0, 0, cur, aux, emit);
+            else
+                visit_row<unsigned int>((unsigned int)mm, j0, 0, cur, aux, emit);
         }
-        unsigned long long mm = 0ull;
-        const float4* __restrict__ base = xp + j0;
-        int t = 0;
-        for (; t + 4 <= L; t += 4) {
-            const float3 c0 = xyz_at(base, t), c1 = xyz_at(base, t + 1), c2 = xyz_at(base, t + 2),
-                         c3 = xyz_at(base, t + 3);
-            mm |= within_h(pi, c0, k) ? (1ull << t) : 0ull;
-            mm |= within_h(pi, c1, k) ? (2ull << t) : 0ull;
-            mm |= within_h(pi, c2, k) ? (4ull << t) : 0ull;
-            mm |= within_h(pi, c3, k) ? (8ull << t) : 0ull;
-        }
-        for (; t < L; ++t) mm |= within_h(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
-        if (__any((int)((mm >> 32) != 0ull)))
-            visit_row<unsigned long long>(mm, j0, 0, cur, aux, emit);
-        else
-            visit_row<unsigned int>((unsigned int)mm, j0, 0, cur, aux, emit);
     }
     lw.finish(i, hdr, lst);
 }
@@ -412,11 +426,8 @@ __device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict_
         visit_list<kWalk>(hd, lst, cur, aux, f);
         return;
     }
-    const int key = key2[i] & ~kOvf;
-    const int cy = key / k.gXZ;
-    const int rm = key - cy * k.gXZ;
-    const int cx = rm / k.gZ;
-    const int cz = rm - cx * k.gZ;
+    int cy, cx, cz;
+    key_cell(key2[i] & ~kOvf, k, cy, cx, cz);
     const int z0 = cz > 0 ? cz - 1 : 0;
     const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
     const float4 pi = xp[i];
@@ -425,11 +436,13 @@ __device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict_
         const int y = cy + r / 3 - 1;
         const int x = cx + r % 3 - 1;
         if (y < 0 || y >= k.gY || x < 0 || x >= k.gX) continue;
-        const int row = y * k.gXZ + x * k.gZ;
-        const int j0 = cs[row + z0];
-        const int j1 = cs[row + z1 + 1];
-        for (int j = j0; j < j1; ++j)
-            if (within_h(pi, xp[j], k)) f(j, cur[j], aux_at(aux, j));
+        for (int c = 0; c < k.nph; ++c) {
+            const int row = row_key(y, x, c, k);
+            const int j0 = cs[row + z0];
+            const int j1 = cs[row + z1 + 1];
+            for (int j = j0; j < j1; ++j)
+                if (within_h(pi, xp[j], k)) f(j, cur[j], aux_at(aux, j));
+        }
     }
 }
 
@@ -655,8 +668,9 @@ __global__ __launch_bounds__(kBlock) void k_pbf_key(const float4* __restrict__ x
     if (active) {
         const Bounds b = bounds_of(k, k.scenes ? k.sid[i] : 0);
         // solids of a mixed scene keep their position
-        const float4 xs = (k.ph && k.ph[i] == 2) ? x[i] : pbf_predict(x[i], v[i], k, b, dt);
-        c = pbf_cell(xs, k, b.yoff);
+        const int ph = k.ph ? k.ph[i] : 0;
+        const float4 xs = ph == 2 ? x[i] : pbf_predict(x[i], v[i], k, b, dt);
+        c = pbf_cell(xs, k, b.yoff, k.nph == 1 ? 0 : ph);
         key[i] = c;
     }
     const int sl = wave_atomic_slot(counts, c, active);
@@ -933,20 +947,21 @@ constexpr int kStageSegs = 4;
 // be indexed with 64-bit arithmetic)
 typedef __attribute__((address_space(3))) float lds_float;
 
-// x, y, z of staged slot q
+// staged slot q (kS floats: x, y, z and, for kS = 4, w)
+template <int kS>
 __device__ __forceinline__ float4 staged_at(const lds_float* __restrict__ S, int q) {
-    const lds_float* __restrict__ p = S + __umul24((unsigned)q, 3u);  // q < kStageCap
-    return make_float4(p[0], p[1], p[2], 0.0f);
+    const lds_float* __restrict__ p = S + __umul24((unsigned)q, (unsigned)kS);  // q < kStageCap
+    return make_float4(p[0], p[1], p[2], kS == 4 ? p[3] : 0.0f);
 }
 
 // Visit the set bits of a row mask in ascending order from the staged copy:
 // f(j0 + t, S[lo + t]) (LDS latency is short: no read ahead)
-template <typename M, class F>
+template <typename M, int kS, class F>
 __device__ __forceinline__ void visit_row_staged(M mm, int j0, int lo, const lds_float* __restrict__ S, F& f) {
     while (mm) {
         const int t = low_bit(mm);
         mm &= mm - (M)1;
-        f(j0 + t, staged_at(S, lo + t));
+        f(j0 + t, staged_at<kS>(S, lo + t));
     }
 }
 
@@ -955,22 +970,22 @@ __device__ __forceinline__ int high_bit(unsigned long long m) { return 63 - __bu
 
 // Hit mask of the L (<= bits of M) staged candidates S[lo + t] of pi: four
 // tests make a nibble of inline-constant bits, shifted into the mask once
-template <typename M>
+template <typename M, int kS>
 __device__ __forceinline__ M test_row_staged(const float4 pi, int lo, int L, const lds_float* __restrict__ S,
                                              const PbfK& k) {
     M mm = (M)0;
     int t = 0;
-    const lds_float* __restrict__ b = S + 3 * lo;
-    for (; t + 4 <= L; t += 4, b += 12) {
+    const lds_float* __restrict__ b = S + kS * lo;
+    for (; t + 4 <= L; t += 4, b += 4 * kS) {
         const float4 c0 = make_float4(b[0], b[1], b[2], 0.0f);
-        const float4 c1 = make_float4(b[3], b[4], b[5], 0.0f);
-        const float4 c2 = make_float4(b[6], b[7], b[8], 0.0f);
-        const float4 c3 = make_float4(b[9], b[10], b[11], 0.0f);
+        const float4 c1 = make_float4(b[kS], b[kS + 1], b[kS + 2], 0.0f);
+        const float4 c2 = make_float4(b[2 * kS], b[2 * kS + 1], b[2 * kS + 2], 0.0f);
+        const float4 c3 = make_float4(b[3 * kS], b[3 * kS + 1], b[3 * kS + 2], 0.0f);
         const unsigned int nib = (within_h(pi, c0, k) ? 1u : 0u) | (within_h(pi, c1, k) ? 2u : 0u) |
                                  (within_h(pi, c2, k) ? 4u : 0u) | (within_h(pi, c3, k) ? 8u : 0u);
         mm |= (M)nib << t;
     }
-    for (; t < L; ++t, b += 3) {
+    for (; t < L; ++t, b += kS) {
         const float4 c = make_float4(b[0], b[1], b[2], 0.0f);
         mm |= (M)(within_h(pi, c, k) ? 1u : 0u) << t;
     }
@@ -985,7 +1000,40 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The mixed scene's lambda sums (fluid i): rho and the gradient over every
+// neighbour, the own-gradient-norm term over fluid neighbours (w == 0) only
 template <bool kFast>
+struct MixLambdaAcc {
+    float rho = 0.0f, a = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    __device__ __forceinline__ void add(const float4 pi, const float4 pj, const PbfK& k) {
+        const float dx = pi.x - pj.x;
+        const float dy = pi.y - pj.y;
+        const float dz = pi.z - pj.z;
+        const float r2 = dist2<kFast>(dx, dy, dz);
+        if (r2 <= k.h2) {
+            const float t = poly_t<kFast>(r2, k);
+            rho = acc<kFast>(rho, t * t, t);
+            if (r2 > 0.0f) {
+                const float s = spiky_s<kFast>(r2, k);
+                if (pj.w == 0.0f) a = acc<kFast>(a, s * s, r2);
+                gx = acc<kFast>(gx, s, dx);
+                gy = acc<kFast>(gy, s, dy);
+                gz = acc<kFast>(gz, s, dz);
+            }
+        }
+    }
+    __device__ __forceinline__ float lambda(const PbfK& k) const {
+        const float Ci = rho * k.c_rho - 1.0f;
+        const float gg = gx * gx + gy * gy + gz * gz;
+        const float den = k.mr0sq * (a + gg) + k.eps;
+        return -Ci / den;
+    }
+};
+
+// kMixed: the mixed scene's first lambda pass (phase runs, nph = 3): fluid
+// lanes sum MixLambdaAcc, sand lanes only build their lists, solid lanes
+// write an empty list (no pass reads it); the staged copy keeps w (the phase).
+template <bool kFast, bool kMixed>
 __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __restrict__ xp,
                                                                  const int* __restrict__ key2,
                                                                  const int* __restrict__ cs, int n, PbfK k,
@@ -993,8 +1041,12 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
                                                                  int* __restrict__ maxc,
                                                                  unsigned long long* __restrict__ nb,
                                                                  const int* __restrict__ gate) {
+    constexpr int kS = kMixed ? 4 : 3;  // floats per staged slot
+    // runs per wave staged (a mixed wave holds a column's fluid, sand and
+    // solid runs, so more of them)
+    constexpr int kSegs = kMixed ? 2 * kStageSegs : kStageSegs;
     if (gate && *gate == 0) return;  // uniform: before any barrier
-    __shared__ float stage[kBlock / 64][3 * kStageCap];
+    __shared__ float stage[kBlock / 64][kS * kStageCap];
     const int i = xcd_index();
     const int lane = (int)(threadIdx.x & 63);
     lds_float* __restrict__ S = (lds_float*)stage[threadIdx.x >> 6];
@@ -1002,25 +1054,32 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
     // (i >= n, the last wave only) has empty windows and stores nothing
     const bool act = i < n;
     const int ii = act ? i : n - 1;
-    const int key = key2[ii] & ~kOvf;
-    const int cy = key / k.gXZ;
-    const int rm = key - cy * k.gXZ;
-    const int cx = rm / k.gZ;
-    const int cz = rm - cx * k.gZ;
+    const int ph = kMixed ? k.ph_out[ii] : 0;
+    const bool lists = act && ph != 2;
+    int cy, cx, cz;
+    key_cell(key2[ii] & ~kOvf, k, cy, cx, cz);
     const int z0 = cz > 0 ? cz - 1 : 0;
     const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
     const float4 pi = xp[ii];
     int4* __restrict__ lst = nbr_list(nb, n, ii);
     LambdaAcc<kFast, false> acc(k);
+    MixLambdaAcc<kFast> macc;
     ListWriter lw(ii);
+    auto sum = [&](const float4 pj) {
+        if constexpr (kMixed) {
+            if (ph == 0) macc.add(pi, pj, k);
+        } else {
+            acc.add(pi, pj, k);
+        }
+    };
     auto hit = [&](int j, const float4 pj) {
         lw.emit(j, lst);
-        acc.add(pi, pj, k);
+        sum(pj);
     };
     // a hit of a row announced with row_hits
     auto hit_row = [&](int j, const float4 pj) {
         if (lw.append_row(j)) lst[(size_t)((lw.c >> 3) - 1) * 64] = lw.group();
-        acc.add(pi, pj, k);
+        sum(pj);
     };
     const unsigned long long below = (1ull << lane) - 1ull;  // lanes before this one
 #pragma unroll 1
@@ -1028,112 +1087,129 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
         lw.row(r, ii);
         const int y = cy + r / 3 - 1;
         const int x = cx + r % 3 - 1;
-        const bool valid = act && y >= 0 && y < k.gY && x >= 0 && x < k.gX;
-        int j0 = 0, j1 = 0;
-        if (valid) {
-            const int row = y * k.gXZ + x * k.gZ;
-            j0 = cs[row + z0];
-            j1 = cs[row + z1 + 1];
-        }
-        const int L = j1 - j0;
-        // runs of lanes whose row r lies in one column (-1: no row)
-        const int col = valid ? y * k.gX + x : -1;
+        const bool valid = lists && y >= 0 && y < k.gY && x >= 0 && x < k.gX;
+        // runs of lanes whose row r lies in one column and whose own phase
+        // run is one (the slots of a run ascend with cz, and so do its
+        // windows); -1: no row
+        const int col = valid ? (y * k.gX + x) * k.nph + ph : -1;
         const int pcol = __shfl_up(col, 1, 64);
         const int ncol = __shfl_down(col, 1, 64);
         const unsigned long long fm = __ballot(valid && (lane == 0 || pcol != col));
         const unsigned long long lm = __ballot(valid && (lane == 63 || ncol != col));
         const int nseg = __popcll(fm);
         if (nseg == 0) continue;  // uniform: no lane has row r
-        // the runs' slot ranges [start_s, end_s), packed at LDS offsets off_s
-        int s_start[kStageSegs], s_off[kStageSegs + 1];
-        int tot = 0;
-        {
-            unsigned long long f = fm, l = lm;
-#pragma unroll
-            for (int s = 0; s < kStageSegs; ++s) {
-                s_off[s] = tot;
-                s_start[s] = 0;
-                if (s < nseg) {
-                    const int lf = __ffsll((long long)f) - 1, ll = __ffsll((long long)l) - 1;
-                    f &= f - 1ull;
-                    l &= l - 1ull;
-                    const int a = __builtin_amdgcn_readlane(j0, lf);
-                    const int b = __builtin_amdgcn_readlane(j1, ll);
-                    s_start[s] = a;
-                    tot += b - a;
-                }
+#pragma unroll 1
+        for (int c = 0; c < k.nph; ++c) {  // the row's phase runs (one range for nph = 1)
+            int j0 = 0, j1 = 0;
+            if (valid) {
+                const int row = row_key(y, x, c, k);
+                j0 = cs[row + z0];
+                j1 = cs[row + z1 + 1];
             }
-            s_off[kStageSegs] = tot;
-        }
-        const bool staged = nseg <= kStageSegs && tot <= kStageCap;  // uniform
-        if (staged) {
-            for (int q = lane; q < tot; q += 64) {
-                int j = q - s_off[0] + s_start[0];
+            const int L = j1 - j0;
+            // the runs' slot ranges [start_s, end_s), packed at LDS offsets off_s
+            int s_start[kSegs], s_off[kSegs + 1];
+            int tot = 0;
+            {
+                unsigned long long f = fm, l = lm;
 #pragma unroll
-                for (int s = 1; s < kStageSegs; ++s)
-                    if (q >= s_off[s]) j = q - s_off[s] + s_start[s];
-                const float4 p = at4(xp, j);
-                S[3 * q] = p.x;
-                S[3 * q + 1] = p.y;
-                S[3 * q + 2] = p.z;
-            }
-            wave_lds_sync();
-            // this lane's run and the LDS offset of its window
-            const int seg = __popcll(fm & (below | (1ull << lane))) - 1;
-            int lo = j0 - s_start[0] + s_off[0];
-#pragma unroll
-            for (int s = 1; s < kStageSegs; ++s)
-                if (seg == s) lo = j0 - s_start[s] + s_off[s];
-            if (L > 64) {
-                for (int t = 0; t < L; ++t) {
-                    const float4 c = staged_at(S, lo + t);
-                    if (within_h(pi, c, k)) hit(j0 + t, c);
+                for (int s = 0; s < kSegs; ++s) {
+                    s_off[s] = tot;
+                    s_start[s] = 0;
+                    if (s < nseg) {
+                        const int lf = __ffsll((long long)f) - 1, ll = __ffsll((long long)l) - 1;
+                        f &= f - 1ull;
+                        l &= l - 1ull;
+                        const int a = __builtin_amdgcn_readlane(j0, lf);
+                        const int b = __builtin_amdgcn_readlane(j1, ll);
+                        s_start[s] = a;
+                        tot += b - a;
+                    }
                 }
-            } else if (__all(L <= 32)) {
-                const unsigned int mm = test_row_staged<unsigned int>(pi, lo, L, S, k);
-                if (mm) lw.row_hits(j0 + low_bit(mm), j0 + high_bit(mm));
-                visit_row_staged<unsigned int>(mm, j0, lo, S, hit_row);
-            } else {
-                const unsigned long long mm = test_row_staged<unsigned long long>(pi, lo, L, S, k);
-                if (mm) lw.row_hits(j0 + low_bit(mm), j0 + high_bit(mm));
+                s_off[kSegs] = tot;
+            }
+            // uniform: every window of this range is empty (tot covers every
+            // run only when there are at most kSegs of them)
+            if (tot == 0 && nseg <= kSegs) continue;
+            const bool staged = nseg <= kSegs && tot <= kStageCap;  // uniform
+            if (staged) {
+                for (int q = lane; q < tot; q += 64) {
+                    int j = q - s_off[0] + s_start[0];
+#pragma unroll
+                    for (int s = 1; s < kSegs; ++s)
+                        if (q >= s_off[s]) j = q - s_off[s] + s_start[s];
+                    const float4 p = at4(xp, j);
+                    S[kS * q] = p.x;
+                    S[kS * q + 1] = p.y;
+                    S[kS * q + 2] = p.z;
+                    if (kS == 4) S[kS * q + 3] = p.w;
+                }
+                wave_lds_sync();
+                // this lane's run and the LDS offset of its window
+                const int seg = __popcll(fm & (below | (1ull << lane))) - 1;
+                int lo = j0 - s_start[0] + s_off[0];
+#pragma unroll
+                for (int s = 1; s < kSegs; ++s)
+                    if (seg == s) lo = j0 - s_start[s] + s_off[s];
+                if (L > 64) {
+                    for (int t = 0; t < L; ++t) {
+                        const float4 q = staged_at<kS>(S, lo + t);
+                        if (within_h(pi, q, k)) hit(j0 + t, q);
+                    }
+                } else if (__all(L <= 32)) {
+                    const unsigned int mm = test_row_staged<unsigned int, kS>(pi, lo, L, S, k);
+                    if (mm) lw.row_hits(j0 + low_bit(mm), j0 + high_bit(mm));
+                    visit_row_staged<unsigned int, kS>(mm, j0, lo, S, hit_row);
+                } else {
+                    const unsigned long long mm = test_row_staged<unsigned long long, kS>(pi, lo, L, S, k);
+                    if (mm) lw.row_hits(j0 + low_bit(mm), j0 + high_bit(mm));
+                    if (__any((int)((mm >> 32) != 0ull)))
+                        visit_row_staged<unsigned long long, kS>(mm, j0, lo, S, hit_row);
+                    else
+                        visit_row_staged<unsigned int, kS>((unsigned int)mm, j0, lo, S, hit_row);
+                }
+                wave_lds_sync();  // the next range's staging overwrites this one's
+            } else if (valid) {
+                if (L > 64) {
+                    for (int j = j0; j < j1; ++j) {
+                        const float4 q = at4(xp, j);
+                        if (within_h(pi, q, k)) hit(j, q);
+                    }
+                    continue;
+                }
+                unsigned long long mm = 0ull;
+                const float4* __restrict__ base = xp + j0;
+                int t = 0;
+                for (; t + 4 <= L; t += 4) {
+                    const float3 c0 = xyz_at(base, t), c1 = xyz_at(base, t + 1), c2 = xyz_at(base, t + 2),
+                                 c3 = xyz_at(base, t + 3);
+                    const unsigned int nib = (within_h(pi, c0, k) ? 1u : 0u) | (within_h(pi, c1, k) ? 2u : 0u) |
+                                             (within_h(pi, c2, k) ? 4u : 0u) | (within_h(pi, c3, k) ? 8u : 0u);
+                    mm |= (unsigned long long)nib << t;
+                }
+                for (; t < L; ++t) mm |= within_h(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
+                auto hit4 = [&](int j, const float4 pj, const float4) { hit(j, pj); };
                 if (__any((int)((mm >> 32) != 0ull)))
-                    visit_row_staged<unsigned long long>(mm, j0, lo, S, hit_row);
+                    visit_row<unsigned long long>(mm, j0, 0, xp, nullptr, hit4);
                 else
-                    visit_row_staged<unsigned int>((unsigned int)mm, j0, lo, S, hit_row);
+                    visit_row<unsigned int>((unsigned int)mm, j0, 0, xp, nullptr, hit4);
             }
-            wave_lds_sync();  // the next row's staging overwrites this row's
-        } else if (valid) {
-            if (L > 64) {
-                for (int j = j0; j < j1; ++j) {
-                    const float4 c = at4(xp, j);
-                    if (within_h(pi, c, k)) hit(j, c);
-                }
-                continue;
-            }
-            unsigned long long mm = 0ull;
-            const float4* __restrict__ base = xp + j0;
-            int t = 0;
-            for (; t + 4 <= L; t += 4) {
-                const float3 c0 = xyz_at(base, t), c1 = xyz_at(base, t + 1), c2 = xyz_at(base, t + 2),
-                             c3 = xyz_at(base, t + 3);
-                const unsigned int nib = (within_h(pi, c0, k) ? 1u : 0u) | (within_h(pi, c1, k) ? 2u : 0u) |
-                                         (within_h(pi, c2, k) ? 4u : 0u) | (within_h(pi, c3, k) ? 8u : 0u);
-                mm |= (unsigned long long)nib << t;
-            }
-            for (; t < L; ++t) mm |= within_h(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
-            auto hit4 = [&](int j, const float4 pj, const float4) { hit(j, pj); };
-            if (__any((int)((mm >> 32) != 0ull)))
-                visit_row<unsigned long long>(mm, j0, 0, xp, nullptr, hit4);
-            else
-                visit_row<unsigned int>((unsigned int)mm, j0, 0, xp, nullptr, hit4);
         }
     }
     float Ci = -1e30f;
     if (act) {
-        Ci = acc.finish(pi, xl + i, k);
-        lw.finish(i, nbr_hdr(nb), lst);
+        if constexpr (kMixed) {
+            xl[i] = make_float4(pi.x, pi.y, pi.z, ph == 0 ? macc.lambda(k) : 0.0f);
+            if (lists)
+                lw.finish(i, nbr_hdr(nb), lst);
+            else
+                nbr_hdr(nb)[i] = make_int4(i, i, i, 0);
+        } else {
+            Ci = acc.finish(pi, xl + i, k);
+            lw.finish(i, nbr_hdr(nb), lst);
+        }
     }
-    block_max_store(Ci, maxc);
+    if (!kMixed) block_max_store(Ci, maxc);
 }
 
 template <bool kBuild, bool kFast>
@@ -1188,11 +1264,13 @@ hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, 
         return hipGetLastError();
     }
     const dim3 g(grid_for(n, ta.block)), b(ta.block);
-    if (build && ta.stage && xa == xp && n > 0) {
+    if (build && ta.stage && xa == xp && n > 0 && k.nph == 1) {
         if (k.exact)
-            launch_kernel((k_pbf_lambda_build_ws<false>), g, b, 0, s, xp, key2, cs, n, k, xl, maxc, masks, gate);
+            launch_kernel((k_pbf_lambda_build_ws<false, false>), g, b, 0, s, xp, key2, cs, n, k, xl, maxc, masks,
+                          gate);
         else
-            launch_kernel((k_pbf_lambda_build_ws<true>), g, b, 0, s, xp, key2, cs, n, k, xl, maxc, masks, gate);
+            launch_kernel((k_pbf_lambda_build_ws<true, false>), g, b, 0, s, xp, key2, cs, n, k, xl, maxc, masks,
+                          gate);
         return hipGetLastError();
     }
     if (build && k.exact)
@@ -1206,24 +1284,33 @@ hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, 
     return hipGetLastError();
 }
 
-// delta-p pass with s_corr (PBF eq. 12-14), Jacobi update, box clamp.
+// delta-p sums with s_corr (PBF eq. 12-14) over the neighbours pj of pi
+// (lambda_i li): add; the position step is (ax, ay, az) * step.
 // kNS: s_corr exponent fixed at compile time (4), or 0 for k.n_scorr.
-template <bool kFast, int kNS, class Visit>
-__device__ __forceinline__ void deltap_body(int i, const float4* __restrict__ xl, const PbfK& k,
-                                            float4* __restrict__ xa, const float4* __restrict__ xold,
-                                            float4* __restrict__ vout, float dt, Visit&& visit) {
-    const float4 pi = xl[i];
-    const float li = pi.w;
+template <bool kFast, int kNS>
+struct DeltapAcc {
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    float step = k.neg_inv_r0;
-    if constexpr (kFast && kNS == 4) {
-        // SPf leaves the sums (f_spf_neg_inv_r0); s_corr = -k (t / t(dq))^12
-        // as one constant times t^12 (f_scorr12)
-        const float mf = vreg(-k.f), hf = vreg(k.hf), mf2 = vreg(-k.f2), hf2 = vreg(k.hf2);
-        visit([&](int, const float4 pj, const float4) {
-            const float dx = pi.x - pj.x;
-            const float dy = pi.y - pj.y;
-            const float dz = pi.z - pj.z;
+    float step;
+    float mf, hf, mf2, hf2;  // fast mode, kNS = 4: kernel constants in vector registers
+
+    __device__ __forceinline__ explicit DeltapAcc(const PbfK& k) {
+        if constexpr (kFast && kNS == 4) {
+            // SPf leaves the sums (f_spf_neg_inv_r0); s_corr = -k (t / t(dq))^12
+            // as one constant times t^12 (f_scorr12)
+            mf = vreg(-k.f);
+            hf = vreg(k.hf);
+            mf2 = vreg(-k.f2);
+            hf2 = vreg(k.hf2);
+            step = k.f_spf_neg_inv_r0;
+        } else {
+            step = k.neg_inv_r0;
+        }
+    }
+    __device__ __forceinline__ void add(const float4 pi, float li, const float4 pj, const PbfK& k) {
+        const float dx = pi.x - pj.x;
+        const float dy = pi.y - pj.y;
+        const float dz = pi.z - pj.z;
+        if constexpr (kFast && kNS == 4) {
             const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
             if (r2 <= k.h2 && r2 > 0.0f) {
                 const float s = spiky_unit(r2, mf, hf);
@@ -1236,34 +1323,42 @@ __device__ __forceinline__ void deltap_body(int i, const float4* __restrict__ xl
                 ay = __builtin_fmaf(c, dy, ay);
                 az = __builtin_fmaf(c, dz, az);
             }
-        });
-        step = k.f_spf_neg_inv_r0;
-    } else visit([&](int, const float4 pj, const float4) {
-        const float dx = pi.x - pj.x;
-        const float dy = pi.y - pj.y;
-        const float dz = pi.z - pj.z;
-        const float r2 = dist2<kFast>(dx, dy, dz);
-        if (r2 <= k.h2 && r2 > 0.0f) {
-            const float s = spiky_s<kFast>(r2, k);
-            const float t = poly_t<kFast>(r2, k);
-            const float ratio = ((t * t) * t) * k.inv_tq3;
-            float pw;
-            if (kNS == 4) {
-                pw = ratio * ratio;
-                pw = pw * ratio;
-                pw = pw * ratio;
-            } else {
-                pw = ratio;
-                for (int e = 1; e < k.n_scorr; ++e) pw = pw * ratio;
+        } else {
+            const float r2 = dist2<kFast>(dx, dy, dz);
+            if (r2 <= k.h2 && r2 > 0.0f) {
+                const float s = spiky_s<kFast>(r2, k);
+                const float t = poly_t<kFast>(r2, k);
+                const float ratio = ((t * t) * t) * k.inv_tq3;
+                float pw;
+                if (kNS == 4) {
+                    pw = ratio * ratio;
+                    pw = pw * ratio;
+                    pw = pw * ratio;
+                } else {
+                    pw = ratio;
+                    for (int e = 1; e < k.n_scorr; ++e) pw = pw * ratio;
+                }
+                const float sc = -k.scorr_k * pw;
+                const float coef = (li + pj.w) + sc;
+                const float c = coef * s;
+                ax = acc<kFast>(ax, c, dx);
+                ay = acc<kFast>(ay, c, dy);
+                az = acc<kFast>(az, c, dz);
             }
-            const float sc = -k.scorr_k * pw;
-            const float coef = (li + pj.w) + sc;
-            const float c = coef * s;
-            ax = acc<kFast>(ax, c, dx);
-            ay = acc<kFast>(ay, c, dy);
-            az = acc<kFast>(az, c, dz);
         }
-    });
+    }
+};
+
+// delta-p pass: the sums of DeltapAcc, Jacobi update, box clamp
+template <bool kFast, int kNS, class Visit>
+__device__ __forceinline__ void deltap_body(int i, const float4* __restrict__ xl, const PbfK& k,
+                                            float4* __restrict__ xa, const float4* __restrict__ xold,
+                                            float4* __restrict__ vout, float dt, Visit&& visit) {
+    const float4 pi = xl[i];
+    const float li = pi.w;
+    DeltapAcc<kFast, kNS> d(k);
+    visit([&](int, const float4 pj, const float4) { d.add(pi, li, pj, k); });
+    const float ax = d.ax, ay = d.ay, az = d.az, step = d.step;
     const Bounds b = bounds_of(k, k.scenes ? k.sid_out[i] : 0);
     float4 r;
     r.x = rclamp(pi.x + ax * step, b.lo[0], b.hi[0]);
@@ -1503,12 +1598,17 @@ hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const 
 // within d, friction against the step's old positions.  Solids do not move.
 // The sand arithmetic is IEEE in both arithmetic modes.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void mix_sand_pair(const float4 pi, const float4 xoi, const float4 pj, const float4 xoj,
-                                              int jph, const PbfK& k, float& dpx, float& dpy, float& dpz) {
+__device__ __forceinline__ void mix_sand_pair(const float4 pi, const float4 xoi, const float4 pj,
+                                              const float4* __restrict__ xoj_at, const PbfK& k, float& dpx,
+                                              float& dpy, float& dpz) {
     float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
+    // most neighbours within h are beyond d: those whose squared length is
+    // well above d^2 skip the square root (its result would exceed d)
+    const float ij2 = dot3(ijx, ijy, ijz, ijx, ijy, ijz);
+    if (ij2 > k.mix_d2_far) return;
     // |ij| == 0 -> ij = (0, 1e-5, 0) (src/Simulate.cpp:236-239); the length
     // is computed again only then
-    float len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
+    float len = sqrtf(ij2);
     if (len == 0.0f) {
         ijx = 0.0f;
         ijy = 0.00001f;
@@ -1516,6 +1616,9 @@ __device__ __forceinline__ void mix_sand_pair(const float4 pi, const float4 xoi,
         len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
     }
     if (len > k.mix_d) return;
+    // j's old position and phase (w), read only for a contact
+    const float4 xoj = *xoj_at;
+    const int jph = (int)xoj.w;
     float sc;
     if (jph == 2) {
         sc = k.mix_collision * (len - k.mix_d);
@@ -1578,9 +1681,13 @@ __global__ __launch_bounds__(kBlock) void k_mix_lambda(const float4* __restrict_
     const float4 pi = xa[i];
     const int ph = k.ph_out[i];
     if (ph != 0) {
-        // no constraint; a building pass still writes the masks
-        if (kBuild)
+        // no constraint; a building pass still writes the lists of sand (the
+        // correction pass walks them), and an empty one for a solid, whose
+        // list no pass reads
+        if (kBuild && ph == 1)
             for_each_neighbor<true>(i, n, key2, cs, masks, xp, xa, nullptr, k, [](int, const float4, const float4) {});
+        else if (kBuild)
+            nbr_hdr(masks)[i] = make_int4(i, i, i, 0);
         xl[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
         return;
     }
@@ -1609,8 +1716,17 @@ __global__ __launch_bounds__(kBlock) void k_mix_lambda(const float4* __restrict_
 }
 
 hipError_t launch_mix_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
-                             unsigned long long* masks, const float4* xp, int build, hipStream_t s) {
+                             unsigned long long* masks, const float4* xp, int build, int stage, hipStream_t s) {
     const dim3 g(grid_for(n)), b(kBlock);
+    if (build && stage && xa == xp && n > 0 && k.ph_out) {
+        if (k.exact)
+            launch_kernel((k_pbf_lambda_build_ws<false, true>), g, b, 0, s, xp, key2, cs, n, k, xl, (int*)nullptr,
+                          masks, (const int*)nullptr);
+        else
+            launch_kernel((k_pbf_lambda_build_ws<true, true>), g, b, 0, s, xp, key2, cs, n, k, xl, (int*)nullptr,
+                          masks, (const int*)nullptr);
+        return hipGetLastError();
+    }
     if (build && k.exact)
         launch_kernel((k_mix_lambda<true, false>), g, b, 0, s, xa, key2, cs, n, k, xl, masks, xp);
     else if (build)
@@ -1622,13 +1738,17 @@ hipError_t launch_mix_lambda(const float4* xa, int* key2, const int* cs, int n, 
     return hipGetLastError();
 }
 
-template <bool kFast>
+constexpr int kMixContacts = 16;  // contacts of a sand particle held for the projection loop
+
+template <bool kFast, int kNS>
 __global__ __launch_bounds__(kBlock) void k_mix_correct(const float4* __restrict__ xl, int* __restrict__ key2,
                                                         const int* __restrict__ cs, int n, PbfK k,
                                                         float4* __restrict__ xa, const float4* __restrict__ xold,
                                                         float4* __restrict__ vout, float dt,
                                                         unsigned long long* __restrict__ masks,
                                                         const float4* __restrict__ xp) {
+    // contacts of a sand lane, in list order: contact[u * kBlock + thread]
+    __shared__ int contact[kMixContacts * kBlock];
     const int i = xcd_index();
     if (i >= n) return;
     const int ph = k.ph_out[i];
@@ -1637,39 +1757,44 @@ __global__ __launch_bounds__(kBlock) void k_mix_correct(const float4* __restrict
     if (ph == 1) {
         const float4 xoi = xold[i];
         float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
+        // A contact (within d) is rare per neighbour but, over a wave, some
+        // lane has one at almost every step of the walk: the walk only
+        // collects the neighbours that can be within d (mix_d2_far), and a
+        // second loop, as long as the wave's largest contact count, projects
+        // them in list order (the same sums in the same order).
+        int nc = 0;
         for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k, [&](int j, const float4 pj, const float4) {
             if (j == i) return;
-            const float4 xoj = xold[j];
-            mix_sand_pair(pi, xoi, pj, xoj, (int)xoj.w, k, dpx, dpy, dpz);
+            const float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
+            if (dot3(ijx, ijy, ijz, ijx, ijy, ijz) > k.mix_d2_far) return;
+            if (nc < kMixContacts) contact[nc * kBlock + (int)threadIdx.x] = j;
+            ++nc;
         });
+        if (nc <= kMixContacts) {
+            for (int u = 0; u < nc; ++u) {
+                const int j = contact[u * kBlock + (int)threadIdx.x];
+                mix_sand_pair(pi, xoi, xl[j], xold + j, k, dpx, dpy, dpz);
+            }
+        } else {
+            // more than the buffer holds: the walk again, every neighbour
+            for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k,
+                                     [&](int j, const float4 pj, const float4) {
+                if (j == i) return;
+                mix_sand_pair(pi, xoi, pj, xold + j, k, dpx, dpy, dpz);
+            });
+        }
         r.x = rclamp(pi.x + dpx, k.lo[0], k.hi[0]);
         r.y = rclamp(pi.y + dpy, k.lo[1], k.hi[1]);
         r.z = rclamp(pi.z + dpz, k.lo[2], k.hi[2]);
     } else if (ph == 0) {
+        // the plain delta-p sums (lambda_j = 0 for sand and solids)
         const float li = pi.w;
-        float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k, [&](int, const float4 pj, const float4) {
-            const float dx = pi.x - pj.x;
-            const float dy = pi.y - pj.y;
-            const float dz = pi.z - pj.z;
-            const float r2 = dist2<kFast>(dx, dy, dz);
-            if (r2 <= k.h2 && r2 > 0.0f) {
-                const float s = spiky_s<kFast>(r2, k);
-                const float t = poly_t<kFast>(r2, k);
-                const float ratio = ((t * t) * t) * k.inv_tq3;
-                float pw = ratio;
-                for (int e = 1; e < k.n_scorr; ++e) pw = pw * ratio;
-                const float sc = -k.scorr_k * pw;
-                const float coef = (li + pj.w) + sc;
-                const float c = coef * s;
-                ax = acc<kFast>(ax, c, dx);
-                ay = acc<kFast>(ay, c, dy);
-                az = acc<kFast>(az, c, dz);
-            }
-        });
-        r.x = rclamp(pi.x + ax * k.neg_inv_r0, k.lo[0], k.hi[0]);
-        r.y = rclamp(pi.y + ay * k.neg_inv_r0, k.lo[1], k.hi[1]);
-        r.z = rclamp(pi.z + az * k.neg_inv_r0, k.lo[2], k.hi[2]);
+        DeltapAcc<kFast, kNS> d(k);
+        for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k,
+                                 [&](int, const float4 pj, const float4) { d.add(pi, li, pj, k); });
+        r.x = rclamp(pi.x + d.ax * d.step, k.lo[0], k.hi[0]);
+        r.y = rclamp(pi.y + d.ay * d.step, k.lo[1], k.hi[1]);
+        r.z = rclamp(pi.z + d.az * d.step, k.lo[2], k.hi[2]);
     }
     xa[i] = r;
     if (vout) {
@@ -1683,9 +1808,11 @@ hipError_t launch_mix_correct(const float4* xl, int* key2, const int* cs, int n,
                               const float4* xp, hipStream_t s) {
     const dim3 g(grid_for(n)), b(kBlock);
     if (k.exact)
-        launch_kernel((k_mix_correct<false>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+        launch_kernel((k_mix_correct<false, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+    else if (k.n_scorr == 4)
+        launch_kernel((k_mix_correct<true, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
     else
-        launch_kernel((k_mix_correct<true>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
+        launch_kernel((k_mix_correct<true, 0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp);
     return hipGetLastError();
 }
 

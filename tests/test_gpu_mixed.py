@@ -22,10 +22,11 @@ def _oracle_run(oracle, m, pos, ph, steps):
     return x, v, ids, errs
 
 
-def _gpu_run(gpu_mod, m, pos, ph, steps, exact=True, graph=False, check=False):
+def _gpu_run(gpu_mod, m, pos, ph, steps, exact=True, graph=False, check=False, staged=True):
     mp = gpu_mod.mix_params(mu_s=m.mu_s, mu_k=m.mu_k, collision=m.collision, friction=m.friction)
     with gpu_mod.Gpu(0) as g:
         g.mix_init(gpu_mod.pbf_params(m.fluid, exact=exact), mp, pos, ph)
+        g.pbf_set_build(staged)
         if check:
             g.pbf_set_mode(learned=False, check=True)
         g.pbf_set_graph(graph)
@@ -91,3 +92,32 @@ def test_mixed_full_size_invariants(gpu_mod):
     mov = x[~solid]
     assert (mov >= 0.5).all() and (mov <= 127.5).all()
     assert np.array_equal(np.sort(gid), np.arange(pos.shape[0]))
+
+
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("scene", ["small", "cfg1"])
+def test_mixed_staged_build_matches_gathers(gpu_mod, scene, exact):
+    # the wave-staged list build of the mixed scene (phase runs staged with
+    # their w, solids without lists) against the per-lane gather build: the
+    # same bits after three steps
+    m = scenes.mixed_small(12, 8, 12, box=(28, 24, 28), field=(28, 28), seed=9) if scene == "small" else scenes.cfg1()
+    pos, ph = m.arrays()
+    a = _gpu_run(gpu_mod, m, pos, ph, 3, exact=exact, staged=False)
+    b = _gpu_run(gpu_mod, m, pos, ph, 3, exact=exact, staged=True)
+    for u, w in zip(a[:3], b[:3]):
+        assert np.array_equal(u, w)
+
+
+def test_mixed_dense_sand_many_contacts_bits(gpu_mod, oracle):
+    # a sand clump at twice the lattice density (about 30 contacts within d
+    # per particle, more than the projection loop's buffer of 16) next to a
+    # fluid block, over the solid field: exact mode bit for bit
+    m = scenes.mixed_small(8, 6, 8, box=(24, 24, 24), field=(24, 24), seed=11)
+    pos, ph = m.arrays()
+    clump = scenes.jittered_lattice(6, 6, 6, cell=0.5, origin=(4.0, 8.0, 4.0), amplitude=0.01, seed=5)
+    pos = np.concatenate([clump, pos]).astype(np.float32)
+    ph = np.concatenate([np.ones(clump.shape[0], np.int32), ph])
+    ox, ov, oid, _ = _oracle_run(oracle, m, pos, ph, 2)
+    gx, gv, gid, _ = _gpu_run(gpu_mod, m, pos, ph, 2)
+    assert np.array_equal(gid, oid)
+    assert np.array_equal(gx, ox) and np.array_equal(gv, ov)
