@@ -1691,28 +1691,10 @@ __global__ __launch_bounds__(kBlock) void k_mix_lambda(const float4* __restrict_
         xl[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
         return;
     }
-    float rho = 0.0f, a = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    for_each_neighbor<kBuild>(i, n, key2, cs, masks, xp, xa, nullptr, k, [&](int j, const float4 pj, const float4) {
-        const float dx = pi.x - pj.x;
-        const float dy = pi.y - pj.y;
-        const float dz = pi.z - pj.z;
-        const float r2 = dist2<kFast>(dx, dy, dz);
-        if (r2 <= k.h2) {
-            const float t = poly_t<kFast>(r2, k);
-            rho = acc<kFast>(rho, t * t, t);
-            if (r2 > 0.0f) {
-                const float s = spiky_s<kFast>(r2, k);
-                if (pj.w == 0.0f) a = acc<kFast>(a, s * s, r2);
-                gx = acc<kFast>(gx, s, dx);
-                gy = acc<kFast>(gy, s, dy);
-                gz = acc<kFast>(gz, s, dz);
-            }
-        }
-    });
-    const float Ci = rho * k.c_rho - 1.0f;
-    const float gg = gx * gx + gy * gy + gz * gz;
-    const float den = k.mr0sq * (a + gg) + k.eps;
-    xl[i] = make_float4(pi.x, pi.y, pi.z, -Ci / den);
+    MixLambdaAcc<kFast> m;
+    for_each_neighbor<kBuild, kWalkWide>(i, n, key2, cs, masks, xp, xa, nullptr, k,
+                                         [&](int, const float4 pj, const float4) { m.add(pi, pj, k); });
+    xl[i] = make_float4(pi.x, pi.y, pi.z, m.lambda(k));
 }
 
 hipError_t launch_mix_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,
@@ -1763,7 +1745,7 @@ __global__ __launch_bounds__(kBlock) void k_mix_correct(const float4* __restrict
         // second loop, as long as the wave's largest contact count, projects
         // them in list order (the same sums in the same order).
         int nc = 0;
-        for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k, [&](int j, const float4 pj, const float4) {
+        for_each_neighbor<false, kWalkWide>(i, n, key2, cs, masks, xp, xl, nullptr, k, [&](int j, const float4 pj, const float4) {
             if (j == i) return;
             const float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
             if (dot3(ijx, ijy, ijz, ijx, ijy, ijz) > k.mix_d2_far) return;
@@ -1790,8 +1772,8 @@ __global__ __launch_bounds__(kBlock) void k_mix_correct(const float4* __restrict
         // the plain delta-p sums (lambda_j = 0 for sand and solids)
         const float li = pi.w;
         DeltapAcc<kFast, kNS> d(k);
-        for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k,
-                                 [&](int, const float4 pj, const float4) { d.add(pi, li, pj, k); });
+        for_each_neighbor<false, kWalkWide>(i, n, key2, cs, masks, xp, xl, nullptr, k,
+                                            [&](int, const float4 pj, const float4) { d.add(pi, li, pj, k); });
         r.x = rclamp(pi.x + d.ax * d.step, k.lo[0], k.hi[0]);
         r.y = rclamp(pi.y + d.ay * d.step, k.lo[1], k.hi[1]);
         r.z = rclamp(pi.z + d.az * d.step, k.lo[2], k.hi[2]);
