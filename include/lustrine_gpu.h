@@ -180,6 +180,13 @@ int lus_pbf_set_tiling(LusGpu* ctx, int tiled, int zt);
  * Same results bit for bit.  No reference counterpart (launch configuration;
  * the scan replaced is src/neighbors/Neighbors.cpp:306-361). */
 int lus_pbf_set_build(LusGpu* ctx, int staged);
+/* Lanes per particle in the list walks (lambda, delta-p, XSPH) of fused plain
+ * steps: 0 (default) = 4 for scenes of at most 131,072 particles, else 1;
+ * 1 or 4 forces it.  With 4 lanes each walks every fourth group of eight
+ * entries and the partial sums are combined in a fixed order: deterministic,
+ * not in the oracle's summation order (fused mode only; exact mode always
+ * walks with one lane).  No reference counterpart (launch configuration). */
+int lus_pbf_set_lanes(LusGpu* ctx, int lanes);
 /* Threads per block (64, 128 or 256; default 256) of the per-slot neighbour
  * passes.  Same results.  No reference counterpart (launch configuration). */
 int lus_pbf_set_launch(LusGpu* ctx, int block);

@@ -205,6 +205,7 @@ struct PbfState {
     int nred = 0;          // block-reduction entries the last step wrote
     int fblock = lus::kBlock;  // threads per block of the flat neighbour passes
     bool wsbuild = true;       // flat list build with wave-staged candidates
+    int lanes = 0;             // lanes per particle of the list walks (0: automatic)
 };
 
 // the network of the learned step (device copies), kept across scene inits
@@ -388,6 +389,10 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
     }
     ta.block = tx.block = P.fblock;
     ta.stage = P.wsbuild ? 1 : 0;
+    // small fused plain scenes: several lanes per particle in the list walks
+    if (!ta.tiles && !P.mixed && !P.learned && !rec && !k.exact &&
+        (P.lanes == lus::kSubLanes || (P.lanes == 0 && P.n <= lus::kSubMaxN)))
+        ta.sub = tx.sub = lus::kSubLanes;
     // block-reduction entries the passes write
     const int nred = ta.tiles ? P.tgrid : lus::grid_for(P.n, P.fblock);
     const int K = P.p.iterations;
@@ -501,7 +506,9 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
             LUS_HIP(mark(10));
         }
     }
-    P.nred = nred;
+    // the last lambda pass's block maxima: a list walk with ta.sub lanes per
+    // particle for K >= 2 (the first pass builds the lists, one lane each)
+    P.nred = (ta.sub > 1 && K >= 2) ? lus::grid_for(P.n * ta.sub) : nred;
     std::swap(P.x, P.x2);
     std::swap(P.id, P.id2);
     std::swap(P.sid, P.sid2);
@@ -1136,6 +1143,17 @@ int lus_pbf_set_build(LusGpu* c, int staged) {
     LUS_HIP(hipStreamSynchronize(c->stream));
     pbf_drop_graph(c->pbf);
     c->pbf.wsbuild = staged != 0;
+    return LUS_OK;
+}
+
+int lus_pbf_set_lanes(LusGpu* c, int lanes) {
+    if (!c) return set_error(LUS_EINVAL, "lus_pbf_set_lanes: NULL context");
+    if (lanes != 0 && lanes != 1 && lanes != lus::kSubLanes)
+        return set_error(LUS_EINVAL, "lus_pbf_set_lanes: lanes must be 0, 1 or %d", lus::kSubLanes);
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    pbf_drop_graph(c->pbf);
+    c->pbf.lanes = lanes;
     return LUS_OK;
 }
 

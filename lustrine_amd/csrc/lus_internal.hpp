@@ -154,7 +154,12 @@ struct TileArgs {
     int lcap = 0;                 // staged float4 per staged array
     int block = kBlock;           // threads per block of a flat launch (64, 128 or 256)
     int stage = 1;                // flat list build: wave-staged candidates (0: per-lane gathers)
+    int sub = 1;                  // lanes per particle of the list walks (small fused scenes: kSubLanes)
 };
+// A fused plain scene of at most kSubMaxN particles walks its lists with
+// kSubLanes lanes per particle (lus_pbf.hip, "Small scenes").
+constexpr int kSubLanes = 4;
+constexpr int kSubMaxN = 131072;
 int tile_chunks(const PbfK& k, int zt);
 // tcount: tile_chunks + 1 ints (last one 0); toff: the same; scan_tiles:
 // scan_tile_count(tile_chunks + 1) + 1; tiles: n / kTileT + tile_chunks int4.

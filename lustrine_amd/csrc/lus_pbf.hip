@@ -42,6 +42,16 @@ namespace lus {
 
 constexpr int kOvf = 1 << 30;  // keys are < 2^30 (checked at init)
 
+// the small-scene list walks (kSubLanes lanes per particle), defined below
+hipError_t launch_lambda_sub(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl, int* maxc,
+                             unsigned long long* masks, const float4* xp, const int* gate, hipStream_t s);
+hipError_t launch_deltap_sub(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
+                             const float4* xold, float4* vout, float dt, unsigned long long* masks,
+                             const float4* xp, const int* gate, hipStream_t s);
+hipError_t launch_xsph_sub(const float4* xa, const float4* vt, int* key2, const int* cs, int n, PbfK k,
+                           float4* vout, float4* xout, unsigned long long* masks, const float4* xp,
+                           const int* gate, hipStream_t s);
+
 // clamp box and cell y offset of a particle: the scene table entry of a batch,
 // or the single scene's bounds
 struct Bounds {
@@ -1263,6 +1273,7 @@ hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, 
                                gate, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
+    if (!build && ta.sub > 1 && !k.exact) return launch_lambda_sub(xa, key2, cs, n, k, xl, maxc, masks, xp, gate, s);
     const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (build && ta.stage && xa == xp && n > 0 && k.nph == 1) {
         if (k.exact)
@@ -1349,16 +1360,11 @@ struct DeltapAcc {
     }
 };
 
-// delta-p pass: the sums of DeltapAcc, Jacobi update, box clamp
-template <bool kFast, int kNS, class Visit>
-__device__ __forceinline__ void deltap_body(int i, const float4* __restrict__ xl, const PbfK& k,
-                                            float4* __restrict__ xa, const float4* __restrict__ xold,
-                                            float4* __restrict__ vout, float dt, Visit&& visit) {
-    const float4 pi = xl[i];
-    const float li = pi.w;
-    DeltapAcc<kFast, kNS> d(k);
-    visit([&](int, const float4 pj, const float4) { d.add(pi, li, pj, k); });
-    const float ax = d.ax, ay = d.ay, az = d.az, step = d.step;
+// x* = clamp(x* + (ax, ay, az) step) and, when vout is set, v = (x* - x) / dt
+__device__ __forceinline__ void deltap_commit(int i, const float4 pi, float ax, float ay, float az, float step,
+                                              const PbfK& k, float4* __restrict__ xa,
+                                              const float4* __restrict__ xold, float4* __restrict__ vout,
+                                              float dt) {
     const Bounds b = bounds_of(k, k.scenes ? k.sid_out[i] : 0);
     float4 r;
     r.x = rclamp(pi.x + ax * step, b.lo[0], b.hi[0]);
@@ -1370,6 +1376,18 @@ __device__ __forceinline__ void deltap_body(int i, const float4* __restrict__ xl
         const float4 xo = xold[i];
         vout[i] = make_float4((r.x - xo.x) / dt, (r.y - xo.y) / dt, (r.z - xo.z) / dt, 0.0f);
     }
+}
+
+// delta-p pass: the sums of DeltapAcc, Jacobi update, box clamp
+template <bool kFast, int kNS, class Visit>
+__device__ __forceinline__ void deltap_body(int i, const float4* __restrict__ xl, const PbfK& k,
+                                            float4* __restrict__ xa, const float4* __restrict__ xold,
+                                            float4* __restrict__ vout, float dt, Visit&& visit) {
+    const float4 pi = xl[i];
+    const float li = pi.w;
+    DeltapAcc<kFast, kNS> d(k);
+    visit([&](int, const float4 pj, const float4) { d.add(pi, li, pj, k); });
+    deltap_commit(i, pi, d.ax, d.ay, d.az, d.step, k, xa, xold, vout, dt);
 }
 
 template <bool kFast, int kNS>
@@ -1438,6 +1456,7 @@ hipError_t launch_pbf_deltap(const float4* xl, int* key2, const int* cs, int n, 
                                xp, gate, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
+    if (ta.sub > 1 && !k.exact) return launch_deltap_sub(xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate, s);
     const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (k.exact && n4)
         launch_kernel((k_pbf_deltap<false, 4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
@@ -1554,6 +1573,190 @@ __global__ __launch_bounds__(kTileT) void k_pbf_xsph_t(const float4* __restrict_
     if (kCheck) block_max_sum_store(cmax, csum, maxd);
 }
 
+// ---------------------------------------------------------------------------
+// Small scenes (fused arithmetic): the list walks of the plain step with kSub
+// lanes per particle.  A scene of tens of thousands of particles fills a
+// fraction of the GPU's wave slots, and each pass takes as long as one
+// wave's walk of its longest list; kSub lanes walking every kSub-th group of
+// eight entries shorten that walk kSub times.  The partial sums are combined
+// in a fixed order ((s0 + s1) + (s2 + s3), the same in every lane), so the
+// results are deterministic, but not in the oracle's order: only fused
+// mode uses these kernels.  A particle whose list overflowed is walked by
+// its first lane alone (the re-test path).
+// ---------------------------------------------------------------------------
+constexpr int kSub = kSubLanes;
+static_assert(kSub == 4, "sub_sum combines four lanes");
+
+// Walk the groups sub, sub + kSub, ... of the list of header hd
+// (|N(i)| <= kNbrCap), calling f(j, cur[j], aux[j]) in list order within
+// each group; the eight gathers of a group are in flight together.
+template <class F>
+__device__ __forceinline__ void visit_list_sub(const int4 hd, const int4* __restrict__ lst,
+                                               const float4* __restrict__ cur, const float4* __restrict__ aux,
+                                               int sub, F& f) {
+    const int c = hd.w & 0xFFFF;
+    const int n0 = (hd.w >> 16) & 0xFF, n01 = (int)((unsigned int)hd.w >> 24);
+    auto slot = [&](int t, unsigned int e) { return (t < n0 ? hd.x : (t < n01 ? hd.y : hd.z)) + (int)e; };
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int g = sub; g * 8 < c; g += kSub) {
+        const int4 q = lst[(size_t)g * 64];
+        const int t = g * 8;
+        const unsigned int w0 = (unsigned int)q.x, w1 = (unsigned int)q.y;
+        const unsigned int w2 = (unsigned int)q.z, w3 = (unsigned int)q.w;
+        const int j[8] = {slot(t, w0 & 0xFFFFu), slot(t + 1, w0 >> 16), slot(t + 2, w1 & 0xFFFFu),
+                          slot(t + 3, w1 >> 16), slot(t + 4, w2 & 0xFFFFu), slot(t + 5, w2 >> 16),
+                          slot(t + 6, w3 & 0xFFFFu), slot(t + 7, w3 >> 16)};
+        float4 p[8], a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            p[u] = at4(cur, j[u]);
+            a[u] = aux ? at4(aux, j[u]) : z;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (t + u < c) f(j[u], p[u], a[u]);
+    }
+}
+
+// the sum of v over the kSub lanes of a particle, (s0 + s1) + (s2 + s3) in every lane
+__device__ __forceinline__ float sub_sum(float v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    return v;
+}
+
+// N(i) for lane sub of particle i: its share of the list, or the whole re-test
+// walk for the first lane
+template <class F>
+__device__ __forceinline__ void for_each_neighbor_sub(int i, int sub, int n, int* __restrict__ key2,
+                                                      const int* __restrict__ cs,
+                                                      unsigned long long* __restrict__ nb,
+                                                      const float4* __restrict__ xp, const float4* __restrict__ cur,
+                                                      const float4* __restrict__ aux, const PbfK& k, F&& f) {
+    const int4 hd = nbr_hdr(nb)[i];
+    if ((hd.w & 0xFFFF) <= kNbrCap)
+        visit_list_sub(hd, nbr_list(nb, n, i), cur, aux, sub, f);
+    else if (sub == 0)
+        for_each_neighbor<false>(i, n, key2, cs, nb, xp, cur, aux, k, f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pbf_lambda_sub(const float4* __restrict__ xa,
+                                                            int* __restrict__ key2, const int* __restrict__ cs,
+                                                            int n, PbfK k, float4* __restrict__ xl,
+                                                            int* __restrict__ maxc,
+                                                            unsigned long long* __restrict__ nb,
+                                                            const float4* __restrict__ xp,
+                                                            const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;  // uniform: before any barrier
+    const int t = xcd_index();
+    const int i = t / kSub, sub = t % kSub;
+    LambdaAcc<true> acc(k);
+    float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < n) {
+        pi = xa[i];
+        for_each_neighbor_sub(i, sub, n, key2, cs, nb, xp, xa, nullptr, k,
+                              [&](int, const float4 pj, const float4) { acc.add(pi, pj, k); });
+    }
+    // every lane takes part in the exchanges
+    acc.rho = sub_sum(acc.rho);
+    acc.a = sub_sum(acc.a);
+    acc.gx = sub_sum(acc.gx);
+    acc.gy = sub_sum(acc.gy);
+    acc.gz = sub_sum(acc.gz);
+    float Ci = -1e30f;
+    if (i < n && sub == 0) Ci = acc.finish(pi, xl + i, k);
+    block_max_store(Ci, maxc);
+}
+
+template <int kNS>
+__global__ __launch_bounds__(kBlock) void k_pbf_deltap_sub(const float4* __restrict__ xl,
+                                                            int* __restrict__ key2, const int* __restrict__ cs,
+                                                            int n, PbfK k, float4* __restrict__ xa,
+                                                            const float4* __restrict__ xold,
+                                                            float4* __restrict__ vout, float dt,
+                                                            unsigned long long* __restrict__ nb,
+                                                            const float4* __restrict__ xp,
+                                                            const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int t = xcd_index();
+    const int i = t / kSub, sub = t % kSub;
+    DeltapAcc<true, kNS> d(k);
+    float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < n) {
+        pi = xl[i];
+        const float li = pi.w;
+        for_each_neighbor_sub(i, sub, n, key2, cs, nb, xp, xl, nullptr, k,
+                              [&](int, const float4 pj, const float4) { d.add(pi, li, pj, k); });
+    }
+    const float ax = sub_sum(d.ax), ay = sub_sum(d.ay), az = sub_sum(d.az);
+    if (i < n && sub == 0) deltap_commit(i, pi, ax, ay, az, d.step, k, xa, xold, vout, dt);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pbf_xsph_sub(const float4* __restrict__ xa,
+                                                          const float4* __restrict__ vt,
+                                                          int* __restrict__ key2, const int* __restrict__ cs,
+                                                          int n, PbfK k, float4* __restrict__ vout,
+                                                          float4* __restrict__ xout,
+                                                          unsigned long long* __restrict__ nb,
+                                                          const float4* __restrict__ xp,
+                                                          const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int t = xcd_index();
+    const int i = t / kSub, sub = t % kSub;
+    float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = pi;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    if (i < n) {
+        pi = xa[i];
+        vi = vt[i];
+        for_each_neighbor_sub(i, sub, n, key2, cs, nb, xp, xa, vt, k, [&](int, const float4 pj, const float4 vj) {
+            const float dx = pi.x - pj.x;
+            const float dy = pi.y - pj.y;
+            const float dz = pi.z - pj.z;
+            const float r2 = dist2<true>(dx, dy, dz);
+            if (r2 <= k.h2) {
+                const float tt = poly_t<true>(r2, k);
+                const float w = (tt * tt) * tt;
+                sx = acc<true>(sx, vj.x - vi.x, w);
+                sy = acc<true>(sy, vj.y - vi.y, w);
+                sz = acc<true>(sz, vj.z - vi.z, w);
+            }
+        });
+    }
+    sx = sub_sum(sx);
+    sy = sub_sum(sy);
+    sz = sub_sum(sz);
+    if (i < n && sub == 0) {
+        vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
+        xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
+    }
+}
+
+hipError_t launch_lambda_sub(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl, int* maxc,
+                             unsigned long long* masks, const float4* xp, const int* gate, hipStream_t s) {
+    launch_kernel(k_pbf_lambda_sub, dim3(grid_for(n * kSub)), dim3(kBlock), 0, s, xa, key2, cs, n, k, xl, maxc,
+                  masks, xp, gate);
+    return hipGetLastError();
+}
+
+hipError_t launch_deltap_sub(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
+                             const float4* xold, float4* vout, float dt, unsigned long long* masks,
+                             const float4* xp, const int* gate, hipStream_t s) {
+    const dim3 g(grid_for(n * kSub)), b(kBlock);
+    if (k.n_scorr == 4)
+        launch_kernel((k_pbf_deltap_sub<4>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
+    else
+        launch_kernel((k_pbf_deltap_sub<0>), g, b, 0, s, xl, key2, cs, n, k, xa, xold, vout, dt, masks, xp, gate);
+    return hipGetLastError();
+}
+
+hipError_t launch_xsph_sub(const float4* xa, const float4* vt, int* key2, const int* cs, int n, PbfK k,
+                           float4* vout, float4* xout, unsigned long long* masks, const float4* xp,
+                           const int* gate, hipStream_t s) {
+    launch_kernel(k_pbf_xsph_sub, dim3(grid_for(n * kSub)), dim3(kBlock), 0, s, xa, vt, key2, cs, n, k, vout, xout,
+                  masks, xp, gate);
+    return hipGetLastError();
+}
+
 hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
                            PbfK k, float4* vout, float4* xout, unsigned long long* masks,
                            const float4* xp, int* maxd, const int* gate, const TileArgs& ta, hipStream_t s) {
@@ -1574,6 +1777,7 @@ hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const 
                                xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
+    if (ta.sub > 1 && !k.exact && !maxd) return launch_xsph_sub(xa, vt, key2, cs, n, k, vout, xout, masks, xp, gate, s);
     const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (k.exact && maxd)
         launch_kernel((k_pbf_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);

@@ -103,6 +103,7 @@ _SIGS = {
     "lus_pbf_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "lus_pbf_set_build": (C.c_int, [C.c_void_p, C.c_int]),
+    "lus_pbf_set_lanes": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_set_launch": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_step_latency": (C.c_int, [C.c_void_p, C.c_float, C.c_int, _f]),
     "lus_pbf_download": (C.c_int, [C.c_void_p, C.c_int, _f, _f, _i]),
@@ -404,6 +405,10 @@ class Gpu:
         """staged: the list build tests wave-staged candidates in LDS; else
         per-lane gathers (same results)."""
         _check(self._lib.lus_pbf_set_build(self._h, int(bool(staged))), "lus_pbf_set_build")
+
+    def pbf_set_lanes(self, lanes=0):
+        """Lanes per particle in fused list walks: 0 automatic, 1 or 4."""
+        _check(self._lib.lus_pbf_set_lanes(self._h, int(lanes)), "lus_pbf_set_lanes")
 
     def pbf_step_latency(self, dt, steps):
         ms = np.zeros(steps, np.float32)

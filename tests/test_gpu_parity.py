@@ -486,3 +486,47 @@ def test_pbf_staged_build_matches_gathers(gpu_mod, scene, exact):
             out.append(g.pbf_download(order=0) + (g.pbf_last_max_constraint(), g.pbf_retest_count()))
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def test_pbf_four_lane_walks(gpu_mod, oracle):
+    # fused steps of a small scene walk each list with four lanes (every
+    # fourth group of eight entries, partial sums combined in a fixed order):
+    # against the one-lane walks and the oracle, within the fused bars; a
+    # dense block (lists past 128 entries: the first lane's re-test walk)
+    # against the oracle
+    cfg = scenes.cfg0()
+    x0 = cfg.positions()
+    out = []
+    with gpu_mod.Gpu(0) as g:
+        for lanes in (1, 4):
+            g.pbf_init(gpu_mod.pbf_params(cfg, exact=False), x0)
+            g.pbf_set_lanes(lanes)
+            g.pbf_step(np.float32(cfg.dt), 3)
+            out.append(g.pbf_download(order=1))
+    x1, v1, _ = out[0]
+    x4, v4, _ = out[1]
+    assert rel_err(x4, x1) <= 1e-6
+    x, v = x0.copy(), np.zeros_like(x0)
+    ids = np.arange(cfg.n, dtype=np.int32)
+    p = oracle.pbf_params(cfg)
+    for _ in range(3):
+        oracle.pbf_step(p, x, v, ids, np.float32(cfg.dt))
+    xo, vo = np.zeros_like(x), np.zeros_like(v)
+    xo[ids], vo[ids] = x, v
+    assert rel_err(x4, xo) <= REL
+    print(f"4 lanes 32k x3: x vs 1 lane {rel_err(x4, x1):.2e}, vs oracle {rel_err(x4, xo):.2e}")
+
+    dcfg = scenes.small(8, 8, 8, box=(24, 24, 24), origin=(8.0, 8.0, 8.0))
+    d0 = scenes.jittered_lattice(8, 8, 8, cell=0.5, origin=(8.0, 8.0, 8.0), amplitude=0.01, seed=3)
+    with gpu_mod.Gpu(0) as g:
+        g.pbf_init(gpu_mod.pbf_params(dcfg, exact=False), d0)
+        g.pbf_set_lanes(4)
+        g.pbf_step(np.float32(dcfg.dt), 1)
+        gx, gv, gid = g.pbf_download(order=0)
+        retest = g.pbf_retest_count()
+    x, v = d0.copy(), np.zeros_like(d0)
+    ids = np.arange(d0.shape[0], dtype=np.int32)
+    oracle.pbf_step(oracle.pbf_params(dcfg), x, v, ids, np.float32(dcfg.dt))
+    assert retest > 0
+    assert np.array_equal(gid, ids)
+    assert rel_err(gx, x) <= REL

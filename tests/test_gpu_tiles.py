@@ -13,6 +13,7 @@ pytestmark = pytest.mark.gpu
 def _run(gpu_mod, cfg, x0, steps, exact, tiled, zt=16, graph=False):
     with gpu_mod.Gpu(0) as g:
         g.pbf_init(gpu_mod.pbf_params(cfg, exact=exact), x0)
+        g.pbf_set_lanes(1)  # the tiled passes walk with one lane per particle
         g.pbf_set_tiling(tiled, zt)
         g.pbf_set_graph(graph)
         g.pbf_step(np.float32(cfg.dt), steps)
@@ -103,6 +104,7 @@ def test_block_sizes_equal_default(gpu_mod, block, exact):
     ref = _run(gpu_mod, cfg, x0, 4, exact, False)
     with gpu_mod.Gpu(0) as g:
         g.pbf_init(gpu_mod.pbf_params(cfg, exact=exact), x0)
+        g.pbf_set_lanes(1)  # as _run: the block size of the one-lane walks
         g.pbf_set_launch(block)
         g.pbf_set_graph(True)
         g.pbf_step(np.float32(cfg.dt), 4)
