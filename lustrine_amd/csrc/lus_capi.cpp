@@ -506,9 +506,9 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
             LUS_HIP(mark(10));
         }
     }
-    // the last lambda pass's block maxima: a list walk with ta.sub lanes per
-    // particle for K >= 2 (the first pass builds the lists, one lane each)
-    P.nred = (ta.sub > 1 && K >= 2) ? lus::grid_for(P.n * ta.sub) : nred;
+    // the last lambda pass's block maxima: ta.sub lanes per particle (the
+    // building pass and the list walks alike)
+    P.nred = ta.sub > 1 ? lus::grid_for(P.n * ta.sub) : nred;
     std::swap(P.x, P.x2);
     std::swap(P.id, P.id2);
     std::swap(P.sid, P.sid2);

@@ -42,7 +42,9 @@ namespace lus {
 
 constexpr int kOvf = 1 << 30;  // keys are < 2^30 (checked at init)
 
-// the small-scene list walks (kSubLanes lanes per particle), defined below
+// the small-scene list build and walks (kSubLanes lanes per particle), defined below
+hipError_t launch_lambda_build_sub(const float4* xp, const int* key2, const int* cs, int n, PbfK k, float4* xl,
+                                   int* maxc, unsigned long long* masks, const int* gate, hipStream_t s);
 hipError_t launch_lambda_sub(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl, int* maxc,
                              unsigned long long* masks, const float4* xp, const int* gate, hipStream_t s);
 hipError_t launch_deltap_sub(const float4* xl, int* key2, const int* cs, int n, PbfK k, float4* xa,
@@ -1274,6 +1276,8 @@ hipError_t launch_pbf_lambda(const float4* xa, int* key2, const int* cs, int n, 
         return hipGetLastError();
     }
     if (!build && ta.sub > 1 && !k.exact) return launch_lambda_sub(xa, key2, cs, n, k, xl, maxc, masks, xp, gate, s);
+    if (build && ta.sub > 1 && !k.exact && xa == xp && k.nph == 1 && n > 0)
+        return launch_lambda_build_sub(xp, key2, cs, n, k, xl, maxc, masks, gate, s);
     const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (build && ta.stage && xa == xp && n > 0 && k.nph == 1) {
         if (k.exact)
@@ -1729,6 +1733,133 @@ __global__ __launch_bounds__(kBlock) void k_pbf_xsph_sub(const float4* __restric
         vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
         xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
     }
+}
+
+// The list build of a small fused scene with one lane per stencil layer
+// (rows 3L..3L+2; the fourth lane of a particle idles).  A first walk over
+// the layer's candidates counts the hits, notes the layer's first and last
+// neighbour and sums lambda; the lanes exchange their counts, and a second
+// walk stores each hit's 16-bit entry at its place in the list (the same
+// entries in the same order as the one-lane build) with two-byte stores.
+// The last layer's lane zeroes the places past the count in the last group
+// (a walk decodes them, to slot b2).
+__global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_sub(const float4* __restrict__ xp,
+                                                                  const int* __restrict__ key2,
+                                                                  const int* __restrict__ cs, int n, PbfK k,
+                                                                  float4* __restrict__ xl,
+                                                                  int* __restrict__ maxc,
+                                                                  unsigned long long* __restrict__ nb,
+                                                                  const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;  // uniform: before any barrier
+    const int t = xcd_index();
+    const int i = t / kSub, layer = t % kSub;
+    const bool mine = i < n && layer < 3;
+    LambdaAcc<true> acc(k);
+    float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    int cnt = 0, first = -1, last = -1;
+    int cy = 0, cx = 0, cz = 0;
+    if (mine) {
+        key_cell(key2[i] & ~kOvf, k, cy, cx, cz);
+        pi = xp[i];
+    }
+    const int z0 = cz > 0 ? cz - 1 : 0;
+    const int z1 = cz + 1 < k.gZ ? cz + 1 : k.gZ - 1;
+    const int y = cy + layer - 1;
+    const bool yok = mine && y >= 0 && y < k.gY;
+    // the layer's candidate rows (x - 1, x, x + 1), ascending slots; a row of
+    // at most 64 slots keeps its hits as a mask (row_m, from row_j0) for the
+    // second walk, a longer one is tested again there
+    unsigned long long row_m[3] = {0ull, 0ull, 0ull};
+    int row_j0[3] = {0, 0, 0}, row_j1[3] = {0, 0, 0};
+    if (yok) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int x = cx + d - 1;
+            if (x < 0 || x >= k.gX) continue;
+            const int row = row_key(y, x, 0, k);
+            row_j0[d] = cs[row + z0];
+            row_j1[d] = cs[row + z1 + 1];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int j0 = row_j0[d], j1 = row_j1[d];
+        unsigned long long m = 0ull;
+        int j = j0;
+        for (; j + 4 <= j1; j += 4) {  // four candidates' loads in flight
+            const float3 c[4] = {xyz_at(xp, j), xyz_at(xp, j + 1), xyz_at(xp, j + 2), xyz_at(xp, j + 3)};
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (within_h(pi, c[u], k)) {
+                    const int jj = j + u;
+                    first = first < 0 ? jj : first;
+                    last = jj;
+                    ++cnt;
+                    acc.add(pi, make_float4(c[u].x, c[u].y, c[u].z, 0.0f), k);
+                    if (jj - j0 < 64) m |= 1ull << (jj - j0);
+                }
+        }
+        for (; j < j1; ++j) {
+            const float3 c = xyz_at(xp, j);
+            if (within_h(pi, c, k)) {
+                first = first < 0 ? j : first;
+                last = j;
+                ++cnt;
+                acc.add(pi, make_float4(c.x, c.y, c.z, 0.0f), k);
+                if (j - j0 < 64) m |= 1ull << (j - j0);
+            }
+        }
+        row_m[d] = m;
+    }
+    // the particle's three layers (lanes base .. base + 2)
+    const int base = (int)(threadIdx.x & 63) & ~(kSub - 1);
+    const int c0 = __shfl(cnt, base, 64), c1 = __shfl(cnt, base + 1, 64), c2 = __shfl(cnt, base + 2, 64);
+    const int f0 = __shfl(first, base, 64), f1 = __shfl(first, base + 1, 64), f2 = __shfl(first, base + 2, 64);
+    const int wide = (mine && first >= 0 && last - first > 0xFFFF) ? 1 : 0;
+    const int anywide = __shfl(wide, base, 64) | __shfl(wide, base + 1, 64) | __shfl(wide, base + 2, 64);
+    const int c = c0 + c1 + c2;
+    const bool retest = c > kNbrCap || anywide;
+    acc.rho = sub_sum(acc.rho);
+    acc.a = sub_sum(acc.a);
+    acc.gx = sub_sum(acc.gx);
+    acc.gy = sub_sum(acc.gy);
+    acc.gz = sub_sum(acc.gz);
+    if (mine && !retest) {
+        int4* __restrict__ lst = nbr_list(nb, n, i);
+        // entry p of the list: group p / 8, 16-bit place p % 8 of the int4
+        auto place = [&](int p) {
+            return reinterpret_cast<unsigned short*>(lst + (size_t)(p >> 3) * 64) + (p & 7);
+        };
+        int p = layer == 0 ? 0 : (layer == 1 ? c0 : c0 + c1);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int j0 = row_j0[d], j1 = row_j1[d];
+            if (j1 - j0 <= 64) {
+                for (unsigned long long m = row_m[d]; m; m &= m - 1ull)
+                    *place(p++) = (unsigned short)(j0 + low_bit(m) - first);
+            } else {
+                for (int j = j0; j < j1; ++j)
+                    if (within_h(pi, xyz_at(xp, j), k)) *place(p++) = (unsigned short)(j - first);
+            }
+        }
+        if (layer == 2)
+            for (int q = c; (q & 7) != 0; ++q) *place(q) = 0;
+    }
+    float Ci = -1e30f;
+    if (i < n && layer == 0) {
+        Ci = acc.finish(pi, xl + i, k);
+        const int b0 = f0 < 0 ? i : f0, b1 = f1 < 0 ? i : f1, b2 = f2 < 0 ? i : f2;
+        const int w = retest ? kNbrRetest : (c | (c0 << 16) | ((c0 + c1) << 24));
+        nbr_hdr(nb)[i] = make_int4(b0, b1, b2, w);
+    }
+    block_max_store(Ci, maxc);
+}
+
+hipError_t launch_lambda_build_sub(const float4* xp, const int* key2, const int* cs, int n, PbfK k, float4* xl,
+                                   int* maxc, unsigned long long* masks, const int* gate, hipStream_t s) {
+    launch_kernel(k_pbf_lambda_build_sub, dim3(grid_for(n * kSub)), dim3(kBlock), 0, s, xp, key2, cs, n, k, xl,
+                  maxc, masks, gate);
+    return hipGetLastError();
 }
 
 hipError_t launch_lambda_sub(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl, int* maxc,
