@@ -980,28 +980,53 @@ __device__ __forceinline__ void visit_row_staged(M mm, int j0, int lo, const lds
 __device__ __forceinline__ int high_bit(unsigned int m) { return 31 - __builtin_clz(m); }
 __device__ __forceinline__ int high_bit(unsigned long long m) { return 63 - __builtin_clzll(m); }
 
-// Hit mask of the L (<= bits of M) staged candidates S[lo + t] of pi: four
-// tests make a nibble of inline-constant bits, shifted into the mask once
+// o shifted left by one with bit 0 = "staged candidate p is outside h of pi":
+// the sign of h2 - r2, with r2 exactly as within_h forms it (the rounded
+// difference of two floats has the sign of their exact difference and is +0
+// when they are equal, so the bit is 1 exactly when r2 > h2).  One funnel
+// shift replaces the compare, the select and the or of a mask update.
+template <int kS>
+__device__ __forceinline__ unsigned int shift_outside(unsigned int o, const float4 pi, const lds_float* __restrict__ p,
+                                                      const PbfK& k) {
+    const float dx = pi.x - p[0];
+    const float dy = pi.y - p[1];
+    const float dz = pi.z - p[2];
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    return __builtin_amdgcn_alignbit(o, __float_as_uint(k.h2 - r2), 31u);
+}
+
+// Hit mask of the cnt (<= 32) staged candidates b[t]: shifted in from the
+// last to the first, so candidate t lands in bit t; the all-ones start leaves
+// ones (no hit) above bit cnt - 1.
+template <int kS>
+__device__ __forceinline__ unsigned int hits32_staged(const float4 pi, const lds_float* __restrict__ b, int cnt,
+                                                      const PbfK& k) {
+    unsigned int o = ~0u;
+    int t = cnt;
+    for (; t >= 4; t -= 4) {
+        const lds_float* __restrict__ q = b + kS * (t - 4);
+        o = shift_outside<kS>(o, pi, q + 3 * kS, k);
+        o = shift_outside<kS>(o, pi, q + 2 * kS, k);
+        o = shift_outside<kS>(o, pi, q + kS, k);
+        o = shift_outside<kS>(o, pi, q, k);
+    }
+    for (; t > 0; --t) o = shift_outside<kS>(o, pi, b + kS * (t - 1), k);
+    return ~o;
+}
+
+// Hit mask of the L (<= bits of M) staged candidates S[lo + t] of pi
 template <typename M, int kS>
 __device__ __forceinline__ M test_row_staged(const float4 pi, int lo, int L, const lds_float* __restrict__ S,
                                              const PbfK& k) {
-    M mm = (M)0;
-    int t = 0;
     const lds_float* __restrict__ b = S + kS * lo;
-    for (; t + 4 <= L; t += 4, b += 4 * kS) {
-        const float4 c0 = make_float4(b[0], b[1], b[2], 0.0f);
-        const float4 c1 = make_float4(b[kS], b[kS + 1], b[kS + 2], 0.0f);
-        const float4 c2 = make_float4(b[2 * kS], b[2 * kS + 1], b[2 * kS + 2], 0.0f);
-        const float4 c3 = make_float4(b[3 * kS], b[3 * kS + 1], b[3 * kS + 2], 0.0f);
-        const unsigned int nib = (within_h(pi, c0, k) ? 1u : 0u) | (within_h(pi, c1, k) ? 2u : 0u) |
-                                 (within_h(pi, c2, k) ? 4u : 0u) | (within_h(pi, c3, k) ? 8u : 0u);
-        mm |= (M)nib << t;
+    if constexpr (sizeof(M) == 4) {
+        return hits32_staged<kS>(pi, b, L, k);
+    } else {
+        const int nl = L < 32 ? L : 32;
+        const unsigned int l = hits32_staged<kS>(pi, b, nl, k);
+        const unsigned int h = hits32_staged<kS>(pi, b + 32 * kS, L - nl, k);
+        return (M)h << 32 | (M)l;
     }
-    for (; t < L; ++t, b += kS) {
-        const float4 c = make_float4(b[0], b[1], b[2], 0.0f);
-        mm |= (M)(within_h(pi, c, k) ? 1u : 0u) << t;
-    }
-    return mm;
 }
 
 // LDS accesses of one wave complete, and are seen by all its lanes, before
