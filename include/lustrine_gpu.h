@@ -187,6 +187,12 @@ int lus_pbf_set_build(LusGpu* ctx, int staged);
  * not in the oracle's summation order (fused mode only; exact mode always
  * walks with one lane).  No reference counterpart (launch configuration). */
 int lus_pbf_set_lanes(LusGpu* ctx, int lanes);
+/* Sort tile of the step's counting sort: cells per block of its prefix scan
+ * over the cell histogram, 1024, 2048 (default) or 4096.  Same sorted order.
+ * Kept across scene inits; chosen per size bucket by the autotuner.  The scan
+ * replaced is the inclusive prefix of Sorting::counting_sort
+ * (src/neighbors/Sorting.cpp:11-33). */
+int lus_pbf_set_sort_tile(LusGpu* ctx, int cells);
 /* Threads per block (64, 128 or 256; default 256) of the per-slot neighbour
  * passes.  Same results.  No reference counterpart (launch configuration). */
 int lus_pbf_set_launch(LusGpu* ctx, int block);

@@ -2,19 +2,21 @@
 
 Candidates: threads per block of the neighbour passes (64, 128, 256), cell
 size (h or 1.125 h; any cell size >= h gives the same neighbour sets), and the
-neighbour-pass tiling (off, or on with chunks of `zt` cells).  Each candidate
-is timed on the scene itself: the scene is re-initialised from the same state
-for every candidate, stepped `warmup` steps untimed and `steps` steps timed
-(hipEvents on the library's stream), so all candidates see the same simulation
-window; the fastest wins.  Choices are kept per size bucket (floor(log2 of the
-particle count)) in memory and, when a path is given, in a JSON file.
+neighbour-pass tiling (off, or on with chunks of `zt` cells); then, for the
+fastest of those, the sort tile (cells per block of the counting sort's prefix
+scan: 1024, 2048 or 4096).  Each candidate is timed on the scene itself: the
+scene is re-initialised from the same state for every candidate, stepped
+`warmup` steps untimed and `steps` steps timed (hipEvents on the library's
+stream), so all candidates see the same simulation window; the fastest wins.
+Choices are kept per size bucket (floor(log2 of the particle count)) in memory
+and, when a path is given, in a JSON file.
 """
 from __future__ import annotations
 
 import json
 import math
 import os
-from dataclasses import asdict, dataclass
+from dataclasses import asdict, dataclass, replace
 
 import numpy as np
 
@@ -27,9 +29,11 @@ class Launch:
     cell_scale: float = 1.0   # cell size / kernel radius
     tiled: bool = False       # LDS-staged tiles instead of one thread per slot
     zt: int = 32              # cells per tile chunk along z (tiled only)
+    sort_tile: int = 2048     # cells per block of the counting sort's prefix scan
 
 
 CANDIDATES = tuple(Launch(b, c, t) for b in (64, 128, 256) for c in (1.0, 1.125) for t in (False, True))
+SORT_TILES = (1024, 2048, 4096)
 
 
 def bucket(n: int) -> int:
@@ -50,14 +54,16 @@ def cell_params(base: "G.LusPbfParams", box, cell_scale: float) -> "G.LusPbfPara
 def apply(g: "G.Gpu", launch: Launch) -> None:
     g.pbf_set_launch(launch.block)
     g.pbf_set_tiling(launch.tiled, launch.zt)
+    g.pbf_set_sort_tile(launch.sort_tile)
 
 
 class Tuner:
     """Chooses a Launch per size bucket by timing the candidates."""
 
-    def __init__(self, path: str | None = None, candidates=CANDIDATES):
+    def __init__(self, path: str | None = None, candidates=CANDIDATES, sort_tiles=SORT_TILES):
         self.path = path
         self.candidates = tuple(candidates)
+        self.sort_tiles = tuple(sort_tiles)
         self.choices: dict[int, Launch] = {}
         if path and os.path.exists(path):
             with open(path) as f:
@@ -79,12 +85,20 @@ class Tuner:
         if not force and b in self.choices:
             return self.choices[b], []
         table = []
-        for cand in self.candidates:
+
+        def time(cand):
             init(cand)
             apply(g, cand)
             g.pbf_set_graph(True)
             ms, _, _ = g.pbf_bench(dt, warmup, steps, phases=False)
             table.append((cand, ms / steps))
+
+        for cand in self.candidates:
+            time(cand)
+        best = min(table, key=lambda t: t[1])[0]
+        for st in self.sort_tiles:
+            if st != best.sort_tile:
+                time(replace(best, sort_tile=st))
         best = min(table, key=lambda t: t[1])[0]
         self.choices[b] = best
         self.save()

@@ -206,6 +206,7 @@ struct PbfState {
     int fblock = lus::kBlock;  // threads per block of the flat neighbour passes
     bool wsbuild = true;       // flat list build with wave-staged candidates
     int lanes = 0;             // lanes per particle of the list walks (0: automatic)
+    int sort_tile = lus::kScanTileCells;  // cells per block of the sort's prefix scan
 };
 
 // the network of the learned step (device copies), kept across scene inits
@@ -369,7 +370,7 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
     if (ev) LUS_HIP(mark(-1));
     LUS_HIP(lus::launch_pbf_key(P.x, P.v, P.n, k, dt, P.key, P.slot, P.counts, s));
     LUS_HIP(mark(0));
-    LUS_HIP(lus::launch_exclusive_scan(P.counts, P.cs, P.C + 1, P.tiles, 1, s));
+    LUS_HIP(lus::launch_exclusive_scan(P.counts, P.cs, P.C + 1, P.tiles, 1, s, P.sort_tile));
     LUS_HIP(mark(1));
     LUS_HIP(lus::launch_scatter(P.key, P.slot, P.cs, P.n, P.tmp, s));
     LUS_HIP(mark(2));
@@ -914,7 +915,7 @@ int lus_mix_init(LusGpu* c, const LusPbfParams* p, const LusMixParams* mp, int n
     P.k.nph = 3;
     LUS_HIP(P.mem.alloc(&P.counts, P.C + 1));
     LUS_HIP(P.mem.alloc(&P.cs, P.C + 1));
-    LUS_HIP(P.mem.alloc(&P.tiles, lus::scan_tile_count(P.C + 1) + 1));
+    LUS_HIP(P.mem.alloc(&P.tiles, lus::scan_tile_count(P.C + 1, lus::kScanTileMin) + 1));
     LUS_HIP(hipMemset(P.counts, 0, sizeof(int) * (P.C + 1)));
     LUS_HIP(P.mem.alloc(&P.ph, n));
     LUS_HIP(P.mem.alloc(&P.ph2, n));
@@ -989,7 +990,7 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
     LUS_HIP(m.alloc(&P.key2, n));
     LUS_HIP(m.alloc(&P.counts, P.C + 1));
     LUS_HIP(m.alloc(&P.cs, P.C + 1));
-    LUS_HIP(m.alloc(&P.tiles, lus::scan_tile_count(P.C + 1) + 1));
+    LUS_HIP(m.alloc(&P.tiles, lus::scan_tile_count(P.C + 1, lus::kScanTileMin) + 1));
     P.nblk = lus::grid_for(n, 64);  // block-reduction entries of the finest flat launch
     LUS_HIP(m.alloc(&P.maxc, (size_t)P.nmaxc * P.nblk));
     LUS_HIP(m.alloc(&P.maxd, 2 * (size_t)P.nblk));
@@ -1154,6 +1155,17 @@ int lus_pbf_set_lanes(LusGpu* c, int lanes) {
     LUS_HIP(hipStreamSynchronize(c->stream));
     pbf_drop_graph(c->pbf);
     c->pbf.lanes = lanes;
+    return LUS_OK;
+}
+
+int lus_pbf_set_sort_tile(LusGpu* c, int cells) {
+    if (!c) return set_error(LUS_EINVAL, "lus_pbf_set_sort_tile: NULL context");
+    if (!lus::scan_tile_valid(cells))
+        return set_error(LUS_EINVAL, "lus_pbf_set_sort_tile: cells must be 1024, 2048 or 4096");
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    pbf_drop_graph(c->pbf);
+    c->pbf.sort_tile = cells;
     return LUS_OK;
 }
 

@@ -91,10 +91,14 @@ FluidState*& ctx_fluid(LusGpu* c);
 
 // ---- launchers (lus_kernels.hip); all enqueue on `s` and return hipError_t ----
 // prefix scan of m ints: out[k] = sum in[0..k); zero_in clears `in` after use.
-// tile_sums needs scan_tile_count(m) + 1 ints.
-int scan_tile_count(int m);
+// tile: the sort tile, ints per scan block (1024, 2048 or 4096); tile_sums
+// needs scan_tile_count(m, tile) + 1 ints.
+constexpr int kScanTileCells = 2048;
+constexpr int kScanTileMin = 1024;
+int scan_tile_count(int m, int tile = kScanTileCells);
+bool scan_tile_valid(int tile);
 hipError_t launch_exclusive_scan(const int* in, int* out, int m, int* tile_sums, int zero_in,
-                                 hipStream_t s);
+                                 hipStream_t s, int tile = kScanTileCells);
 
 hipError_t launch_cell_keys(const float4* pos, int n, LusGrid g, int* key, hipStream_t s);
 hipError_t launch_count_keys(const int* key, int n, int* counts, int* slot, hipStream_t s);

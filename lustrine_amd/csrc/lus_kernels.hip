@@ -32,17 +32,18 @@ KernelTimes& kernel_times() {
 }
 
 // ------------------------------------------------------------------ scan
-// Two passes over tiles of kScanTile ints: k_scan_reduce writes each tile's
-// sum, k_scan_down rescans the tile and adds the sum of all earlier tiles.
-// Up to kScanFuseTiles tiles every down block sums the earlier tile sums
-// itself (at most 16 KB, L2 resident); above that k_scan_tiles scans them
-// first in one block.  Each thread owns 8 consecutive ints, moved as two
-// 16-byte accesses when the arrays are 16-byte aligned.
-constexpr int kScanV = 8;
-constexpr int kScanTile = kBlock * kScanV;
+// Two passes over tiles of kV * 256 ints (the sort tile: 1024, 2048 or 4096
+// cells; kScanTileCells by default): k_scan_reduce writes each tile's sum,
+// k_scan_down rescans the tile and adds the sum of all earlier tiles.  Up to
+// kScanFuseTiles tiles every down block sums the earlier tile sums itself (at
+// most 16 KB, L2 resident); above that k_scan_tiles scans them first in one
+// block.  Each thread owns kV consecutive ints, moved as 16-byte accesses when
+// the arrays are 16-byte aligned.
 constexpr int kScanFuseTiles = 4096;
 
-int scan_tile_count(int m) { return (m + kScanTile - 1) / kScanTile; }
+int scan_tile_count(int m, int tile) { return (m + tile - 1) / tile; }
+
+bool scan_tile_valid(int tile) { return tile == 1024 || tile == 2048 || tile == 4096; }
 
 // exclusive scan of one value per thread across the block (any multiple of 64
 // threads up to 1024); *total receives the block sum
@@ -69,29 +70,30 @@ __device__ __forceinline__ int block_excl_scan(int v, int* lds, int* total) {
     return woff + incl - v;
 }
 
-template <bool kVec>
-__device__ __forceinline__ void scan_load8(const int* __restrict__ in, int base, int m, int v[kScanV]) {
-    if (kVec && base + kScanV <= m) {
-        const int4 a = *reinterpret_cast<const int4*>(in + base);
-        const int4 b = *reinterpret_cast<const int4*>(in + base + 4);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+template <bool kVec, int kV>
+__device__ __forceinline__ void scan_load(const int* __restrict__ in, int base, int m, int v[kV]) {
+    if (kVec && base + kV <= m) {
+#pragma unroll
+        for (int q = 0; q < kV; q += 4) {
+            const int4 a = *reinterpret_cast<const int4*>(in + base + q);
+            v[q] = a.x; v[q + 1] = a.y; v[q + 2] = a.z; v[q + 3] = a.w;
+        }
     } else {
 #pragma unroll
-        for (int k = 0; k < kScanV; ++k) v[k] = (base + k < m) ? in[base + k] : 0;
+        for (int k = 0; k < kV; ++k) v[k] = (base + k < m) ? in[base + k] : 0;
     }
 }
 
-template <bool kVec>
+template <bool kVec, int kV>
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int* __restrict__ in, int m,
                                                          int* __restrict__ tile_sums) {
     __shared__ int lds[16];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanV;
-    int v[kScanV];
-    scan_load8<kVec>(in, base, m, v);
+    const int base = blockIdx.x * (kBlock * kV) + threadIdx.x * kV;
+    int v[kV];
+    scan_load<kVec, kV>(in, base, m, v);
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanV; ++k) s += v[k];
+    for (int k = 0; k < kV; ++k) s += v[k];
     int tot;
     block_excl_scan(s, lds, &tot);
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
@@ -114,17 +116,17 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(int* __restrict__ tile_sums
 
 // kFuse: tile_sums holds plain tile sums and the block adds those of tiles
 // 0..blockIdx.x-1; otherwise tile_sums holds their exclusive scan.
-template <bool kVec, bool kFuse>
+template <bool kVec, bool kFuse, int kV>
 __global__ __launch_bounds__(kBlock) void k_scan_down(int* in, int m, const int* __restrict__ tile_sums,
                                                        int* out, int zero_in) {
     __shared__ int lds[16];
     __shared__ int pre_lds[kBlock / 64];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanV;
-    int v[kScanV];
-    scan_load8<kVec>(in, base, m, v);
+    const int base = blockIdx.x * (kBlock * kV) + threadIdx.x * kV;
+    int v[kV];
+    scan_load<kVec, kV>(in, base, m, v);
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanV; ++k) s += v[k];
+    for (int k = 0; k < kV; ++k) s += v[k];
     int pre;
     if (kFuse) {
         int t = 0;
@@ -144,22 +146,21 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(int* in, int m, const int*
         pre = tile_sums[blockIdx.x];
     }
     e += pre;
-    int o[kScanV];
+    int o[kV];
 #pragma unroll
-    for (int k = 0; k < kScanV; ++k) {
+    for (int k = 0; k < kV; ++k) {
         o[k] = e;
         e += v[k];
     }
-    if (kVec && base + kScanV <= m) {
-        *reinterpret_cast<int4*>(out + base) = make_int4(o[0], o[1], o[2], o[3]);
-        *reinterpret_cast<int4*>(out + base + 4) = make_int4(o[4], o[5], o[6], o[7]);
-        if (zero_in) {
-            *reinterpret_cast<int4*>(in + base) = make_int4(0, 0, 0, 0);
-            *reinterpret_cast<int4*>(in + base + 4) = make_int4(0, 0, 0, 0);
+    if (kVec && base + kV <= m) {
+#pragma unroll
+        for (int q = 0; q < kV; q += 4) {
+            *reinterpret_cast<int4*>(out + base + q) = make_int4(o[q], o[q + 1], o[q + 2], o[q + 3]);
+            if (zero_in) *reinterpret_cast<int4*>(in + base + q) = make_int4(0, 0, 0, 0);
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < kScanV; ++k) {
+        for (int k = 0; k < kV; ++k) {
             if (base + k < m) {
                 out[base + k] = o[k];
                 if (zero_in) in[base + k] = 0;
@@ -168,28 +169,36 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(int* in, int m, const int*
     }
 }
 
-template <bool kVec>
+template <bool kVec, int kV>
 static void scan_launch(int* in, int* out, int m, int* tile_sums, int zero_in, hipStream_t s) {
-    const int nt = scan_tile_count(m);
-    launch_kernel(k_scan_reduce<kVec>, dim3(nt), dim3(kBlock), 0, s, in, m, tile_sums);
+    const int nt = scan_tile_count(m, kBlock * kV);
+    launch_kernel((k_scan_reduce<kVec, kV>), dim3(nt), dim3(kBlock), 0, s, in, m, tile_sums);
     if (nt <= kScanFuseTiles) {
-        launch_kernel((k_scan_down<kVec, true>), dim3(nt), dim3(kBlock), 0, s, in, m,
+        launch_kernel((k_scan_down<kVec, true, kV>), dim3(nt), dim3(kBlock), 0, s, in, m,
                            tile_sums, out, zero_in);
     } else {
         launch_kernel(k_scan_tiles, dim3(1), dim3(1024), 0, s, tile_sums, nt);
-        launch_kernel((k_scan_down<kVec, false>), dim3(nt), dim3(kBlock), 0, s, in, m,
+        launch_kernel((k_scan_down<kVec, false, kV>), dim3(nt), dim3(kBlock), 0, s, in, m,
                            tile_sums, out, zero_in);
     }
 }
 
+template <bool kVec>
+static void scan_launch_tile(int* in, int* out, int m, int* tile_sums, int zero_in, hipStream_t s, int tile) {
+    if (tile == 1024) scan_launch<kVec, 4>(in, out, m, tile_sums, zero_in, s);
+    else if (tile == 4096) scan_launch<kVec, 16>(in, out, m, tile_sums, zero_in, s);
+    else scan_launch<kVec, 8>(in, out, m, tile_sums, zero_in, s);
+}
+
 hipError_t launch_exclusive_scan(const int* in, int* out, int m, int* tile_sums, int zero_in,
-                                 hipStream_t s) {
+                                 hipStream_t s, int tile) {
     if (m <= 0) return hipSuccess;
-    // out may alias in (only when zero_in is 0: every thread reads its 8 ints
+    if (!scan_tile_valid(tile)) return hipErrorInvalidValue;
+    // out may alias in (only when zero_in is 0: every thread reads its ints
     // before it writes them)
     const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
-    if (vec) scan_launch<true>(const_cast<int*>(in), out, m, tile_sums, zero_in, s);
-    else scan_launch<false>(const_cast<int*>(in), out, m, tile_sums, zero_in, s);
+    if (vec) scan_launch_tile<true>(const_cast<int*>(in), out, m, tile_sums, zero_in, s, tile);
+    else scan_launch_tile<false>(const_cast<int*>(in), out, m, tile_sums, zero_in, s, tile);
     return hipGetLastError();
 }
 

@@ -15,13 +15,16 @@ class FakeGpu:
     def __init__(self, cost):
         self.cost = cost
         self.cur = None
-        self.launch = (None, None, None)
+        self.launch = (None, None, None, None)
 
     def pbf_set_launch(self, block):
         self.launch = (block,) + self.launch[1:]
 
     def pbf_set_tiling(self, tiled, zt):
-        self.launch = self.launch[:1] + (tiled, zt)
+        self.launch = self.launch[:1] + (tiled, zt) + self.launch[3:]
+
+    def pbf_set_sort_tile(self, cells):
+        self.launch = self.launch[:3] + (cells,)
 
     def pbf_set_graph(self, on):
         pass
@@ -34,9 +37,9 @@ def test_tuner_picks_fastest_and_caches(tmp_path):
     seen = []
 
     def cost(cand, launch):
-        # fastest: block 128, cell 1.125, not tiled
+        # fastest: block 128, cell 1.125, not tiled, sort tile 4096
         t = 1.0 + 0.1 * abs(launch[0] - 128) / 64 + (0.0 if cand.cell_scale > 1 else 0.05) + (0.5 if launch[1] else 0)
-        return t
+        return t + (0.0 if launch[3] == 4096 else 0.01)
 
     g = FakeGpu(cost)
 
@@ -47,15 +50,19 @@ def test_tuner_picks_fastest_and_caches(tmp_path):
     path = str(tmp_path / "tune.json")
     tu = T.Tuner(path)
     best, table = tu.tune(g, init, 3_000_000, np.float32(1 / 60))
-    assert best == T.Launch(128, 1.125, False, 32)
-    assert len(table) == len(T.CANDIDATES) == len(seen)
+    assert best == T.Launch(128, 1.125, False, 32, 4096)
+    # the launch candidates at the default sort tile, then the other sort
+    # tiles for the fastest of them
+    per = len(T.CANDIDATES) + len(T.SORT_TILES) - 1
+    assert len(table) == per == len(seen)
+    assert [c.sort_tile for c, _ in table[len(T.CANDIDATES):]] == [1024, 4096]
     assert min(t for _, t in table) == pytest.approx(1.0)
     # cached per bucket: same bucket, no retiming
     best2, table2 = tu.tune(g, init, 2_500_000, np.float32(1 / 60))
-    assert best2 == best and table2 == [] and len(seen) == len(T.CANDIDATES)
+    assert best2 == best and table2 == [] and len(seen) == per
     # another bucket is timed
     tu.tune(g, init, 10_000, np.float32(1 / 60))
-    assert len(seen) == 2 * len(T.CANDIDATES)
+    assert len(seen) == 2 * per
     # persisted
     tu2 = T.Tuner(path)
     assert tu2.choice(3_100_000) == best and tu2.choice(9_000) is not None

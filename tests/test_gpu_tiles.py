@@ -140,3 +140,26 @@ def test_batch_with_larger_cells_matches_oracle(gpu_mod, oracle):
         ox[ids], ov[ids] = x, v
         assert np.array_equal(gx[off:off + s.n], ox) and np.array_equal(gv[off:off + s.n], ov), s.name
         off += s.n
+
+
+def test_sort_tiles_equal_default(gpu_mod):
+    # the counting sort's prefix scan in tiles of 1024, 2048 or 4096 cells:
+    # the same cell starts, so the same steps bit for bit -- on the 32K scene
+    # (18 to 71 tiles) and on 1,000 particles in a 420 x 200 x 420 box
+    # (4,496,621 cells: 4,392 tiles of 1024, above the 4,096 tiles up to which
+    # the down pass sums the tile sums itself)
+    for cfg in (scenes.cfg0(), scenes.small(10, 10, 10, box=(420, 200, 420))):
+        x0 = cfg.positions()
+        res = []
+        for tile in (2048, 1024, 4096):
+            with gpu_mod.Gpu(0) as g:
+                g.pbf_init(gpu_mod.pbf_params(cfg, exact=True), x0)
+                g.pbf_set_sort_tile(tile)
+                g.pbf_set_graph(True)
+                g.pbf_step(np.float32(cfg.dt), 4)
+                res.append(g.pbf_download(order=0))
+        for x, v, ids in res[1:]:
+            assert np.array_equal(ids, res[0][2]) and np.array_equal(x, res[0][0]) and np.array_equal(v, res[0][1])
+    with gpu_mod.Gpu(0) as g:
+        with pytest.raises(gpu_mod.LusError):
+            g.pbf_set_sort_tile(512)
