@@ -207,6 +207,7 @@ struct PbfState {
     bool wsbuild = true;       // flat list build with wave-staged candidates
     int lanes = 0;             // lanes per particle of the list walks (0: automatic)
     int sort_tile = lus::kScanTileCells;  // cells per block of the sort's prefix scan
+    int* scan_status = nullptr;  // two rows of the one-pass scan's tile sums (launch_scan_lookback)
 };
 
 // the network of the learned step (device copies), kept across scene inits
@@ -330,6 +331,17 @@ lus::PbfK pbf_constants(const LusPbfParams& p) {
     return k;
 }
 
+// Tiles up to which the step's cell scan is one look-back pass.
+constexpr int kLookbackTiles = 1024;
+
+// The look-back scan's two rows of tile sums for the scene's cell count, zeroed.
+hipError_t pbf_alloc_scan(PbfState& P, Arena& m) {
+    const size_t rows = (size_t)lus::scan_lookback_rows(P.C + 1);
+    hipError_t e = m.alloc(&P.scan_status, 2 * rows);
+    if (e != hipSuccess) return e;
+    return hipMemset(P.scan_status, 0, 2 * rows * sizeof(int));
+}
+
 // Enqueues one PBF substep.  ev (nullable): LUS_PBF_NPHASES-indexed event log,
 // ev_phase[k] receives the phase of event k.  rec (nullable): a plain step
 // that also writes training rows.
@@ -370,9 +382,18 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
     if (ev) LUS_HIP(mark(-1));
     LUS_HIP(lus::launch_pbf_key(P.x, P.v, P.n, k, dt, P.key, P.slot, P.counts, s));
     LUS_HIP(mark(0));
-    LUS_HIP(lus::launch_exclusive_scan(P.counts, P.cs, P.C + 1, P.tiles, 1, s, P.sort_tile));
+    // the cell scan: one pass with look-back up to kLookbackTiles tiles (the
+    // histogram cleared by the scatter), else the tile passes (which clear it)
+    const bool lookback = lus::scan_tile_count(P.C + 1, P.sort_tile) <= kLookbackTiles;
+    if (lookback) {
+        const size_t rows = (size_t)lus::scan_lookback_rows(P.C + 1);
+        LUS_HIP(lus::launch_scan_lookback(P.counts, P.cs, P.C + 1, P.scan_status + P.parity * rows,
+                                          P.scan_status + (P.parity ^ 1) * rows, P.sort_tile, s));
+    } else {
+        LUS_HIP(lus::launch_exclusive_scan(P.counts, P.cs, P.C + 1, P.tiles, 1, s, P.sort_tile));
+    }
     LUS_HIP(mark(1));
-    LUS_HIP(lus::launch_scatter(P.key, P.slot, P.cs, P.n, P.tmp, s));
+    LUS_HIP(lus::launch_scatter(P.key, P.slot, P.cs, P.n, P.tmp, s, lookback ? P.counts : nullptr, P.C + 1));
     LUS_HIP(mark(2));
     LUS_HIP(lus::launch_pbf_permute(P.tmp, P.key, P.cs, P.x, P.v, P.id, P.n, k, dt, P.x2, P.xp,
                                     P.id2, P.key2, nullptr, s));
@@ -916,6 +937,7 @@ int lus_mix_init(LusGpu* c, const LusPbfParams* p, const LusMixParams* mp, int n
     LUS_HIP(P.mem.alloc(&P.counts, P.C + 1));
     LUS_HIP(P.mem.alloc(&P.cs, P.C + 1));
     LUS_HIP(P.mem.alloc(&P.tiles, lus::scan_tile_count(P.C + 1, lus::kScanTileMin) + 1));
+    LUS_HIP(pbf_alloc_scan(P, P.mem));
     LUS_HIP(hipMemset(P.counts, 0, sizeof(int) * (P.C + 1)));
     LUS_HIP(P.mem.alloc(&P.ph, n));
     LUS_HIP(P.mem.alloc(&P.ph2, n));
@@ -991,6 +1013,7 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
     LUS_HIP(m.alloc(&P.counts, P.C + 1));
     LUS_HIP(m.alloc(&P.cs, P.C + 1));
     LUS_HIP(m.alloc(&P.tiles, lus::scan_tile_count(P.C + 1, lus::kScanTileMin) + 1));
+    LUS_HIP(pbf_alloc_scan(P, m));
     P.nblk = lus::grid_for(n, 64);  // block-reduction entries of the finest flat launch
     LUS_HIP(m.alloc(&P.maxc, (size_t)P.nmaxc * P.nblk));
     LUS_HIP(m.alloc(&P.maxd, 2 * (size_t)P.nblk));

@@ -99,11 +99,20 @@ int scan_tile_count(int m, int tile = kScanTileCells);
 bool scan_tile_valid(int tile);
 hipError_t launch_exclusive_scan(const int* in, int* out, int m, int* tile_sums, int zero_in,
                                  hipStream_t s, int tile = kScanTileCells);
+// The same scan in one pass (decoupled look-back).  `in` is left as it is
+// (clear it afterwards: launch_scatter's clear); status and status_next: two
+// rows of scan_lookback_rows(m) ints, both zero before the first call; the
+// call reads status and zeroes status_next, so successive calls alternate
+// the rows.  out must not alias in.
+int scan_lookback_rows(int m);
+hipError_t launch_scan_lookback(const int* in, int* out, int m, int* status, int* status_next, int tile,
+                                hipStream_t s);
 
 hipError_t launch_cell_keys(const float4* pos, int n, LusGrid g, int* key, hipStream_t s);
 hipError_t launch_count_keys(const int* key, int n, int* counts, int* slot, hipStream_t s);
+// clear (nullable): clear_n ints zeroed as well (the histogram, for the next sort)
 hipError_t launch_scatter(const int* key, const int* slot, const int* cell_start, int n, int* tmp,
-                          hipStream_t s);
+                          hipStream_t s, int* clear = nullptr, int clear_n = 0);
 // generic stable rank: perm[o] = i for the stable order; key_sorted[o] = key[i]
 hipError_t launch_stable_rank(const int* tmp, const int* key, const int* cell_start, int n,
                               int* perm, int* key_sorted, hipStream_t s);

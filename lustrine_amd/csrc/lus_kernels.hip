@@ -202,6 +202,97 @@ hipError_t launch_exclusive_scan(const int* in, int* out, int m, int* tile_sums,
     return hipGetLastError();
 }
 
+// One-pass scan with decoupled look-back (the PBF step's cell histogram).
+// Block b scans tile b, publishes the tile's sum in status[b] (bit 31 set:
+// published), and the lanes of its first wave add the published sums of the
+// earlier tiles, one tile per lane.  A sum not published after kLookbackSpins
+// polls is computed by that lane from the tile's ints (`in` is not written by
+// this kernel: the scatter that follows clears it), so no block waits on a
+// block that has not started.  status holds this step's row, zeroed;
+// status_next (nt_max ints) is the next step's row, zeroed here.
+constexpr int kLookbackSpins = 1 << 16;
+constexpr int kPublished = 1 << 31;
+
+template <bool kVec, int kV>
+__global__ __launch_bounds__(kBlock) void k_scan_lookback(const int* __restrict__ in, int m, int* __restrict__ out,
+                                                           int* __restrict__ status, int* __restrict__ status_next,
+                                                           int nt_max) {
+    __shared__ int lds[16];
+    __shared__ int prefix;
+    constexpr int kTile = kBlock * kV;
+    const int b = blockIdx.x;
+    for (int q = b * kBlock + (int)threadIdx.x; q < nt_max; q += gridDim.x * kBlock) status_next[q] = 0;
+    const int base = b * kTile + threadIdx.x * kV;
+    int v[kV];
+    scan_load<kVec, kV>(in, base, m, v);
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kV; ++k) s += v[k];
+    int tot;
+    int e = block_excl_scan(s, lds, &tot);
+    if (threadIdx.x == 0) __hip_atomic_store(status + b, tot | kPublished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 64) {
+        const int lane = (int)threadIdx.x;
+        int acc = 0;
+        for (int q = b - 1 - lane; q >= 0; q -= 64) {
+            int st = __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int spins = 0; (st & kPublished) == 0 && spins < kLookbackSpins; ++spins) {
+                __builtin_amdgcn_s_sleep(1);
+                st = __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (st & kPublished) {
+                acc += st & ~kPublished;
+            } else {
+                const int hi = (q + 1) * kTile < m ? (q + 1) * kTile : m;
+                for (int c = q * kTile; c < hi; ++c) acc += in[c];
+            }
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+        if (lane == 0) prefix = acc;
+    }
+    __syncthreads();
+    e += prefix;
+    int o[kV];
+#pragma unroll
+    for (int k = 0; k < kV; ++k) {
+        o[k] = e;
+        e += v[k];
+    }
+    if (kVec && base + kV <= m) {
+#pragma unroll
+        for (int q = 0; q < kV; q += 4)
+            *reinterpret_cast<int4*>(out + base + q) = make_int4(o[q], o[q + 1], o[q + 2], o[q + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kV; ++k)
+            if (base + k < m) out[base + k] = o[k];
+    }
+}
+
+int scan_lookback_rows(int m) { return scan_tile_count(m, kScanTileMin); }
+
+hipError_t launch_scan_lookback(const int* in, int* out, int m, int* status, int* status_next, int tile,
+                                hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    if (!scan_tile_valid(tile) || in == out) return hipErrorInvalidValue;
+    const int nt = scan_tile_count(m, tile), nt_max = scan_lookback_rows(m);
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+#define LUS_LB(V, KV) launch_kernel((k_scan_lookback<V, KV>), dim3(nt), dim3(kBlock), 0, s, in, m, out, status, \
+                                    status_next, nt_max)
+    if (vec) {
+        if (tile == 1024) LUS_LB(true, 4);
+        else if (tile == 4096) LUS_LB(true, 16);
+        else LUS_LB(true, 8);
+    } else {
+        if (tile == 1024) LUS_LB(false, 4);
+        else if (tile == 4096) LUS_LB(false, 16);
+        else LUS_LB(false, 8);
+    }
+#undef LUS_LB
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ sort
 __global__ __launch_bounds__(kBlock) void k_cell_keys(const float4* __restrict__ pos, int n, LusGrid g,
                                                        int* __restrict__ key) {
@@ -231,19 +322,23 @@ hipError_t launch_count_keys(const int* key, int n, int* counts, int* slot, hipS
     return hipGetLastError();
 }
 
+// clear (nullable): the histogram to zero, clear_n ints, for the next sort
 __global__ __launch_bounds__(kBlock) void k_scatter(const int* __restrict__ key,
                                                      const int* __restrict__ slot,
                                                      const int* __restrict__ cell_start, int n,
-                                                     int* __restrict__ tmp) {
+                                                     int* __restrict__ tmp, int* __restrict__ clear,
+                                                     int clear_n) {
     const int i = xcd_index();
+    if (clear)
+        for (int c = blockIdx.x * kBlock + (int)threadIdx.x; c < clear_n; c += gridDim.x * kBlock) clear[c] = 0;
     if (i >= n) return;
     tmp[cell_start[key[i]] + slot[i]] = i;
 }
 
 hipError_t launch_scatter(const int* key, const int* slot, const int* cell_start, int n, int* tmp,
-                          hipStream_t s) {
+                          hipStream_t s, int* clear, int clear_n) {
     launch_kernel(k_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, key, slot, cell_start, n,
-                       tmp);
+                       tmp, clear, clear_n);
     return hipGetLastError();
 }
 
