@@ -193,6 +193,15 @@ int lus_pbf_set_lanes(LusGpu* ctx, int lanes);
  * replaced is the inclusive prefix of Sorting::counting_sort
  * (src/neighbors/Sorting.cpp:11-33). */
 int lus_pbf_set_sort_tile(LusGpu* ctx, int cells);
+/* on = 1 (default): the XSPH pass of a plain step also predicts the next
+ * step's positions and writes their cell keys and histogram slots, and the
+ * next step skips its key kernel when it runs with the same dt (any other dt,
+ * a learned or mixed step, tiles, or a new scene run the key kernel).  The
+ * same keys, so the same results bit for bit; phase 0 then has no launch and
+ * its work is timed in phase 6.  0: every step runs the key kernel.  No
+ * reference counterpart (kernel fusion of predict (src/Simulate.cpp:48-51)
+ * and get_cell_id into the previous step's commit). */
+int lus_pbf_set_key_fusion(LusGpu* ctx, int on);
 /* Threads per block (64, 128 or 256; default 256) of the per-slot neighbour
  * passes.  Same results.  No reference counterpart (launch configuration). */
 int lus_pbf_set_launch(LusGpu* ctx, int block);
@@ -219,7 +228,9 @@ int lus_pbf_last_max_constraint(LusGpu* ctx, float* out);
  *         7 first lambda pass of the step (builds the neighbour sets),
  *         8 warm-start features (build the neighbour sets), 9 warm-start
  *         network, 10 density decision (learned mode or density check),
- *         11 tile list of the tiled neighbour passes */
+ *         11 tile list of the tiled neighbour passes (phase 0 has no launch
+ *         in a step whose keys the previous step's XSPH wrote:
+ *         lus_pbf_set_key_fusion) */
 int lus_pbf_bench(LusGpu* ctx, float dt, int warmup, int steps, float* ms,
                   float* phase_ms, int* phase_launches);
 /* Average device time per launch (microseconds) of the neighbour-pass kernels

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <limits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -208,6 +209,10 @@ struct PbfState {
     int lanes = 0;             // lanes per particle of the list walks (0: automatic)
     int sort_tile = lus::kScanTileCells;  // cells per block of the sort's prefix scan
     int* scan_status = nullptr;  // two rows of the one-pass scan's tile sums (launch_scan_lookback)
+    // the next step's cell keys, written by the XSPH pass for step length
+    // keys_dt (NaN: none; the next step then runs the key kernel)
+    bool fuse_keys = true;
+    float keys_dt = std::numeric_limits<float>::quiet_NaN();
 };
 
 // the network of the learned step (device copies), kept across scene inits
@@ -380,8 +385,15 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
         k.ph_out = P.ph2;
     }
     if (ev) LUS_HIP(mark(-1));
-    LUS_HIP(lus::launch_pbf_key(P.x, P.v, P.n, k, dt, P.key, P.slot, P.counts, s));
-    LUS_HIP(mark(0));
+    // the keys, unless the previous step's XSPH pass wrote them for this dt;
+    // keys it wrote for another dt left their histogram in counts: cleared
+    // before the key kernel counts again
+    if (!(P.keys_dt == dt)) {
+        if (!std::isnan(P.keys_dt)) LUS_HIP(lus::launch_zero_ints(P.counts, P.C + 1, s));
+        LUS_HIP(lus::launch_pbf_key(P.x, P.v, P.n, k, dt, P.key, P.slot, P.counts, s));
+        LUS_HIP(mark(0));
+    }
+    P.keys_dt = std::numeric_limits<float>::quiet_NaN();
     // the cell scan: one pass with look-back up to kLookbackTiles tiles (the
     // histogram cleared by the scatter), else the tile passes (which clear it)
     const bool lookback = lus::scan_tile_count(P.C + 1, P.sort_tile) <= kLookbackTiles;
@@ -519,8 +531,19 @@ int pbf_enqueue(LusGpu* c, float dt, std::vector<hipEvent_t>* ev, std::vector<in
                                                nullptr, ta, s));
             LUS_HIP(lus::launch_pbf_velocity(P.xp, P.x2, P.n, dt, P.v2, s));
         }
+        // the next step's keys with this step's dt (its key kernel is then
+        // skipped when it runs with the same dt)
+        lus::NextKeys nk;
+        const bool keys_out = P.fuse_keys && !tx.tiles;
+        if (keys_out) {
+            nk.key = P.key;
+            nk.slot = P.slot;
+            nk.counts = P.counts;
+            nk.dt = dt;
+        }
         LUS_HIP(lus::launch_pbf_xsph(fin, P.v2, P.key2, P.cs, P.n, k, P.v, P.x2, P.masks, P.xp,
-                                     P.check ? P.maxd : nullptr, nullptr, tx, s));
+                                     P.check ? P.maxd : nullptr, nullptr, tx, s, keys_out ? &nk : nullptr));
+        if (keys_out) P.keys_dt = dt;
         LUS_HIP(mark(6));
         if (P.check) {
             LUS_HIP(lus::launch_pbf_decide(P.maxd, nred, P.n, P.bound, 0, P.flag, P.counters, P.hist,
@@ -565,6 +588,18 @@ int pbf_run(LusGpu* c, float dt, int steps) {
     int left = steps;
     while (left >= 2) {
         if (!P.graph_exec || P.graph_dt != dt || P.graph_parity != P.parity) {
+            // a captured step that skips the key kernel relies on the step
+            // before it: capture only when the keys at hand are exactly what
+            // the captured steps leave behind (written for dt when the steps
+            // write keys, none otherwise), so every replay sees the state the
+            // capture saw
+            const bool writes_keys = P.fuse_keys && !P.learned && !P.mixed && !P.tiled;
+            if (writes_keys != (P.keys_dt == dt)) {
+                int r = pbf_enqueue(c, dt, nullptr, nullptr);
+                if (r) return r;
+                --left;
+                continue;
+            }
             pbf_drop_graph(P);
             LUS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int r1 = pbf_enqueue(c, dt, nullptr, nullptr);
@@ -976,6 +1011,7 @@ int pbf_setup(LusGpu* c, const LusPbfParams* p, int n, const float* pos, const f
     P.ready = false;
     P.stepped = false;
     P.parity = 0;
+    P.keys_dt = std::numeric_limits<float>::quiet_NaN();
     P.scenes = nullptr;
     P.sid = P.sid2 = nullptr;
     P.mixed = false;
@@ -1189,6 +1225,15 @@ int lus_pbf_set_sort_tile(LusGpu* c, int cells) {
     LUS_HIP(hipStreamSynchronize(c->stream));
     pbf_drop_graph(c->pbf);
     c->pbf.sort_tile = cells;
+    return LUS_OK;
+}
+
+int lus_pbf_set_key_fusion(LusGpu* c, int on) {
+    if (!c) return set_error(LUS_EINVAL, "lus_pbf_set_key_fusion: NULL context");
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    pbf_drop_graph(c->pbf);
+    c->pbf.fuse_keys = on != 0;
     return LUS_OK;
 }
 

@@ -105,6 +105,8 @@ hipError_t launch_exclusive_scan(const int* in, int* out, int m, int* tile_sums,
 // call reads status and zeroes status_next, so successive calls alternate
 // the rows.  out must not alias in.
 int scan_lookback_rows(int m);
+// a[0..m) = 0 with a kernel (graph capture safe, unlike a memset node)
+hipError_t launch_zero_ints(int* a, int m, hipStream_t s);
 hipError_t launch_scan_lookback(const int* in, int* out, int m, int* status, int* status_next, int tile,
                                 hipStream_t s);
 
@@ -205,9 +207,21 @@ hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, floa
 // XSPH + commit; maxd (nullable, 2 * blocks ints): per-block max of the
 // density constraint C_i = rho_i / rho0 - 1 at the final positions
 // (ordered() ints), then per-block sums of max(0, C_i) (float bits)
+// The next step's predicted cell keys, computed by the XSPH pass from the
+// positions and velocities it commits (launch_pbf_key's outputs: key, slot
+// and the histogram counts, which must be zero), with the next step's dt.
+struct NextKeys {
+    int* key = nullptr;  // nullptr: none
+    int* slot = nullptr;
+    int* counts = nullptr;
+    float dt = 0.0f;
+};
+// nk (nullable): also write the next step's keys (one-thread and four-lane
+// passes; not with tiles)
 hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
                            PbfK k, float4* vout, float4* xout, unsigned long long* masks,
-                           const float4* xp, int* maxd, const int* gate, const TileArgs& ta, hipStream_t s);
+                           const float4* xp, int* maxd, const int* gate, const TileArgs& ta, hipStream_t s,
+                           const NextKeys* nk = nullptr);
 
 // ---- mixed sand + fluid passes (k.ph, k.ph_out set) ----
 hipError_t launch_mix_lambda(const float4* xa, int* key2, const int* cs, int n, PbfK k, float4* xl,

@@ -18,6 +18,7 @@
 // neighbours in the L2 of the XCD that gathers them.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include "lus_device.hpp"
@@ -290,6 +291,17 @@ hipError_t launch_scan_lookback(const int* in, int* out, int m, int* status, int
         else LUS_LB(false, 8);
     }
 #undef LUS_LB
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_ints(int* __restrict__ a, int m) {
+    for (int c = blockIdx.x * kBlock + (int)threadIdx.x; c < m; c += gridDim.x * kBlock) a[c] = 0;
+}
+
+hipError_t launch_zero_ints(int* a, int m, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    const int g = std::min((m + kBlock - 1) / kBlock, 2048);
+    launch_kernel(k_zero_ints, dim3(g), dim3(kBlock), 0, s, a, m);
     return hipGetLastError();
 }
 

@@ -52,7 +52,8 @@ hipError_t launch_deltap_sub(const float4* xl, int* key2, const int* cs, int n, 
                              const float4* xp, const int* gate, hipStream_t s);
 hipError_t launch_xsph_sub(const float4* xa, const float4* vt, int* key2, const int* cs, int n, PbfK k,
                            float4* vout, float4* xout, unsigned long long* masks, const float4* xp,
-                           const int* gate, hipStream_t s);
+                           const int* gate, hipStream_t s,
+                           const NextKeys& nk);
 
 // clamp box and cell y offset of a particle: the scene table entry of a batch,
 // or the single scene's bounds
@@ -1521,7 +1522,7 @@ hipError_t launch_pbf_velocity(const float4* xa, const float4* xold, int n, floa
 template <bool kFast, bool kCheck, class Visit>
 __device__ __forceinline__ float xsph_body(int i, const float4* __restrict__ xa, const float4* __restrict__ vt,
                                            const PbfK& k, float4* __restrict__ vout, float4* __restrict__ xout,
-                                           Visit&& visit) {
+                                           Visit&& visit, float4* xn = nullptr, float4* vn = nullptr) {
     const float4 pi = xa[i];
     const float4 vi = vt[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, rho = 0.0f;
@@ -1539,13 +1540,32 @@ __device__ __forceinline__ float xsph_body(int i, const float4* __restrict__ xa,
             if (kCheck) rho += w;
         }
     });
-    vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
-    xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
+    const float4 vc = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
+    const float4 xc = make_float4(pi.x, pi.y, pi.z, 0.0f);
+    vout[i] = vc;
+    xout[i] = xc;
+    if (xn) *xn = xc;
+    if (vn) *vn = vc;
     const float Ci = rho * k.c_rho - 1.0f;
     // a non-finite position (no pair passes the distance test) or a NaN
     // must fail the density test
     const bool finite = __builtin_isfinite(pi.x) && __builtin_isfinite(pi.y) && __builtin_isfinite(pi.z);
     return (finite && Ci == Ci) ? Ci : __builtin_inff();
+}
+
+// The next step's predict + cell key of slot i from the committed x and v,
+// as k_pbf_key computes them from the stored arrays (the same floats, so the
+// same keys); every lane of the wave calls it (histogram slots by wave runs).
+__device__ __forceinline__ void next_key(int i, bool active, const float4 xn, const float4 vn, const PbfK& k,
+                                         const NextKeys& nk) {
+    int c = 0;
+    if (active) {
+        const Bounds b = bounds_of(k, k.scenes ? k.sid_out[i] : 0);
+        c = pbf_cell(pbf_predict(xn, vn, k, b, nk.dt), k, b.yoff, 0);
+        nk.key[i] = c;
+    }
+    const int sl = wave_atomic_slot(nk.counts, c, active);
+    if (active) nk.slot[i] = sl;
 }
 
 template <bool kFast, bool kCheck>
@@ -1558,14 +1578,16 @@ __global__ __launch_bounds__(kBlock) void k_pbf_xsph(const float4* __restrict__ 
                                                       unsigned long long* __restrict__ masks,
                                                       const float4* __restrict__ xp,
                                                       int* __restrict__ maxd,
-                                                      const int* __restrict__ gate) {
+                                                      const int* __restrict__ gate, NextKeys nk) {
     if (gate && *gate == 0) return;  // uniform: before any barrier
     const int i = xcd_index();
     float Ci = -1e30f;
+    float4 xn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vn = xn;
     if (i < n)
         Ci = xsph_body<kFast, kCheck>(i, xa, vt, k, vout, xout, [&](auto&& f) {
             for_each_neighbor<false, kWalkPairs>(i, n, key2, cs, masks, xp, xa, vt, k, f);
-        });
+        }, &xn, &vn);
+    if (nk.key) next_key(i, i < n, xn, vn, k, nk);
     if (kCheck) block_max_sum_store(Ci, Ci > 0.0f ? Ci : 0.0f, maxd);
 }
 
@@ -1728,7 +1750,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_xsph_sub(const float4* __restric
                                                           float4* __restrict__ xout,
                                                           unsigned long long* __restrict__ nb,
                                                           const float4* __restrict__ xp,
-                                                          const int* __restrict__ gate) {
+                                                          const int* __restrict__ gate, NextKeys nk) {
     if (gate && *gate == 0) return;
     const int t = xcd_index();
     const int i = t / kSub, sub = t % kSub;
@@ -1754,9 +1776,21 @@ __global__ __launch_bounds__(kBlock) void k_pbf_xsph_sub(const float4* __restric
     sx = sub_sum(sx);
     sy = sub_sum(sy);
     sz = sub_sum(sz);
+    const float4 vn = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
+    const float4 xn = make_float4(pi.x, pi.y, pi.z, 0.0f);
     if (i < n && sub == 0) {
-        vout[i] = make_float4(vi.x + k.cW * sx, vi.y + k.cW * sy, vi.z + k.cW * sz, 0.0f);
-        xout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
+        vout[i] = vn;
+        xout[i] = xn;
+    }
+    if (nk.key) {
+        // the wave's 16 particles to lanes 0..15 (in slot order), so equal
+        // keys of neighbouring slots share one histogram atomic
+        const int lane = (int)(threadIdx.x & 63);
+        const int src = (lane & 15) * kSub;
+        const float4 xq = make_float4(__shfl(xn.x, src, 64), __shfl(xn.y, src, 64), __shfl(xn.z, src, 64), 0.0f);
+        const float4 vq = make_float4(__shfl(vn.x, src, 64), __shfl(vn.y, src, 64), __shfl(vn.z, src, 64), 0.0f);
+        const int iq = (t - lane) / kSub + (lane & 15);
+        next_key(iq, lane < 16 && iq < n, xq, vq, k, nk);
     }
 }
 
@@ -1907,16 +1941,19 @@ hipError_t launch_deltap_sub(const float4* xl, int* key2, const int* cs, int n, 
 
 hipError_t launch_xsph_sub(const float4* xa, const float4* vt, int* key2, const int* cs, int n, PbfK k,
                            float4* vout, float4* xout, unsigned long long* masks, const float4* xp,
-                           const int* gate, hipStream_t s) {
+                           const int* gate, hipStream_t s, const NextKeys& nk) {
     launch_kernel(k_pbf_xsph_sub, dim3(grid_for(n * kSub)), dim3(kBlock), 0, s, xa, vt, key2, cs, n, k, vout, xout,
-                  masks, xp, gate);
+                  masks, xp, gate, nk);
     return hipGetLastError();
 }
 
 hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const int* cs, int n,
                            PbfK k, float4* vout, float4* xout, unsigned long long* masks,
-                           const float4* xp, int* maxd, const int* gate, const TileArgs& ta, hipStream_t s) {
+                           const float4* xp, int* maxd, const int* gate, const TileArgs& ta, hipStream_t s,
+                           const NextKeys* next) {
+    const NextKeys nk = next ? *next : NextKeys();
     if (ta.tiles) {
+        if (nk.key) return hipErrorInvalidValue;
         const dim3 g(ta.grid), b(kTileT);
         const size_t sh = 2 * (size_t)ta.lcap * sizeof(float4);
         if (k.exact && maxd)
@@ -1933,16 +1970,21 @@ hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const 
                                xp, maxd, gate, ta.tiles, ta.ntiles, ta.lcap);
         return hipGetLastError();
     }
-    if (ta.sub > 1 && !k.exact && !maxd) return launch_xsph_sub(xa, vt, key2, cs, n, k, vout, xout, masks, xp, gate, s);
+    if (ta.sub > 1 && !k.exact && !maxd)
+        return launch_xsph_sub(xa, vt, key2, cs, n, k, vout, xout, masks, xp, gate, s, nk);
     const dim3 g(grid_for(n, ta.block)), b(ta.block);
     if (k.exact && maxd)
-        launch_kernel((k_pbf_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+        launch_kernel((k_pbf_xsph<false, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate,
+                      nk);
     else if (k.exact)
-        launch_kernel((k_pbf_xsph<false, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+        launch_kernel((k_pbf_xsph<false, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate,
+                      nk);
     else if (maxd)
-        launch_kernel((k_pbf_xsph<true, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+        launch_kernel((k_pbf_xsph<true, true>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate,
+                      nk);
     else
-        launch_kernel((k_pbf_xsph<true, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate);
+        launch_kernel((k_pbf_xsph<true, false>), g, b, 0, s, xa, vt, key2, cs, n, k, vout, xout, masks, xp, maxd, gate,
+                      nk);
     return hipGetLastError();
 }
 

@@ -105,6 +105,7 @@ _SIGS = {
     "lus_pbf_set_build": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_set_lanes": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_set_sort_tile": (C.c_int, [C.c_void_p, C.c_int]),
+    "lus_pbf_set_key_fusion": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_set_launch": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_pbf_step_latency": (C.c_int, [C.c_void_p, C.c_float, C.c_int, _f]),
     "lus_pbf_download": (C.c_int, [C.c_void_p, C.c_int, _f, _f, _i]),
@@ -414,6 +415,10 @@ class Gpu:
     def pbf_set_sort_tile(self, cells=2048):
         """Cells per block of the counting sort's prefix scan: 1024, 2048 or 4096."""
         _check(self._lib.lus_pbf_set_sort_tile(self._h, int(cells)), "lus_pbf_set_sort_tile")
+
+    def pbf_set_key_fusion(self, on=True):
+        """The XSPH pass writes the next step's cell keys (True) or every step runs the key kernel."""
+        _check(self._lib.lus_pbf_set_key_fusion(self._h, int(bool(on))), "lus_pbf_set_key_fusion")
 
     def pbf_step_latency(self, dt, steps):
         ms = np.zeros(steps, np.float32)

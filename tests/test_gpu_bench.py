@@ -13,16 +13,20 @@ def _init(gpu_mod, g, cfg, x0, exact):
     g.pbf_init(gpu_mod.pbf_params(cfg, exact=exact), x0, None)
 
 
+@pytest.mark.parametrize("fused_keys", [True, False])
 @pytest.mark.parametrize("exact", [True, False])
-def test_phase_run_counts_and_sums(gpu_mod, exact):
+def test_phase_run_counts_and_sums(gpu_mod, exact, fused_keys):
     cfg = scenes.cfg0()
     x0 = cfg.positions()
     steps, K = 6, cfg.iterations
     with gpu_mod.Gpu(0) as g:
         _init(gpu_mod, g, cfg, x0, exact)
+        g.pbf_set_key_fusion(fused_keys)
         ms, ph, nl = g.pbf_bench(np.float32(cfg.dt), 2, steps, phases=True)
-    # one launch per phase per step; the first lambda pass of a step builds the lists
-    assert nl["predict_key"] == steps and nl["cell_scan"] == steps
+    # one launch per phase per step; the first lambda pass of a step builds the
+    # lists; with fused keys every timed step's keys come from the XSPH pass
+    # of the step before it (the warm-up's last step included)
+    assert nl["predict_key"] == (0 if fused_keys else steps) and nl["cell_scan"] == steps
     assert nl["scatter"] == steps and nl["stable_permute"] == steps
     assert nl["lambda_build"] == steps and nl["lambda"] == (K - 1) * steps
     assert nl["deltap"] == K * steps and nl["xsph"] == steps

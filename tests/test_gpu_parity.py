@@ -530,3 +530,38 @@ def test_pbf_four_lane_walks(gpu_mod, oracle):
     assert retest > 0
     assert np.array_equal(gid, ids)
     assert rel_err(gx, x) <= REL
+
+
+def test_pbf_keys_from_xsph_match_key_kernel(gpu_mod):
+    # the XSPH pass writing the next step's cell keys (default) against every
+    # step running the key kernel: the same bits, with graphs and without, on
+    # the 32K scene (four-lane passes), a 1,000-particle scene with one-lane
+    # passes, and a batch of scenes; a call with another dt in between runs
+    # the key kernel again
+    cfg = scenes.cfg0()
+    small = scenes.small(10, 10, 10, box=(20, 20, 20))
+    b = scenes.BatchConfig(name="t", scenes=[scenes.dam_break_of_size(s, 70 + k)
+                                             for k, s in enumerate([900, 3000, 1700])])
+    for case in ("cfg0", "small", "batch"):
+        out = []
+        for fused in (False, True):
+            for graph in (False, True):
+                with gpu_mod.Gpu(0) as g:
+                    if case == "batch":
+                        g.pbf_batch_init(gpu_mod.pbf_params(b.scenes[0], exact=True), b.counts(), b.boxes(),
+                                         b.positions())
+                        dt = b.scenes[0].dt
+                    else:
+                        c = cfg if case == "cfg0" else small
+                        g.pbf_init(gpu_mod.pbf_params(c, exact=(case != "cfg0")), c.positions())
+                        dt = c.dt
+                        if case == "small":
+                            g.pbf_set_lanes(1)
+                    g.pbf_set_key_fusion(fused)
+                    g.pbf_set_graph(graph)
+                    g.pbf_step(np.float32(dt), 5)
+                    g.pbf_step(np.float32(dt * 0.5), 3)
+                    g.pbf_step(np.float32(dt), 4)
+                    out.append(g.pbf_download(order=0))
+        for x, v, ids in out[1:]:
+            assert np.array_equal(ids, out[0][2]) and np.array_equal(x, out[0][0]) and np.array_equal(v, out[0][1])
