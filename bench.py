@@ -40,12 +40,12 @@ PEAK_HBM_GBS = 8000.0
 PHASE_BYTES = {
     "predict_key": 64 + 24,   # read x, v (predict) + cell key, histogram
     "cell_scan": None,        # 8 bytes per cell, set from the grid
-    "scatter": 12,            # read key, slot; write slot index
+    "scatter": None,          # read key, slot; write slot index (12) + zero the histogram, set from the grid
     "stable_permute": 96,     # read rank, x, x*, v; write sorted x, x*, id, key
     "lambda": 28,             # read x*, cell range; write lambda
     "lambda_build": 28,
     "deltap": 44,             # read x*, lambda, range; write x*
-    "xsph": 88,               # read x, x*, v, range; write v, x
+    "xsph": 96,               # read x, x*, v, range; write v, x; the next step's key and slot
     "features": 104,          # read x*, x, range; write 32 bf16 features
     "mlp": 96,                # read 32 bf16 features, x*; write x*
     "decide": None,           # one block over the block maxima
@@ -574,6 +574,7 @@ def main(argv=None):
     if phase_ms is not None:
         pb = dict(PHASE_BYTES)
         pb["cell_scan"] = 8.0 * (ncell + 1) / cfg.n
+        pb["scatter"] = 12.0 + 4.0 * (ncell + 1) / cfg.n
         phases = {}
         for name, t in phase_ms.items():
             nl = launches[name]
