@@ -337,10 +337,27 @@ int lus_sand_init(LusGpu* ctx, const LusSandParams* p, int n_sand, const float* 
                   const float* vel, const int* attracted, int n_solid,
                   const float* solid_pos, const int* solid_index);
 /* One simulate_sand call.  player = Bullet player position (3 floats, may be
- * NULL when both flags are 0).  The call waits once for the device mid-step:
- * the neighbour lists' total length sizes their buffer before they are
- * written (the lists have no fixed capacity, as the reference's vectors). */
+ * NULL when both flags are 0).  By default the call waits once for the device
+ * mid-step: the neighbour lists' total length sizes their buffer before they
+ * are written (the lists have no fixed capacity, as the reference's vectors).
+ * With a fixed capacity (lus_sand_set_list_capacity) it only enqueues. */
 int lus_sand_step(LusGpu* ctx, float dt, int attract_flag, int blow_flag, const float* player);
+/* entries > 0: the neighbour lists get a fixed buffer of that many entries
+ * and lus_sand_step no longer waits for the device (the reference's lists
+ * grow instead, src/neighbors/Neighbors.cpp:306-361; ~33 entries per
+ * particle on the 1 M sand lattice).  A step whose lists would not fit writes
+ * none, skips its substeps and leaves an invalid state; the next
+ * lus_gpu_sync, lus_sand_download or lus_sand_neighbors reports it
+ * (LUS_ESTATE) and clears the mark.  In the same way a simulate_sand_v3
+ * prediction outside the grid is reported there (LUS_EINVAL).  0 (default):
+ * the waiting step.  Reset by lus_sand_init.  No reference counterpart. */
+int lus_sand_set_list_capacity(LusGpu* ctx, long entries);
+/* on = 1: each lus_sand_step replays a hipGraph of the whole step, captured
+ * once per launch configuration (dt, flags, player position when a flag is
+ * set, the previous attract flag, variant, and the alternating buffers: a run
+ * with fixed arguments uses two graphs); needs a fixed list capacity.  Same
+ * results bit for bit.  Reset by lus_sand_init.  No reference counterpart. */
+int lus_sand_set_graph(LusGpu* ctx, int on);
 /* Which of the reference's sand Simulate_funs lus_sand_step computes (after
  * lus_sand_init; default 0): 0 simulate_sand (src/Simulate.cpp:156-325), the
  * default simulate_fun; 1 simulate_sand_credits (:327-510), installed by

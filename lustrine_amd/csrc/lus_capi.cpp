@@ -253,7 +253,30 @@ struct SandState {
     int* err = nullptr;   // one int: a simulate_sand_v3 prediction left the grid
     float* stage = nullptr;  // 6 n floats: packed positions and velocities for downloads
     bool stepped = false;
+    // fixed index-buffer capacity (lus_sand_set_list_capacity): the step
+    // enqueues without waiting; *fail marks a step whose lists did not fit
+    long fixed_cap = 0;
+    int* fail = nullptr;
+    // hipGraph replay of whole steps (fixed capacity only), one graph per
+    // launch configuration: the step's arguments and buffer assignment
+    struct Graph {
+        float dt = 0.0f;
+        int attract = 0, blow = 0, prev_attract = 0, variant = 0;
+        float player[3] = {0.0f, 0.0f, 0.0f};
+        const void* x = nullptr;
+        const void* A = nullptr;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    bool use_graph = false;
+    std::vector<Graph> graphs;
 };
+
+struct LusGpu;
+namespace {
+void sand_drop_graphs(SandState& S);
+int sand_check(LusGpu* c);
+}
 
 struct LusMlpTrainer;
 
@@ -664,6 +687,7 @@ void lus_gpu_destroy(LusGpu* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     pbf_drop_graph(c->pbf);
+    sand_drop_graphs(c->sand);
     for (LusMlpTrainer* t : c->trainers) {  // detach: the handles stay valid for destroy
         t->mem.release();
         t->ctx = nullptr;
@@ -711,7 +735,7 @@ int lus_gpu_stream_copy(LusGpu* c, size_t bytes, int reps, float* us) {
 int lus_gpu_sync(LusGpu* c) {
     if (!c) return set_error(LUS_EINVAL, "lus_gpu_sync: NULL context");
     LUS_HIP(hipStreamSynchronize(c->stream));
-    return LUS_OK;
+    return sand_check(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1882,8 +1906,15 @@ int lus_sand_init(LusGpu* c, const LusSandParams* p, int n_sand, const float* po
     LUS_HIP(hipSetDevice(c->device));
     SandState& S = c->sand;
     LUS_HIP(hipStreamSynchronize(c->stream));
+    sand_drop_graphs(S);
     S.mem.release();
     S.nbr_mem.release();
+    // everything that pointed into the released memory
+    S.nbr_idx = nullptr;
+    S.nbr_cap = 0;
+    S.fail = nullptr;
+    S.fixed_cap = 0;
+    S.use_graph = false;
     S.stage = nullptr;
     S.ready = false;
     S.stepped = false;
@@ -1979,30 +2010,52 @@ int lus_sand_init(LusGpu* c, const LusSandParams* p, int n_sand, const float* po
     return LUS_OK;
 }
 
-int lus_sand_step(LusGpu* c, float dt, int attract_flag, int blow_flag, const float* player) {
-    if (!c || !c->sand.ready) return set_error(LUS_ESTATE, "lus_sand_step: no sand scene");
-    if (!(dt != 0.0f) || ((attract_flag || blow_flag) && !player))
-        return set_error(LUS_EINVAL, "lus_sand_step: bad arguments");
-    LUS_HIP(hipSetDevice(c->device));
+namespace {
+
+void sand_drop_graphs(SandState& S) {
+    for (auto& g : S.graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.g) (void)hipGraphDestroy(g.g);
+    }
+    S.graphs.clear();
+}
+
+// A step enqueued without waiting reported its failure on the device: the
+// lists did not fit the fixed capacity, or (simulate_sand_v3) a predicted
+// position left the grid.  Reported once, with the flags cleared.
+int sand_check(LusGpu* c) {
+    SandState& S = c->sand;
+    // (the waiting step reports both itself)
+    if (!S.ready || !S.fail || S.fixed_cap <= 0) return LUS_OK;
+    int f[2] = {0, 0};
+    LUS_HIP(hipMemcpy(&f[0], S.fail, sizeof(int), hipMemcpyDeviceToHost));
+    if (S.err) LUS_HIP(hipMemcpy(&f[1], S.err, sizeof(int), hipMemcpyDeviceToHost));
+    if (f[0]) LUS_HIP(hipMemset(S.fail, 0, sizeof(int)));
+    if (f[1]) LUS_HIP(hipMemset(S.err, 0, sizeof(int)));
+    if (f[1])
+        return set_error(LUS_EINVAL, "lus_sand_step: a predicted position's cell lies outside the grid "
+                                     "(simulate_sand_v3 does not clamp); the state is invalid");
+    if (f[0])
+        return set_error(LUS_ESTATE, "lus_sand_step: the neighbour lists exceeded the fixed capacity of %ld "
+                                     "entries (lus_sand_set_list_capacity); the state is invalid",
+                         S.fixed_cap);
+    return LUS_OK;
+}
+
+// Enqueues one simulate_sand step.  Without a fixed capacity it waits once
+// for the lists' total length (it sizes their buffer); with one it does not
+// wait, and a step whose lists do not fit is reported by sand_check.  The
+// host-side buffer swaps are applied by sand_after.
+int sand_enqueue(LusGpu* c, float dt, const lus::SandK& k) {
     SandState& S = c->sand;
     hipStream_t s = c->stream;
     const int n = S.n;
-    lus::SandK k = S.k;
-    k.attract = attract_flag ? 1 : 0;
-    k.blow = blow_flag ? 1 : 0;
-    k.prev_attract = S.prev_attract;
-    for (int a = 0; a < 3; ++a) k.player[a] = player ? player[a] : 0.0f;
-    // phase events (profiling): start, after predict, after the lists, after
-    // the substeps, after the velocity update
+    const bool fixed = S.fixed_cap > 0;
     EventSet pe;
     auto mark = [&](int q) -> hipError_t { return c->profiling ? hipEventRecord(pe.e[q], s) : hipSuccess; };
     if (c->profiling) LUS_HIP(pe.create(5));
     LUS_HIP(mark(0));
-    if (k.variant == 2) {  // simulate_sand_v3: dt = clamp(dt, 0.001, 0.016) (src/Simulate.cpp:522)
-        const float t = dt < 0.001f ? 0.001f : dt;
-        dt = 0.016f < t ? 0.016f : t;
-        LUS_HIP(hipMemsetAsync(S.err, 0, sizeof(int), s));
-    }
+    if (k.variant == 2 && !fixed) LUS_HIP(lus::launch_zero_ints(S.err, 1, s));
     LUS_HIP(lus::launch_sand_predict_key(S.x, S.v, S.att, n, k, dt, S.err, S.xs, S.key, S.slot, S.counts,
                                          s));
     LUS_HIP(mark(1));
@@ -2014,30 +2067,33 @@ int lus_sand_step(LusGpu* c, float dt, int attract_flag, int blow_flag, const fl
     LUS_HIP(lus::launch_sand_nbr(S.A, S.key2, S.cs, n, S.solid, S.ocs, k, nullptr, S.nbr_cnt,
                                  S.nbr_cnt1, nullptr, 0, s));
     LUS_HIP(lus::launch_exclusive_scan(S.nbr_cnt, S.nbr_off, n + 1, S.ntiles, 0, s));
-    int total = 0, bad_key = 0;
-    LUS_HIP(hipMemcpyAsync(&total, S.nbr_off + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    if (k.variant == 2) LUS_HIP(hipMemcpyAsync(&bad_key, S.err, sizeof(int), hipMemcpyDeviceToHost, s));
-    LUS_HIP(hipStreamSynchronize(s));
-    if (bad_key)
-        return set_error(LUS_EINVAL,
-                         "lus_sand_step: a predicted position's cell lies outside the grid (simulate_sand_v3 "
-                         "does not clamp; the reference indexes outside its cell vector there)");
-    if (total < 0) return set_error(LUS_EHIP, "lus_sand_step: neighbor count overflow");
-    if ((long)total > S.nbr_cap || !S.nbr_idx) {
-        S.nbr_mem.release();
-        long cap = std::max<long>((long)total + total / 4, 64);
-        LUS_HIP(S.nbr_mem.alloc(&S.nbr_idx, (size_t)cap));
-        S.nbr_cap = cap;
+    if (!fixed) {
+        int total = 0, bad_key = 0;
+        LUS_HIP(hipMemcpyAsync(&total, S.nbr_off + n, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (k.variant == 2) LUS_HIP(hipMemcpyAsync(&bad_key, S.err, sizeof(int), hipMemcpyDeviceToHost, s));
+        LUS_HIP(hipStreamSynchronize(s));
+        if (bad_key)
+            return set_error(LUS_EINVAL,
+                             "lus_sand_step: a predicted position's cell lies outside the grid (simulate_sand_v3 "
+                             "does not clamp; the reference indexes outside its cell vector there)");
+        if (total < 0) return set_error(LUS_EHIP, "lus_sand_step: neighbor count overflow");
+        if ((long)total > S.nbr_cap || !S.nbr_idx) {
+            S.nbr_mem.release();
+            long cap = std::max<long>((long)total + total / 4, 64);
+            LUS_HIP(S.nbr_mem.alloc(&S.nbr_idx, (size_t)cap));
+            S.nbr_cap = cap;
+        }
     }
     LUS_HIP(lus::launch_sand_nbr(S.A, S.key2, S.cs, n, S.solid, S.ocs, k, S.nbr_off, S.nbr_cnt,
-                                 S.nbr_cnt1, S.nbr_idx, 1, s));
+                                 S.nbr_cnt1, S.nbr_idx, 1, s, S.nbr_cap, fixed ? S.fail : nullptr));
     LUS_HIP(mark(2));
     float4* A = S.A;
     float4* B = S.B;
     // simulate_sand_v3's substeps all read the same positions_tmp: one pass
     const int passes = k.variant == 2 ? (S.p.substeps > 0 ? 1 : 0) : S.p.substeps;
     for (int it = 0; it < passes; ++it) {
-        LUS_HIP(lus::launch_sand_substep(A, S.x2, S.solid, S.nbr_off, S.nbr_idx, n, k, B, S.att2, s));
+        LUS_HIP(lus::launch_sand_substep(A, S.x2, S.solid, S.nbr_off, S.nbr_idx, n, k, B, S.att2, s,
+                                         fixed ? S.fail : nullptr));
         std::swap(A, B);
     }
     LUS_HIP(mark(3));
@@ -2053,14 +2109,123 @@ int lus_sand_step(LusGpu* c, float dt, int attract_flag, int blow_flag, const fl
         const double sec[kProfSlots] = {all, 0.0, all, pe.seconds(1, 2), pe.seconds(2, 3), 0.0, pe.seconds(3, 4)};
         prof_store(sec);
     }
-    S.A = A;
-    S.B = B;
-    S.final_iter = A;
+    return LUS_OK;
+}
+
+// The buffer swaps and flags a step leaves behind (after sand_enqueue or a
+// replayed graph of it)
+void sand_after(SandState& S, const lus::SandK& k) {
+    const int passes = k.variant == 2 ? (S.p.substeps > 0 ? 1 : 0) : S.p.substeps;
+    if (passes & 1) std::swap(S.A, S.B);
+    S.final_iter = S.A;
     std::swap(S.x, S.x2);
     std::swap(S.v, S.v2);
     std::swap(S.att, S.att2);
     if (k.variant != 2) S.prev_attract = k.attract;  // v3 keeps no previous flag
     S.stepped = true;
+}
+
+}  // namespace
+
+int lus_sand_step(LusGpu* c, float dt, int attract_flag, int blow_flag, const float* player) {
+    if (!c || !c->sand.ready) return set_error(LUS_ESTATE, "lus_sand_step: no sand scene");
+    if (!(dt != 0.0f) || ((attract_flag || blow_flag) && !player))
+        return set_error(LUS_EINVAL, "lus_sand_step: bad arguments");
+    LUS_HIP(hipSetDevice(c->device));
+    SandState& S = c->sand;
+    lus::SandK k = S.k;
+    k.attract = attract_flag ? 1 : 0;
+    k.blow = blow_flag ? 1 : 0;
+    k.prev_attract = S.prev_attract;
+    // the player position matters only with a flag set (a graph key then
+    // ignores it otherwise)
+    for (int a = 0; a < 3; ++a) k.player[a] = (player && (k.attract || k.blow)) ? player[a] : 0.0f;
+    if (k.variant == 2) {  // simulate_sand_v3: dt = clamp(dt, 0.001, 0.016) (src/Simulate.cpp:522)
+        const float t = dt < 0.001f ? 0.001f : dt;
+        dt = 0.016f < t ? 0.016f : t;
+    }
+    if (!S.use_graph || S.fixed_cap <= 0 || c->profiling) {
+        if (int r = sand_enqueue(c, dt, k)) return r;
+        sand_after(S, k);
+        return LUS_OK;
+    }
+    SandState::Graph key;
+    key.dt = dt;
+    key.attract = k.attract;
+    key.blow = k.blow;
+    key.prev_attract = k.prev_attract;
+    key.variant = k.variant;
+    for (int a = 0; a < 3; ++a) key.player[a] = k.player[a];
+    key.x = S.x;
+    key.A = S.A;
+    SandState::Graph* hit = nullptr;
+    for (auto& g : S.graphs)
+        if (g.dt == key.dt && g.attract == key.attract && g.blow == key.blow && g.prev_attract == key.prev_attract &&
+            g.variant == key.variant && g.player[0] == key.player[0] && g.player[1] == key.player[1] &&
+            g.player[2] == key.player[2] && g.x == key.x && g.A == key.A)
+            hit = &g;
+    if (!hit) {
+        // a step's buffer assignments alternate, so a few graphs cover a run;
+        // more configurations than that (a moving player with a flag set)
+        // start over
+        if (S.graphs.size() >= 8) sand_drop_graphs(S);
+        LUS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int r = sand_enqueue(c, dt, k);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        LUS_HIP(e);
+        key.g = g;
+        hipError_t ei = hipGraphInstantiate(&key.exec, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            LUS_HIP(ei);
+        }
+        S.graphs.push_back(key);
+        hit = &S.graphs.back();
+    }
+    LUS_HIP(hipGraphLaunch(hit->exec, c->stream));
+    sand_after(S, k);
+    return LUS_OK;
+}
+
+int lus_sand_set_list_capacity(LusGpu* c, long entries) {
+    if (!c || !c->sand.ready) return set_error(LUS_ESTATE, "lus_sand_set_list_capacity: no sand scene");
+    if (entries < 0 || entries > (long)INT_MAX)
+        return set_error(LUS_EINVAL, "lus_sand_set_list_capacity: entries must be in [0, %d]", INT_MAX);
+    LUS_HIP(hipSetDevice(c->device));
+    SandState& S = c->sand;
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    if (int r = sand_check(c)) return r;
+    sand_drop_graphs(S);
+    if (entries == 0) {
+        S.fixed_cap = 0;
+        S.use_graph = false;
+        return LUS_OK;
+    }
+    S.nbr_mem.release();
+    S.nbr_idx = nullptr;
+    S.nbr_cap = 0;
+    LUS_HIP(S.nbr_mem.alloc(&S.nbr_idx, (size_t)entries));
+    S.nbr_cap = entries;
+    if (!S.fail) LUS_HIP(S.mem.alloc(&S.fail, 1));
+    LUS_HIP(hipMemset(S.fail, 0, sizeof(int)));
+    S.fixed_cap = entries;
+    return LUS_OK;
+}
+
+int lus_sand_set_graph(LusGpu* c, int on) {
+    if (!c || !c->sand.ready) return set_error(LUS_ESTATE, "lus_sand_set_graph: no sand scene");
+    SandState& S = c->sand;
+    if (on && S.fixed_cap <= 0)
+        return set_error(LUS_ESTATE, "lus_sand_set_graph: needs a fixed list capacity (lus_sand_set_list_capacity)");
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    sand_drop_graphs(S);
+    S.use_graph = on != 0;
     return LUS_OK;
 }
 
@@ -2068,6 +2233,9 @@ int lus_sand_set_variant(LusGpu* c, int variant) {
     if (!c || !c->sand.ready) return set_error(LUS_ESTATE, "lus_sand_set_variant: no sand scene");
     if (variant < 0 || variant > 2) return set_error(LUS_EINVAL, "lus_sand_set_variant: variant %d", variant);
     SandState& S = c->sand;
+    LUS_HIP(hipSetDevice(c->device));
+    LUS_HIP(hipStreamSynchronize(c->stream));
+    sand_drop_graphs(S);  // the constants are kernel arguments of the captured steps
     lus::SandK& k = S.k;
     // the locals of each function (src/Simulate.cpp:159-163, 330-334, 516-520)
     static const float coeff[3][4] = {{0.8f, 0.7f, 0.95f, 0.9f}, {0.8f, 0.7f, 0.8f, 0.7f}, {0.8f, 0.5f, 0.8f, 0.8f}};
@@ -2105,6 +2273,7 @@ int lus_sand_download(LusGpu* c, float* pos, float* pos_star, float* vel, int* a
     if (attracted) LUS_HIP(hipMemcpyAsync(attracted, S.att, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     if (perm) LUS_HIP(hipMemcpyAsync(perm, S.perm, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     LUS_HIP(hipStreamSynchronize(s));
+    if (int r = sand_check(c)) return r;
     // positions_star equals positions after a step (src/Simulate.cpp:316-319)
     // and before the first one (both are the uploaded positions)
     if (pos && pos_star) memcpy(pos_star, pos, sizeof(float) * 3 * (size_t)n);
@@ -2350,6 +2519,7 @@ int lus_sand_neighbors(LusGpu* c, long* off, int* idx, long cap, long* total) {
     SandState& S = c->sand;
     LUS_HIP(hipSetDevice(c->device));
     LUS_HIP(hipStreamSynchronize(c->stream));
+    if (int r = sand_check(c)) return r;
     const int n = S.n;
     std::vector<int> o((size_t)n + 1);
     LUS_HIP(hipMemcpy(o.data(), S.nbr_off, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));

@@ -297,12 +297,16 @@ hipError_t launch_sand_permute(const int* tmp, const int* key, const int* cell_s
                                const float4* x, const float4* xs, const float4* v, const int* att,
                                int n, float4* x2, float4* xs2, float4* v2, int* att2, int* perm,
                                int* key2, hipStream_t s);
+// fill with fail non-null: idx holds cap entries; a total past cap (or a
+// negative one) writes nothing and sets *fail, and a substep with *fail set
+// does nothing
 hipError_t launch_sand_nbr(const float4* xs, const int* key2, const int* sand_cs, int n,
                            const float4* solid, const int* solid_cs, SandK k, const int* off,
-                           int* cnt, int* cnt1, int* idx, int fill, hipStream_t s);
+                           int* cnt, int* cnt1, int* idx, int fill, hipStream_t s, long cap = 0,
+                           int* fail = nullptr);
 hipError_t launch_sand_substep(const float4* A, const float4* xold, const float4* solid,
                                const int* off, const int* idx, int n, SandK k, float4* B, int* att,
-                               hipStream_t s);
+                               hipStream_t s, const int* fail = nullptr);
 hipError_t launch_sand_vupdate(const float4* A, float4* xold_to_new, float4* vout, int n, float dt,
                                hipStream_t s);
 

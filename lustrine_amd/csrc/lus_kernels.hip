@@ -725,9 +725,19 @@ __global__ __launch_bounds__(kBlock) void k_sand_nbr(const float4* __restrict__ 
                                                       const int* __restrict__ off,
                                                       int* __restrict__ cnt,
                                                       int* __restrict__ cnt1,
-                                                      int* __restrict__ idx, int fill) {
+                                                      int* __restrict__ idx, int fill,
+                                                      long cap, int* __restrict__ fail) {
     const int i = xcd_index();
     if (i >= n) return;
+    // a fixed index buffer (fail non-null): a total past its capacity (or
+    // past the int range) writes nothing and marks the step failed
+    if (fill && fail) {
+        const int tot = off[n];
+        if (tot < 0 || (long)tot > cap) {
+            if (i == 0) *fail = 1;
+            return;
+        }
+    }
     const float4 me = xs[i];
     const int c = key2[i];
     const int yy = c / k.gXZ;
@@ -816,9 +826,9 @@ __global__ __launch_bounds__(kBlock) void k_sand_nbr(const float4* __restrict__ 
 
 hipError_t launch_sand_nbr(const float4* xs, const int* key2, const int* sand_cs, int n,
                            const float4* solid, const int* solid_cs, SandK k, const int* off,
-                           int* cnt, int* cnt1, int* idx, int fill, hipStream_t s) {
+                           int* cnt, int* cnt1, int* idx, int fill, hipStream_t s, long cap, int* fail) {
     launch_kernel(k_sand_nbr, dim3(grid_for(n)), dim3(kBlock), 0, s, xs, key2, sand_cs, n, solid,
-                       solid_cs, k, off, cnt, cnt1, idx, fill);
+                       solid_cs, k, off, cnt, cnt1, idx, fill, cap, fail);
     return hipGetLastError();
 }
 
@@ -942,7 +952,9 @@ __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restric
                                                           const int* __restrict__ off,
                                                           const int* __restrict__ idx, int n,
                                                           SandK k, float4* __restrict__ B,
-                                                          int* __restrict__ att) {
+                                                          int* __restrict__ att,
+                                                          const int* __restrict__ fail) {
+    if (fail && *fail) return;  // the lists were not written (uniform)
     const int i = xcd_index();
     if (i >= n) return;
     const float4 pi = A[i];
@@ -987,9 +999,9 @@ __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restric
 
 hipError_t launch_sand_substep(const float4* A, const float4* xold, const float4* solid,
                                const int* off, const int* idx, int n, SandK k, float4* B, int* att,
-                               hipStream_t s) {
+                               hipStream_t s, const int* fail) {
     launch_kernel(k_sand_substep, dim3(grid_for(n)), dim3(kBlock), 0, s, A, xold, solid, off,
-                       idx, n, k, B, att);
+                       idx, n, k, B, att, fail);
     return hipGetLastError();
 }
 

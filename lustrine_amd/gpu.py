@@ -128,6 +128,8 @@ _SIGS = {
     "lus_mlp_infer": (C.c_int, [C.c_void_p, _f, C.c_int, _f, _f]),
     "lus_sand_init": (C.c_int, [C.c_void_p, _P(LusSandParams), C.c_int, _f, _f, _i, C.c_int, _f, _i]),
     "lus_sand_step": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int, _f]),
+    "lus_sand_set_list_capacity": (C.c_int, [C.c_void_p, C.c_long]),
+    "lus_sand_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_sand_download": (C.c_int, [C.c_void_p, _f, _f, _f, _i, _i]),
     "lus_sand_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "lus_sand_neighbors": (C.c_int, [C.c_void_p, _l, _i, C.c_long, _l]),
@@ -521,6 +523,14 @@ class Gpu:
     def sand_step(self, dt, attract=False, blow=False, player=None):
         pl = None if player is None else _f32(player, (3,))
         _check(self._lib.lus_sand_step(self._h, dt, int(attract), int(blow), _ptr_f(pl)), "lus_sand_step")
+
+    def sand_set_list_capacity(self, entries):
+        """A fixed neighbour-list buffer of `entries` (steps then do not wait); 0: the growing buffer."""
+        _check(self._lib.lus_sand_set_list_capacity(self._h, int(entries)), "lus_sand_set_list_capacity")
+
+    def sand_set_graph(self, on=True):
+        """Replay each sand step as a hipGraph (needs a fixed list capacity)."""
+        _check(self._lib.lus_sand_set_graph(self._h, int(bool(on))), "lus_sand_set_graph")
 
     def sand_set_variant(self, variant):
         """0 simulate_sand, 1 simulate_sand_credits, 2 simulate_sand_v3."""
