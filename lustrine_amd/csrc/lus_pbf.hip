@@ -2272,17 +2272,27 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 //   f[29]       mean |d| / h over N(i) \ i
 //   f[30]       mean |d|^2 / h^2 over N(i) \ i
 //   f[31]       min(1, distance to the nearest wall of the clamp box / h)
-template <bool kFast, bool kRecord, class Visit>
+// kLdsBins: this thread's 8 bin accumulators (x, y, z, count) live in LDS,
+// bin b at bins[b * kBlock]: one read-modify-write per neighbour instead of a
+// one-hot update of 32 registers (the same additions in the same order)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) f32x4 lds_float4;
+
+template <bool kFast, bool kRecord, bool kLdsBins, class Visit>
 __device__ __forceinline__ void features_body(int i, const float4* __restrict__ xp, const float4* __restrict__ xold,
                                               const PbfK& k, const float* __restrict__ mu,
                                               const float* __restrict__ inv_sigma, bf16x8_t* __restrict__ fbf,
                                               float* __restrict__ frec, int rec_stride, int rec_off, float inv_h,
-                                              Visit&& visit) {
+                                              Visit&& visit, lds_float4* bins = nullptr) {
     const float4 pi = xp[i];
     float rho = 0.0f, cnt = 0.0f, sr = 0.0f, sr2 = 0.0f;
     float bx[8], by[8], bz[8], bn[8];
 #pragma unroll
     for (int b = 0; b < 8; ++b) bx[b] = by[b] = bz[b] = bn[b] = 0.0f;
+    if constexpr (kLdsBins) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) bins[b * kBlock] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
     visit([&](int j, const float4 pj, const float4) {
         const float dx = pj.x - pi.x;
         const float dy = pj.y - pi.y;
@@ -2294,19 +2304,39 @@ __device__ __forceinline__ void features_body(int i, const float4* __restrict__ 
             const float q = sqrtf(r2) * inv_h;
             int b = (int)(q * 8.0f);
             b = b < 7 ? b : 7;
+            if constexpr (kLdsBins) {
+                lds_float4* __restrict__ p = bins + b * kBlock;
+                f32x4 v = *p;
+                v.x += dx;
+                v.y += dy;
+                v.z += dz;
+                v.w += 1.0f;
+                *p = v;
+            } else {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const float on = (b == u) ? 1.0f : 0.0f;
-                bx[u] += on * dx;
-                by[u] += on * dy;
-                bz[u] += on * dz;
-                bn[u] += on;
+                for (int u = 0; u < 8; ++u) {
+                    const float on = (b == u) ? 1.0f : 0.0f;
+                    bx[u] += on * dx;
+                    by[u] += on * dy;
+                    bz[u] += on * dz;
+                    bn[u] += on;
+                }
             }
             cnt += 1.0f;
             sr += q;
             sr2 += q * q;
         }
     });
+    if constexpr (kLdsBins) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const f32x4 v = bins[b * kBlock];
+            bx[b] = v.x;
+            by[b] = v.y;
+            bz[b] = v.z;
+            bn[b] = v.w;
+        }
+    }
     float f[kFeat];
     f[0] = rho * k.c_rho - 1.0f;
     const float4 xo = xold[i];
@@ -2354,12 +2384,13 @@ __global__ __launch_bounds__(kBlock) void k_pbf_features(
     unsigned long long* __restrict__ masks, const float4* __restrict__ xp,
     const float4* __restrict__ xold, const float* __restrict__ mu, const float* __restrict__ inv_sigma,
     bf16x8_t* __restrict__ fbf, float* __restrict__ frec, int rec_stride, int rec_off, float inv_h) {
+    __shared__ float4 bins[8 * kBlock];  // blockDim.x <= kBlock
     const int i = xcd_index();
     if (i >= n) return;
-    features_body<kFast, kRecord>(i, xp, xold, k, mu, inv_sigma, fbf, frec, rec_stride, rec_off, inv_h,
+    features_body<kFast, kRecord, true>(i, xp, xold, k, mu, inv_sigma, fbf, frec, rec_stride, rec_off, inv_h,
                                   [&](auto&& f) {
         for_each_neighbor<true>(i, n, key2, cs, masks, xp, xp, nullptr, k, f);
-    });
+    }, (lds_float4*)bins + threadIdx.x);
 }
 
 template <bool kFast>
@@ -2376,7 +2407,7 @@ __global__ __launch_bounds__(kTileT) void k_pbf_features_t(
         tile_stage(S, tiles[t], cs, k, xp, nullptr, lds, nullptr, lcap);
         const int i = S.i0 + (int)threadIdx.x;
         if (i < S.i1)
-            features_body<kFast, false>(i, xp, xold, k, mu, inv_sigma, fbf, nullptr, 1, 0, inv_h, [&](auto&& f) {
+            features_body<kFast, false, false>(i, xp, xold, k, mu, inv_sigma, fbf, nullptr, 1, 0, inv_h, [&](auto&& f) {
                 for_each_neighbor_tiled<true>(S, i, n, key2, masks, xp, xp, nullptr, lds, nullptr, k, f);
             });
         __syncthreads();
