@@ -1986,6 +1986,9 @@ int lus_sand_init(LusGpu* c, const LusSandParams* p, int n_sand, const float* po
     LUS_HIP(hipMemcpy(S.att, ha.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     LUS_HIP(hipMemset(S.counts, 0, sizeof(int) * (S.C + 1)));
     LUS_HIP(hipMemset(S.nbr_cnt, 0, sizeof(int) * (n + 1)));
+    // the out-of-grid flag is read by every no-wait step's check, whatever the
+    // variant: it starts clear (a new allocation may hold an old context's data)
+    LUS_HIP(hipMemset(S.err, 0, sizeof(int)));
     // static solids: stable cell sort once (uniform_grid_cells_static_saved,
     // src/neighbors/Neighbors.cpp:266-272, keeps ascending slot order per cell)
     SortedSet O;

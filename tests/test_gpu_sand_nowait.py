@@ -76,3 +76,27 @@ def test_capacity_overflow_is_reported(gpu_mod):
         g.sand_step(np.float32(0.016), False, False, None)
         x = g.sand_download()[0]
         assert np.isfinite(x).all()
+
+
+def test_no_wait_check_starts_clear_after_a_v3_error(gpu_mod):
+    """Contexts made after one that reported a simulate_sand_v3 out-of-grid
+    prediction (and freed its flag with the error still marked): their
+    no-wait steps report nothing, whatever memory their flag reuses."""
+    pos = scenes.jittered_lattice(2, 2, 2, origin=(6.0, 3.0, 6.0), amplitude=0.2, seed=1)
+    sp = gpu_mod.sand_params(20, 20, 20, kernel_radius_scale=3.1)
+    vel = np.zeros_like(pos)
+    vel[0] = [0.0, -5000.0, 0.0]
+    for _ in range(3):
+        with gpu_mod.Gpu(0) as g:
+            g.sand_init(sp, pos, vel, None, None, None)
+            g.sand_set_variant(2)
+            with pytest.raises(gpu_mod.LusError):
+                g.sand_step(np.float32(0.016), False, False, None)
+        for variant in (0, 2):
+            with gpu_mod.Gpu(0) as g:
+                g.sand_init(sp, pos, None, None, None, None)
+                g.sand_set_variant(variant)
+                g.sand_set_list_capacity(4096)
+                g.sand_step(np.float32(0.016), False, False, None)
+                g.sync()
+                assert np.isfinite(g.sand_download()[0]).all()
