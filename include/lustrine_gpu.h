@@ -51,8 +51,9 @@ typedef struct LusPbfParams {
     /* 1: every float operation in the order of the CPU oracle, IEEE divide and
      *    sqrt, no fused multiply-add (results bit-identical to the oracle);
      * 0: fused multiply-add and hardware reciprocal square root in the
-     *    neighbour loops (results within a few ulp of mode 1).  The
-     *    neighbour sets are built with the exact test in both modes. */
+     *    neighbour loops, and h^2 - r^2 by fused multiply-adds in the
+     *    neighbour test (results within a few ulp of mode 1; the two tests
+     *    differ only for pairs within rounding of r = h). */
     int exact;
 } LusPbfParams;
 

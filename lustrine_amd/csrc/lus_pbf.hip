@@ -29,7 +29,10 @@
 //          FMA: bit-identical to the CPU oracle;
 //   fast   fused multiply-adds and the hardware reciprocal square root
 //          (v_rsq_f32) replace sqrt + divide in the neighbour loops.
-// The neighbour-set test is the exact one in both modes.
+// The neighbour test follows the mode: exact, r2 = dx dx + dy dy + dz dz rounded
+// at every step and r2 <= h2; fast, h2 - r2 by three fused multiply-adds and a
+// hit where it is >= 0 (the two differ only for pairs within rounding of r = h,
+// where every kernel weight and gradient vanishes).
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -119,16 +122,30 @@ __device__ __forceinline__ int row_key(int y, int x, int c, const PbfK& k) {
     return ((y * k.gX + x) * k.nph + c) * k.gZ;
 }
 
-// the exact neighbour test (no FMA, oracle order)
+// h2 - r2 of the fast mode's neighbour test: three fused multiply-adds
+__device__ __forceinline__ float h2_less_r2_fast(float dx, float dy, float dz, const PbfK& k) {
+    return __builtin_fmaf(-dz, dz, __builtin_fmaf(-dy, dy, __builtin_fmaf(-dx, dx, k.h2)));
+}
+
+// the neighbour test of the mode kFast: exact (no FMA, oracle order: r2 <=
+// h2) or fast (h2 - r2 >= 0 by fused multiply-adds)
+template <bool kFast = false>
 __device__ __forceinline__ bool within_h(const float4 a, const float4 b, const PbfK& k) {
     const float dx = a.x - b.x;
     const float dy = a.y - b.y;
     const float dz = a.z - b.z;
+    if constexpr (kFast) return h2_less_r2_fast(dx, dy, dz, k) >= 0.0f;
     const float r2 = dx * dx + dy * dy + dz * dz;
     return r2 <= k.h2;
 }
+template <bool kFast = false>
 __device__ __forceinline__ bool within_h(const float4 a, const float3 b, const PbfK& k) {
-    return within_h(a, make_float4(b.x, b.y, b.z, 0.0f), k);
+    return within_h<kFast>(a, make_float4(b.x, b.y, b.z, 0.0f), k);
+}
+// the neighbour test of the scene's mode (k.exact), for loops that are not
+// specialised on it
+__device__ __forceinline__ bool within_h_k(const float4 a, const float4 b, const PbfK& k) {
+    return k.exact ? within_h<false>(a, b, k) : within_h<true>(a, b, k);
 }
 // x, y, z of slot t of a particle array (one 12-byte load: a candidate test
 // reads no more)
@@ -271,11 +288,11 @@ struct ListWriter {
 // longer row slot by slot -- and call f(j, cur[j], aux[j]) on every hit while
 // appending it to i's list (ListWriter; the lanes of a wave write whole
 // segments of a group).  Writes the header hdr[i].
-template <class F>
-__device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, const int* __restrict__ cs,
-                                           const float4* __restrict__ xp, const PbfK& k, int4* __restrict__ hdr,
-                                           int4* __restrict__ lst, const float4* __restrict__ cur,
-                                           const float4* __restrict__ aux, F& f) {
+template <bool kFast, class F>
+__device__ __forceinline__ void build_list_t(int i, const int* __restrict__ key2, const int* __restrict__ cs,
+                                             const float4* __restrict__ xp, const PbfK& k, int4* __restrict__ hdr,
+                                             int4* __restrict__ lst, const float4* __restrict__ cur,
+                                             const float4* __restrict__ aux, F& f) {
     int cy, cx, cz;
     key_cell(key2[i] & ~kOvf, k, cy, cx, cz);
     const int z0 = cz > 0 ? cz - 1 : 0;
@@ -300,7 +317,7 @@ __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, 
             const int L = j1 - j0;
             if (L > 64) {
                 for (int j = j0; j < j1; ++j)
-                    if (within_h(pi, xp[j], k)) emit(j, at4(cur, j), aux_at(aux, j));
+                    if (within_h<kFast>(pi, xp[j], k)) emit(j, at4(cur, j), aux_at(aux, j));
                 continue;
             }
             unsigned long long mm = 0ull;
@@ -309,12 +326,12 @@ __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, 
             for (; t + 4 <= L; t += 4) {
                 const float3 c0 = xyz_at(base, t), c1 = xyz_at(base, t + 1), c2 = xyz_at(base, t + 2),
                              c3 = xyz_at(base, t + 3);
-                mm |= within_h(pi, c0, k) ? (1ull << t) : 0ull;
-                mm |= within_h(pi, c1, k) ? (2ull << t) : 0ull;
-                mm |= within_h(pi, c2, k) ? (4ull << t) : 0ull;
-                mm |= within_h(pi, c3, k) ? (8ull << t) : 0ull;
+                mm |= within_h<kFast>(pi, c0, k) ? (1ull << t) : 0ull;
+                mm |= within_h<kFast>(pi, c1, k) ? (2ull << t) : 0ull;
+                mm |= within_h<kFast>(pi, c2, k) ? (4ull << t) : 0ull;
+                mm |= within_h<kFast>(pi, c3, k) ? (8ull << t) : 0ull;
             }
-            for (; t < L; ++t) mm |= within_h(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
+            for (; t < L; ++t) mm |= within_h<kFast>(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
             if (__any((int)((mm >> 32) != 0ull)))
                 visit_row<unsigned long long>(mm, j0, 0, cur, aux, emit);
             else
@@ -322,6 +339,17 @@ __device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, 
         }
     }
     lw.finish(i, hdr, lst);
+}
+
+template <class F>
+__device__ __forceinline__ void build_list(int i, const int* __restrict__ key2, const int* __restrict__ cs,
+                                           const float4* __restrict__ xp, const PbfK& k, int4* __restrict__ hdr,
+                                           int4* __restrict__ lst, const float4* __restrict__ cur,
+                                           const float4* __restrict__ aux, F& f) {
+    if (k.exact)
+        build_list_t<false>(i, key2, cs, xp, k, hdr, lst, cur, aux, f);
+    else
+        build_list_t<true>(i, key2, cs, xp, k, hdr, lst, cur, aux, f);
 }
 
 // Walk forms of visit_list: kWalkWide, one group of eight entries per step
@@ -454,7 +482,7 @@ __device__ __forceinline__ void for_each_neighbor(int i, int n, int* __restrict_
             const int j0 = cs[row + z0];
             const int j1 = cs[row + z1 + 1];
             for (int j = j0; j < j1; ++j)
-                if (within_h(pi, xp[j], k)) f(j, cur[j], aux_at(aux, j));
+                if (within_h_k(pi, xp[j], k)) f(j, cur[j], aux_at(aux, j));
         }
     }
 }
@@ -573,7 +601,7 @@ __device__ __forceinline__ void for_each_neighbor_tiled_impl(const TileShared& S
         const int delta = kStaged ? S.lo[r] - S.rb[r] : 0;
         if (ovf) {
             for (int j = j0; j < j1; ++j)
-                if (within_h(pi, xp[j], k)) f(j, src[j + delta], aux_at(src_aux, j + delta));
+                if (within_h_k(pi, xp[j], k)) f(j, src[j + delta], aux_at(src_aux, j + delta));
             continue;
         }
         unsigned long long mm;
@@ -585,12 +613,12 @@ __device__ __forceinline__ void for_each_neighbor_tiled_impl(const TileShared& S
             int t = 0;
             for (; t + 4 <= L; t += 4) {
                 const float4 c0 = base[t], c1 = base[t + 1], c2 = base[t + 2], c3 = base[t + 3];
-                mm |= within_h(pi, c0, k) ? (1ull << t) : 0ull;
-                mm |= within_h(pi, c1, k) ? (2ull << t) : 0ull;
-                mm |= within_h(pi, c2, k) ? (4ull << t) : 0ull;
-                mm |= within_h(pi, c3, k) ? (8ull << t) : 0ull;
+                mm |= within_h_k(pi, c0, k) ? (1ull << t) : 0ull;
+                mm |= within_h_k(pi, c1, k) ? (2ull << t) : 0ull;
+                mm |= within_h_k(pi, c2, k) ? (4ull << t) : 0ull;
+                mm |= within_h_k(pi, c3, k) ? (8ull << t) : 0ull;
             }
-            for (; t < L; ++t) mm |= within_h(pi, base[t], k) ? (1ull << t) : 0ull;
+            for (; t < L; ++t) mm |= within_h_k(pi, base[t], k) ? (1ull << t) : 0ull;
             masks[(size_t)r * n + i] = mm;
         } else {
             mm = masks[(size_t)r * n + i];
@@ -982,16 +1010,19 @@ __device__ __forceinline__ int high_bit(unsigned int m) { return 31 - __builtin_
 __device__ __forceinline__ int high_bit(unsigned long long m) { return 63 - __builtin_clzll(m); }
 
 // o shifted left by one with bit 0 = "staged candidate p is outside h of pi":
-// the sign of h2 - r2, with r2 exactly as within_h forms it (the rounded
+// the sign of h2 - r2 as within_h<kFast> forms it.  Exact mode: the rounded
 // difference of two floats has the sign of their exact difference and is +0
-// when they are equal, so the bit is 1 exactly when r2 > h2).  One funnel
-// shift replaces the compare, the select and the or of a mask update.
-template <int kS>
+// when they are equal, so the bit is 1 exactly when r2 > h2.  Fast mode: the
+// fused multiply-adds give -0 never (an exact zero sum rounds to +0), so the
+// bit is 1 exactly when h2 - r2 < 0.  One funnel shift replaces the compare,
+// the select and the or of a mask update.
+template <int kS, bool kFast>
 __device__ __forceinline__ unsigned int shift_outside(unsigned int o, const float4 pi, const lds_float* __restrict__ p,
                                                       const PbfK& k) {
     const float dx = pi.x - p[0];
     const float dy = pi.y - p[1];
     const float dz = pi.z - p[2];
+    if constexpr (kFast) return __builtin_amdgcn_alignbit(o, __float_as_uint(h2_less_r2_fast(dx, dy, dz, k)), 31u);
     const float r2 = dx * dx + dy * dy + dz * dz;
     return __builtin_amdgcn_alignbit(o, __float_as_uint(k.h2 - r2), 31u);
 }
@@ -999,33 +1030,33 @@ __device__ __forceinline__ unsigned int shift_outside(unsigned int o, const floa
 // Hit mask of the cnt (<= 32) staged candidates b[t]: shifted in from the
 // last to the first, so candidate t lands in bit t; the all-ones start leaves
 // ones (no hit) above bit cnt - 1.
-template <int kS>
+template <int kS, bool kFast>
 __device__ __forceinline__ unsigned int hits32_staged(const float4 pi, const lds_float* __restrict__ b, int cnt,
                                                       const PbfK& k) {
     unsigned int o = ~0u;
     int t = cnt;
     for (; t >= 4; t -= 4) {
         const lds_float* __restrict__ q = b + kS * (t - 4);
-        o = shift_outside<kS>(o, pi, q + 3 * kS, k);
-        o = shift_outside<kS>(o, pi, q + 2 * kS, k);
-        o = shift_outside<kS>(o, pi, q + kS, k);
-        o = shift_outside<kS>(o, pi, q, k);
+        o = shift_outside<kS, kFast>(o, pi, q + 3 * kS, k);
+        o = shift_outside<kS, kFast>(o, pi, q + 2 * kS, k);
+        o = shift_outside<kS, kFast>(o, pi, q + kS, k);
+        o = shift_outside<kS, kFast>(o, pi, q, k);
     }
-    for (; t > 0; --t) o = shift_outside<kS>(o, pi, b + kS * (t - 1), k);
+    for (; t > 0; --t) o = shift_outside<kS, kFast>(o, pi, b + kS * (t - 1), k);
     return ~o;
 }
 
 // Hit mask of the L (<= bits of M) staged candidates S[lo + t] of pi
-template <typename M, int kS>
+template <typename M, int kS, bool kFast>
 __device__ __forceinline__ M test_row_staged(const float4 pi, int lo, int L, const lds_float* __restrict__ S,
                                              const PbfK& k) {
     const lds_float* __restrict__ b = S + kS * lo;
     if constexpr (sizeof(M) == 4) {
-        return hits32_staged<kS>(pi, b, L, k);
+        return hits32_staged<kS, kFast>(pi, b, L, k);
     } else {
         const int nl = L < 32 ? L : 32;
-        const unsigned int l = hits32_staged<kS>(pi, b, nl, k);
-        const unsigned int h = hits32_staged<kS>(pi, b + 32 * kS, L - nl, k);
+        const unsigned int l = hits32_staged<kS, kFast>(pi, b, nl, k);
+        const unsigned int h = hits32_staged<kS, kFast>(pi, b + 32 * kS, L - nl, k);
         return (M)h << 32 | (M)l;
     }
 }
@@ -1192,14 +1223,14 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
                 if (L > 64) {
                     for (int t = 0; t < L; ++t) {
                         const float4 q = staged_at<kS>(S, lo + t);
-                        if (within_h(pi, q, k)) hit(j0 + t, q);
+                        if (within_h<kFast>(pi, q, k)) hit(j0 + t, q);
                     }
                 } else if (__all(L <= 32)) {
-                    const unsigned int mm = test_row_staged<unsigned int, kS>(pi, lo, L, S, k);
+                    const unsigned int mm = test_row_staged<unsigned int, kS, kFast>(pi, lo, L, S, k);
                     if (mm) lw.row_hits(j0 + low_bit(mm), j0 + high_bit(mm));
                     visit_row_staged<unsigned int, kS>(mm, j0, lo, S, hit_row);
                 } else {
-                    const unsigned long long mm = test_row_staged<unsigned long long, kS>(pi, lo, L, S, k);
+                    const unsigned long long mm = test_row_staged<unsigned long long, kS, kFast>(pi, lo, L, S, k);
                     if (mm) lw.row_hits(j0 + low_bit(mm), j0 + high_bit(mm));
                     if (__any((int)((mm >> 32) != 0ull)))
                         visit_row_staged<unsigned long long, kS>(mm, j0, lo, S, hit_row);
@@ -1211,7 +1242,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
                 if (L > 64) {
                     for (int j = j0; j < j1; ++j) {
                         const float4 q = at4(xp, j);
-                        if (within_h(pi, q, k)) hit(j, q);
+                        if (within_h<kFast>(pi, q, k)) hit(j, q);
                     }
                     continue;
                 }
@@ -1221,11 +1252,11 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
                 for (; t + 4 <= L; t += 4) {
                     const float3 c0 = xyz_at(base, t), c1 = xyz_at(base, t + 1), c2 = xyz_at(base, t + 2),
                                  c3 = xyz_at(base, t + 3);
-                    const unsigned int nib = (within_h(pi, c0, k) ? 1u : 0u) | (within_h(pi, c1, k) ? 2u : 0u) |
-                                             (within_h(pi, c2, k) ? 4u : 0u) | (within_h(pi, c3, k) ? 8u : 0u);
+                    const unsigned int nib = (within_h<kFast>(pi, c0, k) ? 1u : 0u) | (within_h<kFast>(pi, c1, k) ? 2u : 0u) |
+                                             (within_h<kFast>(pi, c2, k) ? 4u : 0u) | (within_h<kFast>(pi, c3, k) ? 8u : 0u);
                     mm |= (unsigned long long)nib << t;
                 }
-                for (; t < L; ++t) mm |= within_h(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
+                for (; t < L; ++t) mm |= within_h<kFast>(pi, xyz_at(base, t), k) ? (1ull << t) : 0ull;
                 auto hit4 = [&](int j, const float4 pj, const float4) { hit(j, pj); };
                 if (__any((int)((mm >> 32) != 0ull)))
                     visit_row<unsigned long long>(mm, j0, 0, xp, nullptr, hit4);
@@ -1849,7 +1880,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_sub(const float4* _
             const float3 c[4] = {xyz_at(xp, j), xyz_at(xp, j + 1), xyz_at(xp, j + 2), xyz_at(xp, j + 3)};
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (within_h(pi, c[u], k)) {
+                if (within_h<true>(pi, c[u], k)) {
                     const int jj = j + u;
                     first = first < 0 ? jj : first;
                     last = jj;
@@ -1860,7 +1891,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_sub(const float4* _
         }
         for (; j < j1; ++j) {
             const float3 c = xyz_at(xp, j);
-            if (within_h(pi, c, k)) {
+            if (within_h<true>(pi, c, k)) {
                 first = first < 0 ? j : first;
                 last = j;
                 ++cnt;
@@ -1898,7 +1929,7 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_sub(const float4* _
                     *place(p++) = (unsigned short)(j0 + low_bit(m) - first);
             } else {
                 for (int j = j0; j < j1; ++j)
-                    if (within_h(pi, xyz_at(xp, j), k)) *place(p++) = (unsigned short)(j - first);
+                    if (within_h<true>(pi, xyz_at(xp, j), k)) *place(p++) = (unsigned short)(j - first);
             }
         }
         if (layer == 2)
