@@ -2031,9 +2031,10 @@ hipError_t launch_pbf_xsph(const float4* xa, const float4* vt, int* key2, const 
 // within d, friction against the step's old positions.  Solids do not move.
 // The sand arithmetic is IEEE in both arithmetic modes.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void mix_sand_pair(const float4 pi, const float4 xoi, const float4 pj,
-                                              const float4* __restrict__ xoj_at, const PbfK& k, float& dpx,
-                                              float& dpy, float& dpz) {
+// xoj_of(): j's old position (its w: j's phase), asked for only for a contact
+template <class XoJ>
+__device__ __forceinline__ void mix_sand_pair(const float4 pi, const float4 xoi, const float4 pj, XoJ&& xoj_of,
+                                              const PbfK& k, float& dpx, float& dpy, float& dpz) {
     float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
     // most neighbours within h are beyond d: those whose squared length is
     // well above d^2 skip the square root (its result would exceed d)
@@ -2049,8 +2050,8 @@ __device__ __forceinline__ void mix_sand_pair(const float4 pi, const float4 xoi,
         len = sqrtf(dot3(ijx, ijy, ijz, ijx, ijy, ijz));
     }
     if (len > k.mix_d) return;
-    // j's old position and phase (w), read only for a contact
-    const float4 xoj = *xoj_at;
+    // j's old position and phase (w)
+    const float4 xoj = xoj_of();
     const int jph = (int)xoj.w;
     float sc;
     if (jph == 2) {
@@ -2186,16 +2187,29 @@ __global__ __launch_bounds__(kBlock) void k_mix_correct(const float4* __restrict
             ++nc;
         });
         if (nc <= kMixContacts) {
-            for (int u = 0; u < nc; ++u) {
-                const int j = contact[u * kBlock + (int)threadIdx.x];
-                mix_sand_pair(pi, xoi, xl[j], xold + j, k, dpx, dpy, dpz);
+            // four contacts at a time: their slots, positions and old
+            // positions requested together, then projected in list order
+            // (places past nc read slot i and are not projected)
+            for (int u = 0; u < nc; u += 4) {
+                int jj[4];
+                float4 pj[4], xoj[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) jj[q] = u + q < nc ? contact[(u + q) * kBlock + (int)threadIdx.x] : i;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    pj[q] = xl[jj[q]];
+                    xoj[q] = xold[jj[q]];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (u + q < nc) mix_sand_pair(pi, xoi, pj[q], [&] { return xoj[q]; }, k, dpx, dpy, dpz);
             }
         } else {
             // more than the buffer holds: the walk again, every neighbour
             for_each_neighbor<false>(i, n, key2, cs, masks, xp, xl, nullptr, k,
                                      [&](int j, const float4 pj, const float4) {
                 if (j == i) return;
-                mix_sand_pair(pi, xoi, pj, xold + j, k, dpx, dpy, dpz);
+                mix_sand_pair(pi, xoi, pj, [&] { return xold[j]; }, k, dpx, dpy, dpz);
             });
         }
         r.x = rclamp(pi.x + dpx, k.lo[0], k.hi[0]);
