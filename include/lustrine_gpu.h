@@ -189,7 +189,8 @@ int lus_pbf_set_build(LusGpu* ctx, int staged);
  * walks with one lane).  No reference counterpart (launch configuration). */
 int lus_pbf_set_lanes(LusGpu* ctx, int lanes);
 /* Sort tile of the step's counting sort: cells per block of its prefix scan
- * over the cell histogram, 1024, 2048 (default) or 4096.  Same sorted order.
+ * over the cell histogram, 1024, 2048 or 4096; 0 (default): 4096 for grids of
+ * at least 2^19 cells, else 2048.  Same sorted order.
  * Kept across scene inits; chosen per size bucket by the autotuner.  The scan
  * replaced is the inclusive prefix of Sorting::counting_sort
  * (src/neighbors/Sorting.cpp:11-33). */

@@ -414,8 +414,9 @@ class Gpu:
         """Lanes per particle in fused list walks: 0 automatic, 1 or 4."""
         _check(self._lib.lus_pbf_set_lanes(self._h, int(lanes)), "lus_pbf_set_lanes")
 
-    def pbf_set_sort_tile(self, cells=2048):
-        """Cells per block of the counting sort's prefix scan: 1024, 2048 or 4096."""
+    def pbf_set_sort_tile(self, cells=0):
+        """Cells per block of the counting sort's prefix scan: 1024, 2048 or 4096;
+        0: by the grid's cell count (4096 from 2**19 cells, else 2048)."""
         _check(self._lib.lus_pbf_set_sort_tile(self._h, int(cells)), "lus_pbf_set_sort_tile")
 
     def pbf_set_key_fusion(self, on=True):

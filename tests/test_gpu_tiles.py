@@ -143,15 +143,16 @@ def test_batch_with_larger_cells_matches_oracle(gpu_mod, oracle):
 
 
 def test_sort_tiles_equal_default(gpu_mod):
-    # the counting sort's prefix scan in tiles of 1024, 2048 or 4096 cells:
-    # the same cell starts, so the same steps bit for bit -- on the 32K scene
+    # the counting sort's prefix scan in tiles of 1024, 2048 or 4096 cells
+    # (and the default, 0: chosen by the cell count): the same cell starts,
+    # so the same steps bit for bit -- on the 32K scene
     # (18 to 71 tiles) and on 1,000 particles in a 420 x 200 x 420 box
     # (4,496,621 cells: 4,392 tiles of 1024, above the 4,096 tiles up to which
     # the down pass sums the tile sums itself)
     for cfg in (scenes.cfg0(), scenes.small(10, 10, 10, box=(420, 200, 420))):
         x0 = cfg.positions()
         res = []
-        for tile in (2048, 1024, 4096):
+        for tile in (2048, 1024, 4096, 0):
             with gpu_mod.Gpu(0) as g:
                 g.pbf_init(gpu_mod.pbf_params(cfg, exact=True), x0)
                 g.pbf_set_sort_tile(tile)
