@@ -1936,6 +1936,13 @@ int lus_sand_init(LusGpu* c, const LusSandParams* p, int n_sand, const float* po
     k = lus::SandK{};
     k.r = r;
     k.d = p->particleDiameter;
+    {
+        // (d (1 + 2^-19))^2 rounded up: sqrtf of anything above it exceeds d
+        const double far = (double)k.d * (1.0 + 0x1p-19);
+        float f2 = (float)(far * far);
+        if ((double)f2 < far * far) f2 = std::nextafter(f2, INFINITY);
+        k.d2_far = f2;
+    }
     volatile float hh = p->kernelRadius;
     k.h2 = hh * hh;
     k.w = 1.0f / p->mass;

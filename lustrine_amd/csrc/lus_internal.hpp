@@ -76,6 +76,7 @@ struct SandK {
     int variant;
     float bcoef;              // boundary_collision_coeff of simulate_sand_v3
     float dom[3];             // domainX/Y/Z: simulate_sand_v3's walls
+    float d2_far;             // a squared length above it has sqrtf > d: never a contact
 };
 
 // Records a failure for lus_gpu_last_error (lus_capi.cpp) and returns `code`.

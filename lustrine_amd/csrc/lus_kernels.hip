@@ -832,11 +832,12 @@ hipError_t launch_sand_nbr(const float4* xs, const int* key2, const int* sand_cs
     return hipGetLastError();
 }
 
-// The contact + friction projection of particle (pi, xoi) against neighbour jj
-// at pj (its old position xold[jj] is read for a sand contact only) into dp
-// (src/Simulate.cpp:240-300).
+// The contact + friction projection of particle (pi, xoi) against a neighbour
+// at pj into dp (src/Simulate.cpp:240-300); xoj_of() gives the neighbour's
+// old position, asked for only for a sand contact.
 // Returns whether the pair is in contact (within the diameter).
-__device__ __forceinline__ bool sand_pair(const float4 pi, const float4 xoi, const float4 pj, const float4* __restrict__ xold, int jj,
+template <class XoJ>
+__device__ __forceinline__ bool sand_pair(const float4 pi, const float4 xoi, const float4 pj, XoJ&& xoj_of,
                                           bool sand, const SandK& k, float& dpx, float& dpy, float& dpz) {
     float ijx = pi.x - pj.x, ijy = pi.y - pj.y, ijz = pi.z - pj.z;
     // |ij| == 0 -> ij = (0, 1e-5, 0) (src/Simulate.cpp:236-239; not in
@@ -869,7 +870,7 @@ __device__ __forceinline__ bool sand_pair(const float4 pi, const float4 xoi, con
     float xidz = (pi.z - tz) - xoi.z;
     float xjdx, xjdy, xjdz, nx, ny, nz;
     if (sand) {
-        const float4 xoj = xold[jj];
+        const float4 xoj = xoj_of();
         xjdx = (pj.x + tx) - xoj.x;
         xjdy = (pj.y + ty) - xoj.y;
         xjdz = (pj.z + tz) - xoj.z;
@@ -936,16 +937,18 @@ __device__ __forceinline__ void sand_walls_v3(const float4 q, const SandK& k, fl
 }
 
 // One Jacobi substep of contact + friction (src/Simulate.cpp:228-308).  The
-// list is walked four entries at a time: their indices and positions are all
-// requested before the four are projected in list order (a sand contact's old
-// position is loaded inside the projection, ~a quarter of the entries).
-// Places past the list end read slot i, in bounds and unused.
+// list is walked four entries at a time (their indices and positions all
+// requested before the four are tested; places past the list end read slot
+// i, in bounds and unused), collecting the possible contacts, which a second
+// loop projects in list order.
 //
 // simulate_sand_credits: a particle flagged 2 loses the flag at its first
 // contact (:463-469; the three rand() draws there are not used).
 // simulate_sand_v3: its four substeps all start from the same positions_tmp
 // and give the same bits, so one pass computes them; then the half-space
 // projections of positions_tmp are added and nothing is clamped (:566-616).
+constexpr int kSandContacts = 16;  // contacts of a lane held between its walk and its projection loop
+
 __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restrict__ A,
                                                           const float4* __restrict__ xold,
                                                           const float4* __restrict__ solid,
@@ -954,28 +957,64 @@ __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restric
                                                           SandK k, float4* __restrict__ B,
                                                           int* __restrict__ att,
                                                           const int* __restrict__ fail) {
+    // the lane's contacts in list order: contact[u * kBlock + thread]
+    __shared__ int contact[kSandContacts * kBlock];
     if (fail && *fail) return;  // the lists were not written (uniform)
     const int i = xcd_index();
     if (i >= n) return;
     const float4 pi = A[i];
     const float4 xoi = xold[i];
     float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
-    bool contact = false;
-    const int t1 = off[i + 1];
-    for (int t = off[i]; t < t1; t += 4) {
+    bool contact_any = false;
+    const int t0 = off[i], t1 = off[i + 1];
+    auto at = [&](int j) { return j < n ? A[j] : solid[j - n]; };
+    // Most neighbours lie beyond d, but over a wave some lane has a contact
+    // at almost every step of the walk: the walk only collects the
+    // neighbours whose squared length can give a length within d (d2_far),
+    // and a second loop, as long as the wave's largest contact count,
+    // projects them in list order (the same sums in the same order).
+    int nc = 0;
+    for (int t = t0; t < t1; t += 4) {
         int j[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) j[u] = (t + u < t1) ? idx[t + u] : i;
         float4 pj[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const bool sand = j[u] < n;
-            pj[u] = sand ? A[j[u]] : solid[j[u] - n];
-        }
+        for (int u = 0; u < 4; ++u) pj[u] = at(j[u]);
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (t + u < t1 && j[u] != i)
-                contact |= sand_pair(pi, xoi, pj[u], xold, j[u], j[u] < n, k, dpx, dpy, dpz);
+        for (int u = 0; u < 4; ++u) {
+            const float ijx = pi.x - pj[u].x, ijy = pi.y - pj[u].y, ijz = pi.z - pj[u].z;
+            if (t + u < t1 && j[u] != i && dot3(ijx, ijy, ijz, ijx, ijy, ijz) <= k.d2_far) {
+                if (nc < kSandContacts) contact[nc * kBlock + (int)threadIdx.x] = j[u];
+                ++nc;
+            }
+        }
+    }
+    if (nc <= kSandContacts) {
+        // four contacts at a time: slots, positions and old positions
+        // requested together (places past nc read slot i, not projected)
+        for (int u = 0; u < nc; u += 4) {
+            int jj[4];
+            float4 pj[4], xoj[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) jj[q] = u + q < nc ? contact[(u + q) * kBlock + (int)threadIdx.x] : i;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                pj[q] = at(jj[q]);
+                xoj[q] = xold[jj[q] < n ? jj[q] : i];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (u + q < nc)
+                    contact_any |= sand_pair(pi, xoi, pj[q], [&] { return xoj[q]; }, jj[q] < n, k, dpx, dpy, dpz);
+        }
+    } else {
+        // more than the buffer holds: every neighbour again
+        for (int t = t0; t < t1; ++t) {
+            const int j = idx[t];
+            if (j != i)
+                contact_any |= sand_pair(pi, xoi, at(j), [&] { return xold[j]; }, j < n, k, dpx, dpy, dpz);
+        }
     }
     float4 r;
     if (k.variant == 2) {
@@ -991,7 +1030,7 @@ __global__ __launch_bounds__(kBlock) void k_sand_substep(const float4* __restric
     }
     r.w = 0.0f;
     B[i] = r;
-    if (k.variant == 1 && contact) {
+    if (k.variant == 1 && contact_any) {
         const int a = att[i];
         if (a & 2) att[i] = a & ~2;
     }
