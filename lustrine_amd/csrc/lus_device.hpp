@@ -92,4 +92,16 @@ __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, fl
     return (t0 + t1) + t2;
 }
 
+// LDS float, addressed with 32-bit offsets (a generic pointer into LDS would
+// be indexed with 64-bit arithmetic)
+typedef __attribute__((address_space(3))) float lds_float;
+
+// LDS accesses of one wave complete, and are seen by all its lanes, before
+// the wave goes on (staging -> tests, tests -> the next staging)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 }  // namespace lus

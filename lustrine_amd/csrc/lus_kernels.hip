@@ -717,6 +717,48 @@ hipError_t launch_sand_permute(const int* tmp, const int* key, const int* cell_s
 // in (y, x, z) order, each cell holding its solids first (the cached static
 // lists, :293) then its sand slots; j = i is pushed twice.
 // Solid entries are stored as n + (solid sorted position).
+// Wave-staged: the slots of a wave lie in a few (y, x) cell columns, and
+// within a column their z windows ascend with the slot, so for each stencil
+// row the union of the wave's windows is one slot range per column run, in
+// the sand table and in the solid table.  The wave copies those ranges (x, y,
+// z of each slot) into LDS with coalesced loads and its lanes test their
+// candidates there, in the same order with the same arithmetic; a row whose
+// ranges exceed kSandStage slots or kSandSegs runs is read from global memory.
+constexpr int kSandStage = 256;
+constexpr int kSandSegs = 4;
+
+// The reference's neighbour test of the sand lists (squared length within h2)
+__device__ __forceinline__ bool sand_within(const float4 me, float qx, float qy, float qz, const SandK& k) {
+    const float dx = me.x - qx, dy = me.y - qy, dz = me.z - qz;
+    return dot3(dx, dy, dz, dx, dy, dz) <= k.h2;
+}
+
+// A slot's list entries as its scan finds them: part 1 (sand slots j < i,
+// pushed by j's own scan in the reference) from o1, part 2 (the own scan)
+// from o2; counted only when not filling
+struct SandHits {
+    int o1 = 0, o2 = 0, n1 = 0, n2 = 0;
+    __device__ __forceinline__ void solid(int js, int n, int fill, int* __restrict__ idx) {
+        if (fill) idx[o2 + n2] = n + js;
+        ++n2;
+    }
+    __device__ __forceinline__ void sand(int j, int i, int fill, int* __restrict__ idx, const SandK& k) {
+        if (j < i) {
+            if (i < k.sym_limit) {
+                if (fill) idx[o1 + n1] = j;
+                ++n1;
+            }
+        } else {
+            if (fill) idx[o2 + n2] = j;
+            ++n2;
+            if (j == i && i < k.sym_limit) {  // j = i is pushed twice
+                if (fill) idx[o2 + n2] = j;
+                ++n2;
+            }
+        }
+    }
+};
+
 __global__ __launch_bounds__(kBlock) void k_sand_nbr(const float4* __restrict__ xs,
                                                       const int* __restrict__ key2,
                                                       const int* __restrict__ scs, int n,
@@ -727,19 +769,27 @@ __global__ __launch_bounds__(kBlock) void k_sand_nbr(const float4* __restrict__ 
                                                       int* __restrict__ cnt1,
                                                       int* __restrict__ idx, int fill,
                                                       long cap, int* __restrict__ fail) {
-    const int i = xcd_index();
-    if (i >= n) return;
+    // per wave: the staged sand and solid ranges of a row
+    __shared__ float stage[kBlock / 64][2][3 * kSandStage];
     // a fixed index buffer (fail non-null): a total past its capacity (or
-    // past the int range) writes nothing and marks the step failed
+    // past the int range) writes nothing and marks the step failed (uniform)
     if (fill && fail) {
         const int tot = off[n];
         if (tot < 0 || (long)tot > cap) {
-            if (i == 0) *fail = 1;
+            if (xcd_index() == 0) *fail = 1;
             return;
         }
     }
-    const float4 me = xs[i];
-    const int c = key2[i];
+    const int i = xcd_index();
+    const int lane = (int)(threadIdx.x & 63);
+    // every lane of the wave takes part in the staging; a lane past n has no
+    // rows and writes nothing
+    const bool act = i < n;
+    const int ii = act ? i : n - 1;
+    lds_float* __restrict__ SS = (lds_float*)stage[threadIdx.x >> 6][0];
+    lds_float* __restrict__ SO = (lds_float*)stage[threadIdx.x >> 6][1];
+    const float4 me = xs[ii];
+    const int c = key2[ii];
     const int yy = c / k.gXZ;
     const int tm = c - yy * k.gXZ;
     const int zz = tm % k.gZ;
@@ -751,76 +801,141 @@ __global__ __launch_bounds__(kBlock) void k_sand_nbr(const float4* __restrict__ 
     if (xx + xu >= k.gX) xu = 0;
     if (zz + zl < 0) zl = 0;
     if (zz + zu >= k.gZ) zu = 0;
-    int o1 = fill ? off[i] : 0;            // part 1: sand k < i
-    int o2 = fill ? off[i] + cnt1[i] : 0;  // part 2: own scan
-    int n1 = 0, n2 = 0;
-    for (int y = yl; y <= yu; ++y) {
-        for (int x = xl; x <= xu; ++x) {
-            // the row's z cells are consecutive: their 2 to 4 boundaries in
-            // both cell tables are requested together
-            const int row = (yy + y) * k.gXZ + (xx + x) * k.gZ + zz;
-            int sb[4], ob[4];
+    const int nz = zu - zl + 1;  // z cells of a row (1 to 3)
+    SandHits h;
+    h.o1 = (fill && act) ? off[i] : 0;            // part 1: sand k < i
+    h.o2 = (fill && act) ? off[i] + cnt1[i] : 0;  // part 2: own scan
+    const unsigned long long below = (1ull << lane) - 1ull;  // lanes before this one
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        const int y = r / 3 - 1, x = r % 3 - 1;
+        const bool valid = act && y >= yl && y <= yu && x >= xl && x <= xu;
+        // the row's z cells are consecutive: their 2 to 4 boundaries in
+        // both cell tables are requested together
+        const int row = valid ? (yy + y) * k.gXZ + (xx + x) * k.gZ + zz : 0;
+        int sb[4], ob[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const bool in = zl + q <= zu + 1;
-                sb[q] = in ? scs[row + zl + q] : 0;
-                ob[q] = (in && solid) ? ocs[row + zl + q] : 0;
+        for (int q = 0; q < 4; ++q) {
+            const bool in = valid && q <= nz;
+            sb[q] = in ? scs[row + zl + q] : 0;
+            ob[q] = (in && solid) ? ocs[row + zl + q] : 0;
+        }
+        const int s_hi = nz == 1 ? sb[1] : (nz == 2 ? sb[2] : sb[3]);
+        const int o_hi = nz == 1 ? ob[1] : (nz == 2 ? ob[2] : ob[3]);
+        // runs of lanes whose row lies in one cell column (their windows
+        // ascend with the slot)
+        const int col = valid ? (yy + y) * k.gX + (xx + x) : -1;
+        const int pcol = __shfl_up(col, 1, 64);
+        const int ncol = __shfl_down(col, 1, 64);
+        const unsigned long long fm = __ballot(valid && (lane == 0 || pcol != col));
+        const unsigned long long lm = __ballot(valid && (lane == 63 || ncol != col));
+        const int nseg = __popcll(fm);
+        if (nseg == 0) continue;  // uniform: no lane has this row
+        // the runs' sand and solid ranges, packed in LDS at offsets s_off / o_off
+        int s_start[kSandSegs], s_off[kSandSegs + 1], o_start[kSandSegs], o_off[kSandSegs + 1];
+        int tot_s = 0, tot_o = 0;
+        {
+            unsigned long long fq = fm, lq = lm;
+#pragma unroll
+            for (int g = 0; g < kSandSegs; ++g) {
+                s_off[g] = tot_s;
+                o_off[g] = tot_o;
+                s_start[g] = 0;
+                o_start[g] = 0;
+                if (g < nseg) {
+                    const int lf = __ffsll((long long)fq) - 1, ll = __ffsll((long long)lq) - 1;
+                    fq &= fq - 1ull;
+                    lq &= lq - 1ull;
+                    s_start[g] = __builtin_amdgcn_readlane(sb[0], lf);
+                    tot_s += __builtin_amdgcn_readlane(s_hi, ll) - s_start[g];
+                    o_start[g] = __builtin_amdgcn_readlane(ob[0], lf);
+                    tot_o += __builtin_amdgcn_readlane(o_hi, ll) - o_start[g];
+                }
             }
+            s_off[kSandSegs] = tot_s;
+            o_off[kSandSegs] = tot_o;
+        }
+        // uniform
+        const bool staged = nseg <= kSandSegs && tot_s <= kSandStage && tot_o <= kSandStage;
+        if (staged) {
+            for (int q = lane; q < tot_s; q += 64) {
+                int j = q - s_off[0] + s_start[0];
+#pragma unroll
+                for (int g = 1; g < kSandSegs; ++g)
+                    if (q >= s_off[g]) j = q - s_off[g] + s_start[g];
+                const float4 p = xs[j];
+                SS[3 * q] = p.x;
+                SS[3 * q + 1] = p.y;
+                SS[3 * q + 2] = p.z;
+            }
+            for (int q = lane; q < tot_o; q += 64) {
+                int j = q - o_off[0] + o_start[0];
+#pragma unroll
+                for (int g = 1; g < kSandSegs; ++g)
+                    if (q >= o_off[g]) j = q - o_off[g] + o_start[g];
+                const float4 p = solid[j];
+                SO[3 * q] = p.x;
+                SO[3 * q + 1] = p.y;
+                SO[3 * q + 2] = p.z;
+            }
+            wave_lds_sync();
+            // this lane's run: LDS place of slot j is j + ls (sand), j + lo (solid)
+            const int seg = __popcll(fm & (below | (1ull << lane))) - 1;
+            int ls = s_off[0] - s_start[0], lo = o_off[0] - o_start[0];
+#pragma unroll
+            for (int g = 1; g < kSandSegs; ++g)
+                if (seg == g) {
+                    ls = s_off[g] - s_start[g];
+                    lo = o_off[g] - o_start[g];
+                }
+            if (valid) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    if (q >= nz) break;
+                    for (int js = ob[q]; js < ob[q + 1]; ++js) {
+                        const lds_float* __restrict__ p = SO + 3 * (js + lo);
+                        if (sand_within(me, p[0], p[1], p[2], k)) h.solid(js, n, fill, idx);
+                    }
+                    for (int j = sb[q]; j < sb[q + 1]; ++j) {
+                        const lds_float* __restrict__ p = SS + 3 * (j + ls);
+                        if (sand_within(me, p[0], p[1], p[2], k)) h.sand(j, i, fill, idx, k);
+                    }
+                }
+            }
+            wave_lds_sync();  // the next row's staging overwrites this one's
+        } else if (valid) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                if (zl + q > zu) break;
+                if (q >= nz) break;
                 // candidates four at a time: the four loads are requested
                 // together, then tested in slot order (the last candidate of
                 // the cell stands in for places past its end)
                 if (solid) {
                     const int je = ob[q + 1];
                     for (int j0 = ob[q]; j0 < je; j0 += 4) {
-                        float4 q[4];
+                        float4 p4[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) q[u] = solid[j0 + u < je ? j0 + u : je - 1];
+                        for (int u = 0; u < 4; ++u) p4[u] = solid[j0 + u < je ? j0 + u : je - 1];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const float dx = me.x - q[u].x, dy = me.y - q[u].y, dz = me.z - q[u].z;
-                            if (j0 + u < je && dot3(dx, dy, dz, dx, dy, dz) <= k.h2) {
-                                if (fill) idx[o2 + n2] = n + j0 + u;
-                                ++n2;
-                            }
-                        }
+                        for (int u = 0; u < 4; ++u)
+                            if (j0 + u < je && sand_within(me, p4[u].x, p4[u].y, p4[u].z, k)) h.solid(j0 + u, n, fill, idx);
                     }
                 }
                 const int je = sb[q + 1];
                 for (int j0 = sb[q]; j0 < je; j0 += 4) {
-                    float4 q[4];
+                    float4 p4[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) q[u] = xs[j0 + u < je ? j0 + u : je - 1];
+                    for (int u = 0; u < 4; ++u) p4[u] = xs[j0 + u < je ? j0 + u : je - 1];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int j = j0 + u;
-                        const float dx = me.x - q[u].x, dy = me.y - q[u].y, dz = me.z - q[u].z;
-                        if (j < je && dot3(dx, dy, dz, dx, dy, dz) <= k.h2) {
-                            if (j < i) {
-                                // pushed into slot i by slot j's scan
-                                if (i < k.sym_limit) {
-                                    if (fill) idx[o1 + n1] = j;
-                                    ++n1;
-                                }
-                            } else {
-                                if (fill) idx[o2 + n2] = j;
-                                ++n2;
-                                if (j == i && i < k.sym_limit) {
-                                    if (fill) idx[o2 + n2] = j;
-                                    ++n2;
-                                }
-                            }
-                        }
-                    }
+                    for (int u = 0; u < 4; ++u)
+                        if (j0 + u < je && sand_within(me, p4[u].x, p4[u].y, p4[u].z, k)) h.sand(j0 + u, i, fill, idx, k);
                 }
             }
         }
     }
-    if (!fill) {
-        cnt[i] = n1 + n2;
-        cnt1[i] = n1;
+    if (!fill && act) {
+        cnt[i] = h.n1 + h.n2;
+        cnt1[i] = h.n1;
     }
 }
 

@@ -984,10 +984,6 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda(const float4* __restrict_
 constexpr int kStageCap = 256;
 constexpr int kStageSegs = 4;
 
-// LDS float, addressed with 32-bit offsets (a generic pointer into LDS would
-// be indexed with 64-bit arithmetic)
-typedef __attribute__((address_space(3))) float lds_float;
-
 // staged slot q (kS floats: x, y, z and, for kS = 4, w)
 template <int kS>
 __device__ __forceinline__ float4 staged_at(const lds_float* __restrict__ S, int q) {
@@ -1059,14 +1055,6 @@ __device__ __forceinline__ M test_row_staged(const float4 pi, int lo, int L, con
         const unsigned int h = hits32_staged<kS, kFast>(pi, b + 32 * kS, L - nl, k);
         return (M)h << 32 | (M)l;
     }
-}
-
-// LDS accesses of one wave complete, and are seen by all its lanes, before
-// the wave goes on (staging -> tests, tests -> next row's staging)
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // The mixed scene's lambda sums (fluid i): rho and the gradient over every
