@@ -733,6 +733,22 @@ __device__ __forceinline__ bool sand_within(const float4 me, float qx, float qy,
     return dot3(dx, dy, dz, dx, dy, dz) <= k.h2;
 }
 
+// Hit mask of the cnt (<= 32) staged candidates b[3 t .. 3 t + 2]: bit t set
+// when candidate t is within h, from the sign of h2 - r2 (r2 as sand_within
+// forms it; the rounded difference of two floats has the sign of their exact
+// difference and is +0 when they are equal), shifted in from the last
+// candidate to the first, so one funnel shift replaces compare and branch.
+__device__ __forceinline__ unsigned int sand_hits32(const float4 me, const lds_float* __restrict__ b, int cnt,
+                                                    float h2) {
+    unsigned int o = ~0u;  // outside bits; ones above bit cnt - 1
+    for (int t = cnt - 1; t >= 0; --t) {
+        const lds_float* __restrict__ p = b + 3 * t;
+        const float dx = me.x - p[0], dy = me.y - p[1], dz = me.z - p[2];
+        o = __builtin_amdgcn_alignbit(o, __float_as_uint(h2 - dot3(dx, dy, dz, dx, dy, dz)), 31u);
+    }
+    return ~o;
+}
+
 // A slot's list entries as its scan finds them: part 1 (sand slots j < i,
 // pushed by j's own scan in the reference) from o1, part 2 (the own scan)
 // from o2; counted only when not filling
@@ -892,13 +908,32 @@ __global__ __launch_bounds__(kBlock) void k_sand_nbr(const float4* __restrict__ 
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     if (q >= nz) break;
-                    for (int js = ob[q]; js < ob[q + 1]; ++js) {
-                        const lds_float* __restrict__ p = SO + 3 * (js + lo);
-                        if (sand_within(me, p[0], p[1], p[2], k)) h.solid(js, n, fill, idx);
+                    // each cell's solids then sand slots, 32 candidates a mask
+                    for (int t0 = ob[q]; t0 < ob[q + 1]; t0 += 32) {
+                        const int cn = ob[q + 1] - t0 < 32 ? ob[q + 1] - t0 : 32;
+                        unsigned int m = sand_hits32(me, SO + 3 * (t0 + lo), cn, k.h2);
+                        if (!fill) {
+                            h.n2 += __popc(m);
+                            continue;
+                        }
+                        for (; m; m &= m - 1u) h.solid(t0 + __builtin_ctz(m), n, fill, idx);
                     }
-                    for (int j = sb[q]; j < sb[q + 1]; ++j) {
-                        const lds_float* __restrict__ p = SS + 3 * (j + ls);
-                        if (sand_within(me, p[0], p[1], p[2], k)) h.sand(j, i, fill, idx, k);
+                    for (int t0 = sb[q]; t0 < sb[q + 1]; t0 += 32) {
+                        const int cn = sb[q + 1] - t0 < 32 ? sb[q + 1] - t0 : 32;
+                        unsigned int m = sand_hits32(me, SS + 3 * (t0 + ls), cn, k.h2);
+                        if (!fill) {
+                            // part 1: slots below i (if i receives pushes); part
+                            // 2: the rest, and i itself once more
+                            const int below_i = i - t0;  // candidates of this mask below slot i
+                            const unsigned int lowm =
+                                below_i <= 0 ? 0u : (below_i >= 32 ? ~0u : (1u << below_i) - 1u);
+                            const bool sym = i < k.sym_limit;
+                            h.n1 += sym ? __popc(m & lowm) : 0;
+                            h.n2 += __popc(m & ~lowm);
+                            if (sym && below_i >= 0 && below_i < 32 && ((m >> below_i) & 1u)) ++h.n2;
+                            continue;
+                        }
+                        for (; m; m &= m - 1u) h.sand(t0 + __builtin_ctz(m), i, fill, idx, k);
                     }
                 }
             }
