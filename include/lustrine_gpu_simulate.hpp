@@ -8,7 +8,7 @@
  *   void simulate_sand_gpu(Lustrine::Simulation* s, float dt) {
  *       Lustrine::Bullet::simulate_bullet(&s->bullet_physics_simulation, dt,
  *                                         s->ptr_sand_start, s->ptr_sand_end);
- *       if (lus_adapter::simulate_sand(s, dt) != LUS_OK) {
+ *       if (lus_adapter::simulate_sand(s, dt) != LUS_OK) {   // a HIP failure: logged
  *           lus_adapter::release(s);
  *           Lustrine::simulate_sand(s, dt);                   // the reference, this step
  *       }

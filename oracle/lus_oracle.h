@@ -150,7 +150,7 @@ int orc_pbf_step_f64(const OrcPbfParams* p, int n, double* x, double* v, int* id
                      double dt, double* max_c, double* final_err);
 
 /* Density error of a state: max_i max(0, rho_i/rho0 - 1) with the grid built
- * from x itself (no predict).  Used to judge the learned warm start. */
+ * from x itself (no predict).  Used to measure the learned warm start. */
 double orc_pbf_density_error_f64(const OrcPbfParams* p, int n, const double* x);
 
 /* ---- mixed sand + fluid substep over static solid particles ---- */
