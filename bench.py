@@ -523,8 +523,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described, not built)", file=sys.stderr)
+        print(f"bench.py: the library runs a scene on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding a scene over several GPUs is not implemented)", file=sys.stderr)
         return 2
 
     from lustrine_amd import gpu as G
