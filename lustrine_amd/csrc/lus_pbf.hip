@@ -1192,9 +1192,11 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
             if (staged) {
                 for (int q = lane; q < tot; q += 64) {
                     int j = q - s_off[0] + s_start[0];
+                    if (nseg > 1) {  // uniform: a wave in one column run skips the selects
 #pragma unroll
-                    for (int s = 1; s < kSegs; ++s)
-                        if (q >= s_off[s]) j = q - s_off[s] + s_start[s];
+                        for (int s = 1; s < kSegs; ++s)
+                            if (q >= s_off[s]) j = q - s_off[s] + s_start[s];
+                    }
                     const float4 p = at4(xp, j);
                     S[kS * q] = p.x;
                     S[kS * q + 1] = p.y;
@@ -1203,11 +1205,13 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
                 }
                 wave_lds_sync();
                 // this lane's run and the LDS offset of its window
-                const int seg = __popcll(fm & (below | (1ull << lane))) - 1;
                 int lo = j0 - s_start[0] + s_off[0];
+                if (nseg > 1) {  // uniform
+                    const int seg = __popcll(fm & (below | (1ull << lane))) - 1;
 #pragma unroll
-                for (int s = 1; s < kSegs; ++s)
-                    if (seg == s) lo = j0 - s_start[s] + s_off[s];
+                    for (int s = 1; s < kSegs; ++s)
+                        if (seg == s) lo = j0 - s_start[s] + s_off[s];
+                }
                 if (L > 64) {
                     for (int t = 0; t < L; ++t) {
                         const float4 q = staged_at<kS>(S, lo + t);
