@@ -1192,10 +1192,12 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
             if (staged) {
                 for (int q = lane; q < tot; q += 64) {
                     int j = q - s_off[0] + s_start[0];
-                    // uniform: one run needs no selection, two runs one
-                    if (nseg == 2) {
+                    // uniform: one run needs no selection, two runs (the mixed
+                    // scene's usual fluid and sand runs) one compare; the plain
+                    // build keeps the general form (its registers: 80, 6 waves)
+                    if (kMixed && nseg == 2) {
                         if (q >= s_off[1]) j = q - s_off[1] + s_start[1];
-                    } else if (nseg > 2) {
+                    } else if (nseg > 1) {
 #pragma unroll
                         for (int s = 1; s < kSegs; ++s)
                             if (q >= s_off[s]) j = q - s_off[s] + s_start[s];
