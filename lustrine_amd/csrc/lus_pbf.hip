@@ -1091,13 +1091,10 @@ struct MixLambdaAcc {
 // lanes sum MixLambdaAcc, sand lanes only build their lists, solid lanes
 // write an empty list (no pass reads it); the staged copy keeps w (the phase).
 template <bool kFast, bool kMixed>
-__global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __restrict__ xp,
-                                                                 const int* __restrict__ key2,
-                                                                 const int* __restrict__ cs, int n, PbfK k,
-                                                                 float4* __restrict__ xl,
-                                                                 int* __restrict__ maxc,
-                                                                 unsigned long long* __restrict__ nb,
-                                                                 const int* __restrict__ gate) {
+__device__ __forceinline__ void lambda_build_ws(const float4* __restrict__ xp, const int* __restrict__ key2,
+                                                const int* __restrict__ cs, int n, const PbfK& k,
+                                                float4* __restrict__ xl, int* __restrict__ maxc,
+                                                unsigned long long* __restrict__ nb, const int* __restrict__ gate) {
     constexpr int kS = kMixed ? 4 : 3;  // floats per staged slot
     // runs per wave staged (a mixed wave holds a column's fluid, sand and
     // solid runs, so more of them)
@@ -1276,6 +1273,29 @@ __global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __
         }
     }
     if (!kMixed) block_max_store(Ci, maxc);
+}
+
+// the plain scene's first lambda pass (kMixed is false)
+template <bool kFast, bool kMixed>
+__global__ __launch_bounds__(kBlock) void k_pbf_lambda_build_ws(const float4* __restrict__ xp,
+                                                                 const int* __restrict__ key2,
+                                                                 const int* __restrict__ cs, int n, PbfK k,
+                                                                 float4* __restrict__ xl,
+                                                                 int* __restrict__ maxc,
+                                                                 unsigned long long* __restrict__ nb,
+                                                                 const int* __restrict__ gate) {
+    lambda_build_ws<kFast, kMixed>(xp, key2, cs, n, k, xl, maxc, nb, gate);
+}
+
+// The mixed scene's: held to 72 VGPRs (7 waves per SIMD, a few registers
+// spilled), measured faster there (125.7 -> 117.1 us at 400 K particles) and
+// slower for the plain scene's build, which keeps 80 (6 waves).
+template <bool kFast>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_mix_lambda_build_ws(
+    const float4* __restrict__ xp, const int* __restrict__ key2, const int* __restrict__ cs, int n, PbfK k,
+    float4* __restrict__ xl, int* __restrict__ maxc, unsigned long long* __restrict__ nb,
+    const int* __restrict__ gate) {
+    lambda_build_ws<kFast, true>(xp, key2, cs, n, k, xl, maxc, nb, gate);
 }
 
 template <bool kBuild, bool kFast>
@@ -2133,10 +2153,10 @@ hipError_t launch_mix_lambda(const float4* xa, int* key2, const int* cs, int n, 
     const dim3 g(grid_for(n)), b(kBlock);
     if (build && stage && xa == xp && n > 0 && k.ph_out) {
         if (k.exact)
-            launch_kernel((k_pbf_lambda_build_ws<false, true>), g, b, 0, s, xp, key2, cs, n, k, xl, (int*)nullptr,
+            launch_kernel((k_mix_lambda_build_ws<false>), g, b, 0, s, xp, key2, cs, n, k, xl, (int*)nullptr,
                           masks, (const int*)nullptr);
         else
-            launch_kernel((k_pbf_lambda_build_ws<true, true>), g, b, 0, s, xp, key2, cs, n, k, xl, (int*)nullptr,
+            launch_kernel((k_mix_lambda_build_ws<true>), g, b, 0, s, xp, key2, cs, n, k, xl, (int*)nullptr,
                           masks, (const int*)nullptr);
         return hipGetLastError();
     }
